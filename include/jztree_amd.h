@@ -1,0 +1,164 @@
+/*
+ * jztree_amd.h -- C ABI of the MI355X (gfx950) tree hot path.
+ *
+ * One entry point per XLA-FFI target that jztree binds for this path.  Each declaration cites
+ * the reference interface it replaces (paths relative to the jztree repository):
+ *   handler signature  src/jztree_cuda/generated/ffi_*.cu
+ *   host function      src/jztree_cuda/*.cuh
+ *   Python call site   src/jztree/*.py
+ *
+ * Conventions (mirroring the reference's FFI contract, see ffi_knn.cu:159-168):
+ *   - buffer sizes the FFI handlers infer from buffer dimensions are explicit arguments here;
+ *     interaction-list readers clamp every segment to the list size they are given, so an
+ *     overflowing list (reported by the caller afterwards, as the reference does) never makes a
+ *     kernel read past its buffer;
+ *   - every buffer is a device pointer owned by the caller; outputs are preallocated with the
+ *     shapes the Python layer computes; nothing is allocated inside an entry point;
+ *   - scratch space arrives as an explicit `tmp` buffer whose size is queried with the matching
+ *     *_tmp_bytes function (the reference passes tmp buffers as extra FFI outputs);
+ *   - all work is enqueued on `stream`; no entry point synchronises or reads device data on the
+ *     host, so a sequence of calls can be captured into a hipGraph;
+ *   - the return value is 0 on success; otherwise jzt_last_error() holds the message (the
+ *     reference returns std::string errors mapped to ffi::Error::Internal);
+ *   - positions are AoS (n, dim) arrays of `dtype` (JZT_F32 or JZT_F64); index outputs of the
+ *     sort are int32 for f32 and int64 for f64 (same_width_int, src/jztree/data.py:13-19).
+ */
+#ifndef JZTREE_AMD_H
+#define JZTREE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JZT_ABI_VERSION 1
+
+typedef struct ihipStream_t* jzt_stream_t; /* a hipStream_t */
+
+enum { JZT_F32 = 0, JZT_F64 = 1 };
+
+/* message of the last failed call on this host thread */
+const char* jzt_last_error(void);
+int jzt_abi_version(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* sort                                                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* PosZorderSort -- replaces ffi target "PosZorderSort" (generated/ffi_sort.cu:62-164, host
+ * sort.cuh:40-76, caller tree.py:75-93).  pos_out[i] = pos_in[id_out[i]] in z-order; pos_out may
+ * alias pos_in (the reference aliases {0:0}).  Order: exact Morton order of the float bits, ties by
+ * input index.  NaN rows last.  dim in [2, 3]. */
+size_t jzt_zsort_tmp_bytes(int64_t n, int dim, int dtype);
+int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out, void* tmp,
+                        size_t tmp_bytes, int64_t n, int dim, int dtype);
+
+/* SegmentSort -- replaces ffi target "SegmentSort" (generated/ffi_knn.cu:412-423, knn.cuh:700-722,
+ * caller knn.py:112-127).  Sorts (key, val) within segments spl[i]..spl[i+1] by (key, val). */
+int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const float* key, const int32_t* val,
+                     float* key_out, int32_t* val_out, int64_t size_segs, int64_t size_keys);
+
+/* ------------------------------------------------------------------------------------------ */
+/* tree                                                                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* FlagLeafBoundaries -- replaces ffi target "FlagLeafBoundaries" (generated/ffi_tree.cu:31-141,
+ * kernel tree.cuh:13-98, caller tree.py:189-262).  split_flags/lvl have size_part+1 entries;
+ * lvl_bound (2 int32) and npart (1 int32) are device scalars. */
+int jzt_flag_leaf_boundaries(jzt_stream_t stream, const void* posz, const uint8_t* ptype,
+                             const int32_t* lvl_bound, const int32_t* npart, int8_t* split_flags,
+                             int32_t* lvl, int64_t size_part, int dim, int dtype, int max_size,
+                             int scan_size, int num_types);
+
+/* FindNodeBoundaries -- replaces ffi target "FindNodeBoundaries" (generated/ffi_tree.cu:148-257,
+ * kernel tree.cuh:104-199, caller tree.py:264-284).  pos_in has size_nodes-1 rows; pos_boundary
+ * 2 rows; nleaves a device scalar; outputs have size_nodes entries. */
+int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in, const void* pos_boundary,
+                             const int32_t* nleaves, int32_t* nodes_levels, int32_t* nodes_lbound,
+                             int32_t* nodes_rbound, int64_t size_nodes, int dim, int dtype, int lvl_max,
+                             int lvl_invalid);
+
+/* GetNodeGeometry -- replaces ffi target "GetNodeGeometry" (generated/ffi_tree.cu:409-529, kernel
+ * tree.cuh:312-372, caller tree.py:286-329).  mode_flags: 1 level, 2 center, 4 extent (outputs not
+ * selected may be NULL).  nnodes a device scalar. */
+int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* lbound, const int32_t* rbound,
+                          const int32_t* nnodes, int32_t* level, void* center, void* extent,
+                          int64_t size_nodes, int64_t size_part, int dim, int dtype, int lvl_invalid,
+                          uint32_t mode_flags, int upper_extent);
+
+/* The JAX glue of the tree build, as device ops (no reference FFI counterpart: the reference runs
+ * these steps as XLA ops inside tree.py).
+ *
+ * jzt_where_flags: tree.py:257 `jnp.where(flags, size=alloc, fill_value=npart)`.  out[j] is the
+ * index of the j-th nonzero flag, padded with `fill` (or *fill_dev if non-NULL); *count (device,
+ * may be NULL) = number of nonzero flags. */
+size_t jzt_where_tmp_bytes(int64_t n);
+int jzt_where_flags(jzt_stream_t stream, const int8_t* flags, int64_t n, int32_t* out, int64_t size,
+                    int32_t fill, const int32_t* fill_dev, int32_t* count, void* tmp, size_t tmp_bytes);
+
+/* jzt_regularization_level: tree.py:55-68 + :160-187 find_regularization_level.  Histogram of
+ * `lvl` weighted by `weights` (NULL = 1) over [min_lvl, max_lvl], then the first level whose
+ * volume exceeds max_volume_fac times the mean volume below the min_percentile level, in FP32. */
+size_t jzt_regularization_tmp_bytes(int min_lvl, int max_lvl);
+int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl, const int32_t* weights, int64_t n,
+                             int32_t* out_level, int min_lvl, int max_lvl, float max_volume_fac,
+                             float min_percentile, void* tmp, size_t tmp_bytes);
+
+/* ------------------------------------------------------------------------------------------ */
+/* kNN                                                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* KnnNode2Node -- replaces ffi target "KnnNode2Node" (generated/ffi_knn.cu:191-370, host
+ * knn.cuh:602-694, caller knn.py:62-110).  nodes: (size_nodes, dim+1) of dtype, the last column
+ * holding the node level bit-cast (PosLvl.pos_lvl, data.py:231-239).  Outputs: node_rmax2
+ * (size_nodes), node_ilist_spl (size_nodes+1), node_ilist_isrc/_r2 (node_ilist_size).  Each node's
+ * list is sorted ascending by (r2, isrc).  *overflow semantics as the reference: spl[-1] may exceed
+ * node_ilist_size, the caller checks. */
+int jzt_knn_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                      const float* parent_ilist_r2, const int32_t* parent_spl, const void* nodes,
+                      const int32_t* nodes_npart, float* node_rmax2, int32_t* node_ilist_spl,
+                      int32_t* node_ilist_isrc, float* node_ilist_r2, int64_t size_parents, int64_t size_nodes,
+                      int64_t parent_ilist_size, int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
+
+/* KnnLeaf2Leaf -- replaces ffi target "KnnLeaf2Leaf" (generated/ffi_knn.cu:31-188, host
+ * knn.cuh:309-356, caller knn.py:28-60).  knn_rad (nq, k) of dtype, knn_id (nq, k) int32.
+ * Neighbours ascending by (canonical r2, id); id = z-index in xT, or src_ids[z-index] when src_ids
+ * is non-NULL (an extension: lets the caller report input indices directly). */
+size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype);
+int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
+                      const float* ilist_r2, int64_t ilist_size, const int32_t* splT, const void* xT,
+                      const int32_t* splQ, const void* xQ, const int32_t* src_ids, void* knn_rad, int32_t* knn_id,
+                      void* tmp, size_t tmp_bytes, int64_t size_leaves_query, int64_t size_part_query, int k,
+                      float boxsize, int dim, int dtype);
+
+/* ------------------------------------------------------------------------------------------ */
+/* friends-of-friends                                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+/* NodeToChildLabel -- replaces ffi target "NodeToChildLabel" (generated/ffi_fof.cu:31-124, kernel
+ * fof.cuh:42-71, caller fof.py:85-109).  node_igroup (size_child) is zero-filled first. */
+int jzt_node_to_child_label(jzt_stream_t stream, const int32_t* parent_igroup, const uint8_t* parent_is_local,
+                            const int32_t* parent_lvl, const int32_t* parent_spl, int32_t* node_igroup,
+                            int64_t size_parent, int64_t size_child, float r2link, int dim, int dtype);
+
+/* FofNode2Node -- replaces ffi target "FofNode2Node" (generated/ffi_fof.cu:127-274, host
+ * fof.cuh:210-284, caller fof.py:31-70).  node_igroup may alias node_igroup_in ({4:0}). */
+int jzt_fof_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist,
+                      const int32_t* parent_spl, const void* nodes, const int32_t* node_igroup_in,
+                      int32_t* node_igroup, int32_t* node_ilist_spl, int32_t* node_ilist, int64_t size_parent,
+                      int64_t size_node, int64_t parent_ilist_size, int64_t size_node_ilist, float r2link,
+                      float boxsize, int dim, int dtype);
+
+/* FofLeaf2Leaf -- replaces ffi target "FofLeaf2Leaf" (generated/ffi_fof.cu:277-406, host
+ * fof.cuh:349-379, caller fof.py:153-169).  part_igroup = min z-index of each particle's group. */
+int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist, int64_t ilist_size,
+                      const int32_t* spl, const void* pos, const int32_t* part_igroup_in, int32_t* part_igroup,
+                      int64_t size_leaves, int64_t size_part, float r2link, float boxsize, int dim, int dtype);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JZTREE_AMD_H */

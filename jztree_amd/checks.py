@@ -1,0 +1,44 @@
+"""Deferred device-side error checks (the role of jax_ext.raise_if, src/jztree/jax_ext.py:174-204).
+
+The reference raises from an io_callback when a device-side condition fires (allocation
+overflow).  Here a condition is a device tensor; it is recorded without synchronising and all
+recorded conditions are read back with one copy when the public entry point returns
+(``flush``), so the hot path has at most one host synchronisation per call and none inside.
+"""
+from __future__ import annotations
+
+import threading
+
+import torch
+
+_tls = threading.local()
+
+
+def _pending():
+    if not hasattr(_tls, "pending"):
+        _tls.pending = []
+    return _tls.pending
+
+
+def raise_if(cond: torch.Tensor, msg: str, **values):
+    """Record `cond` (bool device tensor); `msg` is formatted with `values` (device scalars)."""
+    _pending().append((cond.reshape(()), msg, values))
+
+
+def clear():
+    _pending().clear()
+
+
+def flush():
+    """Raise RuntimeError for the first recorded condition that holds; then forget all."""
+    pend = _pending()
+    if not pend:
+        return
+    conds = torch.stack([c.to(torch.bool) for c, _, _ in pend]).cpu()
+    bad = [i for i in range(len(pend)) if bool(conds[i])]
+    items = list(pend)
+    pend.clear()
+    if bad:
+        _, msg, values = items[bad[0]]
+        vals = {k: (v.item() if torch.is_tensor(v) else v) for k, v in values.items()}
+        raise RuntimeError(msg.format(**vals))

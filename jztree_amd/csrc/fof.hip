@@ -1,0 +1,285 @@
+// Friends-of-friends (replaces jztree's fof.cuh).
+//
+// Union-find over int32 labels with parent[x] <= x.  Hooking is lock-free with atomicMin: to join
+// roots hi > lo, atomicMin(&parent[hi], lo) returns the old parent; if hi was no longer a root the
+// displaced link (old) is re-joined with lo, so no link is lost (the reference retries a CAS,
+// fof.cuh:22-36; both leave the group minimum as the root).  Pointer reads are coherent relaxed
+// loads (L2), so a retry always observes progress.  Contraction = pointer jumping to the root.
+//
+//  NodeToChildLabel  fof.cuh:42-71     children of an unlinked, too-large node label themselves,
+//                                      all others inherit the first child of the parent's root
+//  FofNode2Node      fof.cuh:73-284    link guaranteed node pairs, contract, count + scan + insert
+//                                      the uncertain pairs into the child-level interaction list
+//  FofLeaf2Leaf      fof.cuh:290-379   link particle pairs with canonical r2 <= r2link, contract
+//
+// Node-level decisions use the same 2^-16 relative slack as the kNN bounds: a pair is only linked
+// as "guaranteed" if every particle pair certainly passes the particle predicate, and only dropped
+// if none can.  Groups are therefore exactly the connected components of the particle predicate.
+// One wave64 workgroup per parent node / query leaf; source nodes and particles staged in LDS.
+#include "util.h"
+#include "../../include/jztree_amd.h"
+
+namespace jzt {
+namespace {
+
+constexpr float kSlackHi = 1.0f + kBoundSlack;  // inflate for "may link" tests
+constexpr float kSlackLo = 1.0f - kBoundSlack;  // deflate for "must link" tests
+
+__device__ __forceinline__ int uf_parent(const int32_t* p, int x) { return ld_acq(p + x); }
+
+__device__ __forceinline__ int uf_find(const int32_t* p, int x) {
+  while (true) {
+    int q = uf_parent(p, x);
+    if (q == x) return x;
+    x = q;
+  }
+}
+
+__device__ __forceinline__ void uf_link(int32_t* p, int a, int b) {
+  a = uf_find(p, a);
+  b = uf_find(p, b);
+  while (a != b) {
+    int hi = a > b ? a : b, lo = a > b ? b : a;
+    int old = atomicMin(p + hi, lo);
+    if (old == hi || old == lo) return;  // hooked (or already joined to lo)
+    // hi had been hooked to `old` meanwhile; join old's tree with lo and go on
+    a = uf_find(p, old);
+    b = uf_find(p, lo);
+  }
+}
+
+__global__ void k_contract(int32_t* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = uf_find(p, (int)i);
+}
+
+template <int D, typename T> struct NodeX {
+  Vec<D, T> c;
+  Vec<D, T> h;  // half extent
+};
+
+template <int D, typename T>
+__device__ __forceinline__ NodeX<D, T> load_nodex(const T* __restrict__ nodes, int64_t i) {
+  NodeX<D, T> n;
+#pragma unroll
+  for (int d = 0; d < D; d++) n.c[d] = nodes[i * (D + 1) + d];
+  T lv = nodes[i * (D + 1) + D];
+  int level;
+  if constexpr (sizeof(T) == 4) level = __float_as_int(lv);
+  else level = (int)__double_as_longlong(lv);
+  n.h = lvl_half_ext<D, T>(level);
+  return n;
+}
+
+template <int D, typename T> __device__ __forceinline__ T norm2(const Vec<D, T>& v) {
+  T r = v[0] * v[0];
+#pragma unroll
+  for (int d = 1; d < D; d++) r = fmaT(v[d], v[d], r);
+  return r;
+}
+
+// ------------------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(64) void k_node_to_child(const int32_t* __restrict__ pig, const uint8_t* __restrict__ local,
+                                                      const int32_t* __restrict__ plvl, const int32_t* __restrict__ pspl,
+                                                      int32_t* __restrict__ cig, int64_t size_parent, float r2link) {
+  const int64_t node = blockIdx.x;
+  if (node >= size_parent || (local && !local[node])) return;
+  const int root = pig[node];
+  if (root >= size_parent || (local && !local[root])) return;
+  const T L2 = norm2<D, T>(lvl_ext<D, T>(plvl[node]));
+  const bool self = (root == node) && (L2 * (T)kSlackHi > (T)r2link);
+  const int c0 = pspl[node], c1 = pspl[node + 1];
+  const int lab = pspl[root];
+  for (int i = c0 + threadIdx.x; i < c1; i += 64) cig[i] = self ? i : lab;
+}
+
+// PASS 0: link guaranteed pairs; PASS 1: count uncertain pairs; PASS 2: insert them
+template <int PASS, int D, typename T>
+__global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil,
+                                                 const int32_t* __restrict__ spl, const T* __restrict__ nodes,
+                                                 const int32_t* __restrict__ il_spl, int32_t* ig, int32_t* __restrict__ count,
+                                                 int32_t* __restrict__ il, float r2link, T boxsize, int64_t il_size,
+                                                 int64_t pil_size) {
+  __shared__ NodeX<D, T> s_n[64];
+  __shared__ int s_g[64];
+  const int lane = threadIdx.x;
+  const int pq = blockIdx.x;
+  const int q0 = spl[pq], q1 = spl[pq + 1];
+  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+  const T rl_hi = (T)r2link * (T)kSlackHi, rl_lo = (T)r2link * (T)kSlackLo;
+  for (int qoff = q0; qoff < q1; qoff += 64) {
+    const int iq = min(qoff + lane, q1 - 1);
+    const bool valid = qoff + lane < q1;
+    NodeX<D, T> nq = load_nodex<D, T>(nodes, iq);
+    const int gq = ig[iq];
+    int64_t off = (PASS == 2) ? (int64_t)il_spl[iq] : 0;
+    int cnt = 0;
+    for (int e = e0; e < e1; e++) {
+      const int pt = pil[e];
+      if (pt < pq) continue;  // each parent pair once
+      const int t0 = spl[pt], t1 = spl[pt + 1];
+      for (int toff = t0; toff < t1; toff += 64) {
+        const int it = toff + lane;
+        if (it < t1) {
+          s_n[lane] = load_nodex<D, T>(nodes, it);
+          s_g[lane] = ig[it];
+        }
+        __syncthreads();
+        const int nl = min(t1 - toff, 64);
+        if (valid) {
+          for (int j = 0; j < nl; j++) {
+            const int jt = toff + j;
+            if (gq == s_g[j] && iq != jt) continue;  // already linked
+            NodeX<D, T> nt = s_n[j];
+            Vec<D, T> w;
+#pragma unroll
+            for (int d = 0; d < D; d++) w[d] = nq.h[d] + nt.h[d];
+            T r2max = maxdist2<D, T>(nt.c, nq.c, w, boxsize);
+            T r2min = mindist2<D, T>(nt.c, nq.c, w, boxsize);
+            T L2 = norm2<D, T>(nt.h) * T(4);
+            if (max(r2max, L2) <= rl_lo) {
+              if (PASS == 0) uf_link(ig, iq, jt);
+            } else if (PASS >= 1 && r2min <= rl_hi) {
+              if (PASS == 2 && off + cnt < il_size) il[off + cnt] = jt;
+              cnt++;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (PASS == 1 && valid) count[iq] = cnt;
+  }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
+                                                 const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
+                                                 float r2link, T boxsize, int64_t il_size) {
+  __shared__ Vec<D, T> s_x[64];
+  __shared__ int s_g[64];
+  const int lane = threadIdx.x;
+  const int lq = blockIdx.x;
+  const int q0 = spl[lq], q1 = spl[lq + 1];
+  const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
+  for (int qoff = q0; qoff < q1; qoff += 64) {
+    const int iq = min(qoff + lane, q1 - 1);
+    const bool valid = qoff + lane < q1;
+    Vec<D, T> xq = load_vec<D, T>(pos, iq);
+    const int gq = ig[iq];
+    for (int e = e0; e < e1; e++) {
+      const int lt = il[e];
+      if (lt < lq) continue;  // each leaf pair once
+      const int t0 = spl[lt], t1 = spl[lt + 1];
+      for (int toff = t0; toff < t1; toff += 64) {
+        const int it = toff + lane;
+        if (it < t1) {
+          s_x[lane] = load_vec<D, T>(pos, it);
+          s_g[lane] = ig[it];
+        }
+        __syncthreads();
+        const int nl = min(t1 - toff, 64);
+        if (valid) {
+          for (int j = 0; j < nl; j++) {
+            T r2 = dist2<D, T>(xq, s_x[j], boxsize);
+            if (gq != s_g[j] && r2 <= (T)r2link) uf_link(ig, iq, toff + j);
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+template <int D, typename T>
+int node2child_impl(hipStream_t s, const int32_t* pig, const uint8_t* local, const int32_t* plvl, const int32_t* pspl,
+                    int32_t* cig, int64_t size_parent, int64_t size_child, float r2link) {
+  if (fill_u32(s, cig, 0u, size_child)) return 1;
+  if (size_parent <= 0) return 0;
+  hipLaunchKernelGGL((k_node_to_child<D, T>), dim3((unsigned)size_parent), dim3(64), 0, s, pig, local, plvl, pspl, cig,
+                     size_parent, r2link);
+  return check_launch("NodeToChildLabel");
+}
+
+template <int D, typename T>
+int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, const int32_t* pspl, const void* nodes,
+                  const int32_t* ig_in, int32_t* ig, int32_t* il_spl, int32_t* il, int64_t size_parent, int64_t size_node,
+                  int64_t pil_size, int64_t il_size, float r2link, float boxsize) {
+  const T* nd = (const T*)nodes;
+  if (fill_u32(s, il_spl, 0u, size_node + 1)) return 1;
+  if (fill_u32(s, il, 0u, il_size)) return 1;
+  if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_node * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail("FofNode2Node: copy failed");
+  if (size_parent > 0) {
+    dim3 g((unsigned)size_parent), b(64);
+    hipLaunchKernelGGL((k_fof_node<0, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
+                       (int32_t*)nullptr, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size);
+    hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_node, 256)), dim3(256), 0, s, ig, size_node);
+    hipLaunchKernelGGL((k_fof_node<1, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
+                       il_spl + 1, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size);
+    if (check_launch("FofNode2Node link/count")) return 1;
+  }
+  if (scan_tmp_words(size_node) > il_size) return fail("FofNode2Node: interaction list too small for scan temp");
+  if (scan_i32(s, il_spl + 1, il_spl + 1, size_node, true, il)) return 1;
+  if (fill_u32(s, il, 0u, scan_tmp_words(size_node))) return 1;
+  if (size_parent > 0) {
+    hipLaunchKernelGGL((k_fof_node<2, D, T>), dim3((unsigned)size_parent), dim3(64), 0, s, pil_spl, pil, pspl, nd,
+                       il_spl, ig, (int32_t*)nullptr, il, r2link, (T)boxsize, il_size, pil_size);
+  }
+  return check_launch("FofNode2Node insert");
+}
+
+template <int D, typename T>
+int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64_t il_size, const int32_t* spl,
+                  const void* pos, const int32_t* ig_in, int32_t* ig, int64_t size_leaves, int64_t size_part,
+                  float r2link, float boxsize) {
+  if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_part * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return fail("FofLeaf2Leaf: copy failed");
+  if (size_leaves > 0)
+    hipLaunchKernelGGL((k_fof_leaf<D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl, (const T*)pos, ig,
+                       r2link, (T)boxsize, il_size);
+  if (size_part > 0)
+    hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);
+  return check_launch("FofLeaf2Leaf");
+}
+
+}  // namespace
+}  // namespace jzt
+
+using namespace jzt;
+
+#define JZT_DISPATCH2(fn, ...)                                            \
+  do {                                                                    \
+    if (dtype == JZT_F32 && dim == 3) return fn<3, float>(__VA_ARGS__);   \
+    if (dtype == JZT_F32 && dim == 2) return fn<2, float>(__VA_ARGS__);   \
+    if (dtype == JZT_F64 && dim == 3) return fn<3, double>(__VA_ARGS__);  \
+    if (dtype == JZT_F64 && dim == 2) return fn<2, double>(__VA_ARGS__);  \
+    return fail(std::string(#fn) + ": unsupported dim/dtype");            \
+  } while (0)
+
+extern "C" int jzt_node_to_child_label(jzt_stream_t stream, const int32_t* parent_igroup, const uint8_t* parent_is_local,
+                                       const int32_t* parent_lvl, const int32_t* parent_spl, int32_t* node_igroup,
+                                       int64_t size_parent, int64_t size_child, float r2link, int dim, int dtype) {
+  JZT_DISPATCH2(node2child_impl, (hipStream_t)stream, parent_igroup, parent_is_local, parent_lvl, parent_spl, node_igroup,
+                size_parent, size_child, r2link);
+}
+
+extern "C" int jzt_fof_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist,
+                                 const int32_t* parent_spl, const void* nodes, const int32_t* node_igroup_in,
+                                 int32_t* node_igroup, int32_t* node_ilist_spl, int32_t* node_ilist, int64_t size_parent,
+                                 int64_t size_node, int64_t parent_ilist_size, int64_t size_node_ilist, float r2link,
+                                 float boxsize, int dim, int dtype) {
+  JZT_DISPATCH2(fof_node_impl, (hipStream_t)stream, parent_ilist_spl, parent_ilist, parent_spl, nodes, node_igroup_in,
+                node_igroup, node_ilist_spl, node_ilist, size_parent, size_node, parent_ilist_size, size_node_ilist,
+                r2link, boxsize);
+}
+
+extern "C" int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist,
+                                 int64_t ilist_size, const int32_t* spl, const void* pos, const int32_t* part_igroup_in,
+                                 int32_t* part_igroup, int64_t size_leaves, int64_t size_part, float r2link,
+                                 float boxsize, int dim, int dtype) {
+  JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
+                size_leaves, size_part, r2link, boxsize);
+}

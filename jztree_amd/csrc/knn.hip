@@ -1,0 +1,565 @@
+// Exact k-nearest-neighbour search (replaces jztree's knn.cuh).
+//
+// KnnNode2Node (knn.cuh:435-694): for every child node Q of a parent, walk the parent's
+//   r2-sorted interaction list and
+//     FindRmax     accumulate (maxdist2, npart) bins until >= k particles are guaranteed; the
+//                  bin radius (times 1 + 2^-16) bounds the k-th neighbour of every point in Q
+//     Count/Insert every child T of the listed parents with mindist2(Q,T) <= rmax2(Q)
+//     SegmentSort  each node's list ascending by (r2, src)
+//   One wave64 workgroup per parent; its children sit in lanes; source children are staged in LDS.
+//
+// KnnLeaf2Leaf (knn.cuh:203-356): one wave64 per query leaf, one query per lane.  The leaf's
+//   interaction list is walked in ascending r2 and stops once no lane could still accept
+//   (r2T > kth*(1+2^-16) on all lanes).  Source particles are read at wave-uniform addresses
+//   (scalar loads, broadcast to all lanes); each lane keeps its k best (r2, id) pairs sorted in
+//   VGPRs as 64-bit keys (r2 bits << 32 | id), inserted with a static compare/select network.
+//   k > 64 runs ceil(k/64) passes, each resuming after the previous pass's last key.
+//
+// Order: ascending by (canonical FP32 r2, reported id).  The reference inserts with strict '<' in
+// visit order (knn.cuh:84-102), so its tie order depends on the traversal; here it does not.
+#include "util.h"
+#include "../../include/jztree_amd.h"
+
+namespace jzt {
+namespace {
+
+constexpr float kRmaxSlack = 1.0f + kBoundSlack;
+
+template <int D, typename T> struct NodeT {
+  Vec<D, T> cent;
+  int level;
+};
+
+template <int D, typename T>
+__device__ __forceinline__ NodeT<D, T> load_node(const T* __restrict__ nodes, int64_t i) {
+  NodeT<D, T> n;
+#pragma unroll
+  for (int d = 0; d < D; d++) n.cent[d] = nodes[i * (D + 1) + d];
+  T lv = nodes[i * (D + 1) + D];
+  if constexpr (sizeof(T) == 4) n.level = __float_as_int(lv);
+  else n.level = (int)__double_as_longlong(lv);
+  return n;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ Vec<D, T> add(const Vec<D, T>& a, const Vec<D, T>& b) {
+  Vec<D, T> r;
+#pragma unroll
+  for (int d = 0; d < D; d++) r[d] = a[d] + b[d];
+  return r;
+}
+
+// ------------------------------------------------------------------------------------------------
+// per-lane bins of (r2, count) for the guaranteed radius (knn.cuh:106-191, same semantics)
+// ------------------------------------------------------------------------------------------------
+template <int KB> struct RmaxBins {
+  float r2s[KB];
+  int cts[KB];
+  int total, target;
+  __device__ __forceinline__ RmaxBins(int tgt) {
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+      r2s[i] = INFINITY;
+      cts[i] = 0;
+    }
+    total = 0;
+    target = tgt;
+  }
+  __device__ __forceinline__ float max_r2(int count) const {
+    bool active = true;
+    int sum = 0;
+    float r2 = INFINITY;
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+      sum += cts[i];
+      r2 = (active && sum >= count) ? r2s[i] : r2;
+      active = sum < count;
+    }
+    return r2;
+  }
+  __device__ __forceinline__ void insert_add(float r2, int num) {
+    total += num;
+    bool active = true;
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+      bool a = active && (r2s[i] >= r2);
+      cts[i] = a ? cts[i] + num : cts[i];
+      active = active && !a;
+    }
+  }
+  __device__ __forceinline__ void insert_shift(float r2, int num) {
+    total = total + num - cts[KB - 1];
+    bool active = true;
+#pragma unroll
+    for (int i = KB - 2; i >= 0; i--) {
+      bool take = active && (r2 < r2s[i]);
+      r2s[i + 1] = active ? (take ? r2s[i] : r2) : r2s[i + 1];
+      cts[i + 1] = active ? (take ? cts[i] : num) : cts[i + 1];
+      active = take;
+    }
+    r2s[0] = active ? r2 : r2s[0];
+    cts[0] = active ? num : cts[0];
+  }
+  __device__ __forceinline__ void consider(float r2, int num) {
+    if (r2 > r2s[KB - 1]) return;
+    bool need_add = (cts[KB - 1] > 0) && (total + num - cts[KB - 1] < target);
+    need_add |= r2 == r2s[KB - 1];
+    if (need_add) insert_add(r2, num);
+    else insert_shift(r2, num);
+  }
+};
+
+template <int KB, int D, typename T>
+__global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src,
+                                                  const float* __restrict__ pil_r2, const int32_t* __restrict__ parent_spl,
+                                                  const T* __restrict__ nodes, const int32_t* __restrict__ npart,
+                                                  float* __restrict__ rmax_out, int k, T boxsize, int64_t pil_size) {
+  __shared__ Vec<D, T> s_x[64];
+  __shared__ Vec<D, T> s_e[64];
+  __shared__ int s_n[64];
+  const int lane = threadIdx.x;
+  const int pq = blockIdx.x;
+  const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
+  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+  for (int qoff = q0; qoff < q1; qoff += 64) {
+    const int iq = min(qoff + lane, q1 - 1);
+    NodeT<D, T> nq = load_node<D, T>(nodes, iq);
+    Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
+    RmaxBins<KB> bins(k);
+    float rmax2 = INFINITY;
+    for (int e = e0; e < e1; e++) {
+      const int pt = pil_src[e];
+      const float r2t = pil_r2[e];
+      if (!wave_any(r2t <= rmax2)) break;
+      const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
+      for (int toff = t0; toff < t1; toff += 64) {
+        const int it = toff + lane;
+        if (it < t1) {
+          NodeT<D, T> nt = load_node<D, T>(nodes, it);
+          s_x[lane] = nt.cent;
+          s_e[lane] = lvl_half_ext<D, T>(nt.level);
+          s_n[lane] = npart[it];
+        }
+        __syncthreads();
+        const int nl = min(t1 - toff, 64);
+        for (int j = 0; j < nl; j++) {
+          float r2 = (float)maxdist2<D, T>(s_x[j], nq.cent, add<D, T>(eq, s_e[j]), boxsize);
+          bins.consider(r2, s_n[j]);
+        }
+        __syncthreads();
+      }
+      rmax2 = bins.max_r2(k) * kRmaxSlack;
+    }
+    if (qoff + lane < q1) rmax_out[iq] = rmax2;
+  }
+}
+
+template <int PASS, int D, typename T>
+__global__ __launch_bounds__(64) void k_count_insert(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src,
+                                                     const float* __restrict__ pil_r2,
+                                                     const int32_t* __restrict__ parent_spl, const T* __restrict__ nodes,
+                                                     const float* __restrict__ rmax2, int32_t* __restrict__ count_out,
+                                                     const int32_t* __restrict__ out_spl, int32_t* __restrict__ out_src,
+                                                     float* __restrict__ out_r2, int64_t nmax, T boxsize,
+                                                     int64_t pil_size) {
+  __shared__ Vec<D, T> s_x[64];
+  __shared__ Vec<D, T> s_e[64];
+  const int lane = threadIdx.x;
+  const int pq = blockIdx.x;
+  const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
+  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+  for (int qoff = q0; qoff < q1; qoff += 64) {
+    const int iq = min(qoff + lane, q1 - 1);
+    const bool valid = qoff + lane < q1;
+    NodeT<D, T> nq = load_node<D, T>(nodes, iq);
+    Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
+    const float rq = rmax2[iq];
+    int64_t off = (PASS == 1) ? (int64_t)out_spl[iq] : 0;
+    int cnt = 0;
+    for (int e = e0; e < e1; e++) {
+      const int pt = pil_src[e];
+      const float r2t = pil_r2[e];
+      if (!wave_any(r2t <= rq)) break;
+      const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
+      for (int toff = t0; toff < t1; toff += 64) {
+        const int it = toff + lane;
+        if (it < t1) {
+          NodeT<D, T> nt = load_node<D, T>(nodes, it);
+          s_x[lane] = nt.cent;
+          s_e[lane] = lvl_half_ext<D, T>(nt.level);
+        }
+        __syncthreads();
+        const int nl = min(t1 - toff, 64);
+        for (int j = 0; j < nl; j++) {
+          Vec<D, T> ew = add<D, T>(eq, s_e[j]);
+          float r2 = (float)mindist2<D, T>(s_x[j], nq.cent, ew, boxsize);
+          if (valid && r2 <= rq) {
+            if (PASS == 1) {
+              int64_t o = off + cnt;
+              if (o < nmax) {
+                // touching nodes get a tiny positive radius so the node itself sorts first
+                if (r2 == 0.f) r2 = 1e-10f * (float)maxdist2<D, T>(s_x[j], nq.cent, ew, boxsize);
+                out_r2[o] = r2;
+                out_src[o] = toff + j;
+              }
+            }
+            cnt++;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (PASS == 0 && valid) count_out[iq] = cnt;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// segmented sort by (key, val): bitonic network with only ascending comparators (the first stage of
+// each merge compares mirrored pairs), so lengths need not be powers of two (virtual +inf tail)
+// ------------------------------------------------------------------------------------------------
+constexpr int kSegThreads = 256;
+constexpr int kSegLds = 4096;  // entries held in LDS (32 KB)
+
+__device__ __forceinline__ uint64_t kv_pack(float k, int32_t v) {
+  uint32_t kb = __float_as_uint(k);
+  kb = (kb & 0x80000000u) ? ~kb : (kb | 0x80000000u);  // total order of IEEE floats
+  return ((uint64_t)kb << 32) | (uint32_t)v;
+}
+__device__ __forceinline__ float kv_key(uint64_t p) {
+  uint32_t kb = (uint32_t)(p >> 32);
+  kb = (kb & 0x80000000u) ? (kb & 0x7fffffffu) : ~kb;
+  return __uint_as_float(kb);
+}
+
+template <typename Arr>
+__device__ __forceinline__ void bitonic_sort_asc(Arr a, int len) {
+  int np2 = 1;
+  while (np2 < len) np2 <<= 1;
+  for (int size = 2; size <= np2; size <<= 1) {
+    // mirrored compare within each block of `size`
+    for (int t = threadIdx.x; t < np2 / 2; t += blockDim.x) {
+      int blk = t / (size / 2), off = t % (size / 2);
+      int i = blk * size + off;
+      int j = blk * size + size - 1 - off;
+      if (j < len) {
+        uint64_t x = a.load(i), y = a.load(j);
+        if (y < x) {
+          a.store(i, y);
+          a.store(j, x);
+        }
+      }
+    }
+    __syncthreads();
+    for (int stride = size / 4; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < np2 / 2; t += blockDim.x) {
+        int blk = t / stride, off = t % stride;
+        int i = blk * 2 * stride + off;
+        int j = i + stride;
+        if (j < len) {
+          uint64_t x = a.load(i), y = a.load(j);
+          if (y < x) {
+            a.store(i, y);
+            a.store(j, x);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct LdsArr {
+  uint64_t* p;
+  __device__ __forceinline__ uint64_t load(int i) const { return p[i]; }
+  __device__ __forceinline__ void store(int i, uint64_t x) const { p[i] = x; }
+};
+// (key, val) pairs sorted in place in global memory
+struct GlobalKV {
+  float* k;
+  int32_t* v;
+  __device__ __forceinline__ uint64_t load(int i) const { return kv_pack(k[i], v[i]); }
+  __device__ __forceinline__ void store(int i, uint64_t x) const {
+    k[i] = kv_key(x);
+    v[i] = (int32_t)(uint32_t)x;
+  }
+};
+
+// One workgroup per segment.  Segments up to kSegLds entries are sorted in LDS; longer ones in
+// place in the output arrays (global memory, visible within the workgroup across barriers).
+__global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __restrict__ spl, float* key,
+                                                              int32_t* val, const float* key_in, const int32_t* val_in,
+                                                              int64_t nmax) {
+  __shared__ uint64_t s_a[kSegLds];
+  const int seg = blockIdx.x;
+  int64_t b = spl[seg], e = spl[seg + 1];
+  if (e > nmax) e = nmax;
+  if (b >= e) return;
+  const int len = (int)(e - b);
+  if (len <= kSegLds) {
+    for (int i = threadIdx.x; i < len; i += blockDim.x) s_a[i] = kv_pack(key_in[b + i], val_in[b + i]);
+    __syncthreads();
+    bitonic_sort_asc(LdsArr{s_a}, len);
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      key[b + i] = kv_key(s_a[i]);
+      val[b + i] = (int32_t)(uint32_t)s_a[i];
+    }
+  } else {
+    if (key != key_in) {
+      for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        key[b + i] = key_in[b + i];
+        val[b + i] = val_in[b + i];
+      }
+      __syncthreads();
+    }
+    bitonic_sort_asc(GlobalKV{key + b, val + b}, len);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// leaf-vs-leaf top-k
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float key_r2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+// Insert `key` into the ascending list best[0..K) (caller guarantees key < best[K-1]).  Top-down:
+// best[i] takes best[i-1] while key < best[i-1], takes key at the insertion point, and keeps its
+// value below it.  Every 8 steps the wave leaves the (fully unrolled, statically indexed) network
+// once no active lane still shifts; checking less often keeps the register count low.
+template <int K>
+__device__ __forceinline__ void topk_insert(uint64_t (&best)[K], uint64_t key) {
+  bool hi = true;  // key < best[i] (true for i = K-1 by the caller's test)
+#pragma unroll
+  for (int i = K - 1; i >= 1; i--) {
+    const bool lo = key < best[i - 1];
+    best[i] = hi ? (lo ? best[i - 1] : key) : best[i];
+    hi = lo;
+    if ((i & 7) == 0 && !wave_any(lo)) return;
+  }
+  best[0] = hi ? key : best[0];
+}
+
+constexpr uint64_t kEmptyKey = (0x7f800000ull << 32) | 0xffffffffull;  // (r2 = +inf, id = -1)
+
+__device__ __forceinline__ int rdlane(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
+__device__ __forceinline__ float rdlane(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+
+// One wave per query leaf.  Interaction-list metadata (source leaf, r2, particle range) is loaded
+// 64 entries at a time, one per lane, and broadcast with readlane.  Source particles move in
+// chunks of <= 64: the next chunk is loaded into registers (one particle per lane) while the
+// current chunk is consumed from LDS, where every lane reads the same entry (broadcast).
+template <int K, bool PER, int D, typename T>
+__global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
+                                                 const float* __restrict__ il_r2, const int32_t* __restrict__ splT,
+                                                 const T* __restrict__ xT, const int32_t* __restrict__ splQ,
+                                                 const T* __restrict__ xQ, const int32_t* __restrict__ src_ids,
+                                                 T* __restrict__ rad_out, int32_t* __restrict__ id_out,
+                                                 uint64_t* __restrict__ resume, int koff, int ktot, T boxsize,
+                                                 int64_t il_size) {
+  __shared__ Vec<D, T> s_p[64];
+  __shared__ uint32_t s_id[64];
+  const int lane = threadIdx.x;
+  const int lq = blockIdx.x;
+  const int q0 = splQ[lq], q1 = splQ[lq + 1];
+  const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
+  const T half = (T)(float)(T(0.5) * boxsize);
+  const bool use_floor = resume != nullptr && koff > 0;
+  for (int qoff = q0; qoff < q1; qoff += 64) {
+    const int iq = min(qoff + lane, q1 - 1);
+    const Vec<D, T> q = load_vec<D, T>(xQ, iq);
+    const uint64_t floor_key = use_floor ? resume[iq] : 0ull;
+    uint64_t best[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) best[i] = kEmptyKey;
+    bool done = false;
+    for (int eb = e0; eb < e1 && !done; eb += 64) {
+      const int ne = min(64, e1 - eb);
+      int m_t0 = 0, m_t1 = 0;
+      float m_r2 = INFINITY;
+      if (lane < ne) {
+        const int src = il_src[eb + lane];
+        m_r2 = il_r2[eb + lane];
+        m_t0 = splT[src];
+        m_t1 = splT[src + 1];
+      }
+      // first chunk of this batch
+      Vec<D, T> cp;
+      uint32_t cid = 0;
+      int cn;
+      {
+        const int t0 = rdlane(m_t0, 0), t1 = rdlane(m_t1, 0);
+        cn = min(64, t1 - t0);
+        if (lane < cn) {
+          cp = load_vec<D, T>(xT, t0 + lane);
+          cid = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+        }
+      }
+      int i = 0, off = 0;
+      while (i < ne) {
+        if (off == 0) {
+          const float r2t = rdlane(m_r2, i);
+          if (!wave_any(r2t <= key_r2(best[K - 1]) * kRmaxSlack)) {
+            done = true;
+            break;
+          }
+        }
+        const int len = rdlane(m_t1, i) - rdlane(m_t0, i);
+        int ni = i, noff = off + 64;
+        if (noff >= len) {
+          ni = i + 1;
+          noff = 0;
+        }
+        const int n_cur = cn;
+        if (lane < n_cur) {
+          s_p[lane] = cp;
+          s_id[lane] = cid;
+        }
+        __syncthreads();
+        if (ni < ne) {
+          const int t0 = rdlane(m_t0, ni) + noff, t1 = rdlane(m_t1, ni);
+          cn = min(64, t1 - t0);
+          if (lane < cn) {
+            cp = load_vec<D, T>(xT, t0 + lane);
+            cid = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+          }
+        }
+        for (int j = 0; j < n_cur; j++) {
+          const Vec<D, T> p = s_p[j];
+          const float r2 = (float)dist2P<PER, D, T>(q, p, boxsize, half);
+          const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | s_id[j];
+          if (key < best[K - 1] && (!use_floor || key > floor_key)) topk_insert<K>(best, key);
+        }
+        __syncthreads();
+        i = ni;
+        off = noff;
+      }
+    }
+    if (qoff + lane < q1) {
+      const int64_t row = (int64_t)iq * ktot + koff;
+#pragma unroll
+      for (int i = 0; i < K; i++) {
+        if (i < ktot - koff) {
+          rad_out[row + i] = (T)sqrtf(key_r2(best[i]));
+          id_out[row + i] = (int32_t)(uint32_t)best[i];
+        }
+      }
+      if (resume) {
+        const int last = min(K, ktot - koff) - 1;
+        uint64_t lk = best[0];
+#pragma unroll
+        for (int i = 0; i < K; i++) lk = (i == last) ? best[i] : lk;
+        resume[iq] = lk;
+      }
+    }
+  }
+}
+
+template <int D, typename T>
+int node2node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, const float* pil_r2,
+                   const int32_t* parent_spl, const void* nodes, const int32_t* npart, float* rmax2, int32_t* il_spl,
+                   int32_t* il_src, float* il_r2, int64_t size_parents, int64_t size_nodes, int64_t pil_size,
+                   int64_t il_size, int k, float boxsize) {
+  const T* nd = (const T*)nodes;
+  if (fill_u32(s, il_spl, 0u, size_nodes + 1)) return 1;
+  if (size_parents <= 0) return 0;
+  dim3 g((unsigned)size_parents), b(64);
+  if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
+  else if (k <= 32) hipLaunchKernelGGL((k_find_rmax<16, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
+  else hipLaunchKernelGGL((k_find_rmax<32, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
+  if (check_launch("KnnNode2Node FindRmax")) return 1;
+  hipLaunchKernelGGL((k_count_insert<0, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, rmax2, il_spl + 1,
+                     (const int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr, il_size, (T)boxsize, pil_size);
+  // inclusive scan of the counts in place; the scan temp lives in the (not yet written) list
+  if (scan_tmp_words(size_nodes) > il_size) return fail("KnnNode2Node: interaction list too small for scan temp");
+  if (scan_i32(s, il_spl + 1, il_spl + 1, size_nodes, true, il_src)) return 1;
+  hipLaunchKernelGGL((k_count_insert<1, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, rmax2,
+                     (int32_t*)nullptr, il_spl, il_src, il_r2, il_size, (T)boxsize, pil_size);
+  if (check_launch("KnnNode2Node insert")) return 1;
+  // sort each node's list by (r2, src) in place
+  hipLaunchKernelGGL(k_segment_sort, dim3((unsigned)size_nodes), dim3(kSegThreads), 0, s, il_spl, il_r2, il_src,
+                     il_r2, il_src, il_size);
+  return check_launch("KnnNode2Node sort");
+}
+
+template <int D, typename T>
+int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, const float* il_r2, const int32_t* splT,
+                   const void* xT, const int32_t* splQ, const void* xQ, const int32_t* src_ids, void* rad, int32_t* ids,
+                   void* tmp, int64_t nleaves_q, int64_t npart_q, int64_t il_size, int k, float boxsize) {
+  if (nleaves_q <= 0) return 0;
+  uint64_t* resume = k > 64 ? (uint64_t*)tmp : nullptr;
+  dim3 g((unsigned)nleaves_q), b(64);
+  for (int koff = 0; koff < k; koff += 64) {
+    int kn = min(k - koff, 64);
+#define JZT_KNN_LAUNCH(KK)                                                                                      \
+  do {                                                                                                          \
+    if (boxsize > 0.f)                                                                                          \
+      hipLaunchKernelGGL((k_knn_leaf<KK, true, D, T>), g, b, 0, s, il_spl, il_src, il_r2, splT, (const T*)xT,   \
+                         splQ, (const T*)xQ, src_ids, (T*)rad, ids, resume, koff, k, (T)boxsize, il_size);               \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_knn_leaf<KK, false, D, T>), g, b, 0, s, il_spl, il_src, il_r2, splT, (const T*)xT,  \
+                         splQ, (const T*)xQ, src_ids, (T*)rad, ids, resume, koff, k, (T)boxsize, il_size);               \
+  } while (0)
+    if (kn <= 8) JZT_KNN_LAUNCH(8);
+    else if (kn <= 16) JZT_KNN_LAUNCH(16);
+    else if (kn <= 32) JZT_KNN_LAUNCH(32);
+    else JZT_KNN_LAUNCH(64);
+#undef JZT_KNN_LAUNCH
+    if (check_launch("KnnLeaf2Leaf")) return 1;
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace jzt
+
+using namespace jzt;
+
+extern "C" int jzt_knn_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                                 const float* parent_ilist_r2, const int32_t* parent_spl, const void* nodes,
+                                 const int32_t* nodes_npart, float* node_rmax2, int32_t* node_ilist_spl,
+                                 int32_t* node_ilist_isrc, float* node_ilist_r2, int64_t size_parents,
+                                 int64_t size_nodes, int64_t parent_ilist_size, int64_t node_ilist_size, int k,
+                                 float boxsize, int dim, int dtype) {
+  hipStream_t s = (hipStream_t)stream;
+  if (k < 1) return fail("KnnNode2Node: k must be >= 1");
+  if (dtype == JZT_F32 && dim == 3)
+    return node2node_impl<3, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes, nodes_npart,
+                                    node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2, size_parents, size_nodes,
+                                    parent_ilist_size, node_ilist_size, k, boxsize);
+  if (dtype == JZT_F32 && dim == 2)
+    return node2node_impl<2, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes, nodes_npart,
+                                    node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2, size_parents, size_nodes,
+                                    parent_ilist_size, node_ilist_size, k, boxsize);
+  return fail("KnnNode2Node: unsupported dim/dtype");
+}
+
+extern "C" size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype) {
+  (void)dtype;
+  return k > 64 ? (size_t)size_part_query * 8 : 0;
+}
+
+extern "C" int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
+                                 const float* ilist_r2, int64_t ilist_size, const int32_t* splT, const void* xT,
+                                 const int32_t* splQ, const void* xQ, const int32_t* src_ids, void* knn_rad,
+                                 int32_t* knn_id, void* tmp, size_t tmp_bytes, int64_t size_leaves_query,
+                                 int64_t size_part_query, int k, float boxsize, int dim, int dtype) {
+  hipStream_t s = (hipStream_t)stream;
+  if (k < 1) return fail("KnnLeaf2Leaf: k must be >= 1");
+  if (tmp_bytes < jzt_knn_leaf2leaf_tmp_bytes(size_part_query, k, dtype)) return fail("KnnLeaf2Leaf: tmp too small");
+  if (dtype == JZT_F32 && dim == 3)
+    return leaf2leaf_impl<3, float>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, knn_rad, knn_id,
+                                    tmp, size_leaves_query, size_part_query, ilist_size, k, boxsize);
+  if (dtype == JZT_F32 && dim == 2)
+    return leaf2leaf_impl<2, float>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, knn_rad, knn_id,
+                                    tmp, size_leaves_query, size_part_query, ilist_size, k, boxsize);
+  return fail("KnnLeaf2Leaf: unsupported dim/dtype");
+}
+
+extern "C" int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const float* key, const int32_t* val,
+                                float* key_out, int32_t* val_out, int64_t size_segs, int64_t size_keys) {
+  hipStream_t s = (hipStream_t)stream;
+  if (size_segs <= 0) return 0;
+  hipLaunchKernelGGL(k_segment_sort, dim3((unsigned)size_segs), dim3(kSegThreads), 0, s, spl, key_out, val_out, key, val,
+                     size_keys);
+  return check_launch("SegmentSort");
+}

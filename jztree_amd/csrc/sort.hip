@@ -1,0 +1,566 @@
+// Z-order sort of positions (replaces jztree's PosZorderSort, sort.cuh:40-76).
+//
+// The reference sorts the float vectors with a CUB merge sort and the comparator z_pos_less
+// (math.cuh:381-402).  Here the order is computed with an LSD radix sort of a 64-bit Morton key:
+//
+//   1. bounds: max exponent of the finite coordinates, any negative, any inf  (read 12 B/pt)
+//   2. keygen: 63-bit interleaved key + input index; fused 8x256 digit histograms (r12 w12 B/pt)
+//   3. digit offsets: exclusive scan of the 8 histograms (1 workgroup)
+//   4. 8 onesweep passes (8-bit digits): dynamic tile ids, LDS ranking with wave64 ballots,
+//      decoupled look-back on per-tile digit counts, LDS-staged coalesced scatter (r12 w12 B/pt)
+//   5. fix-up: runs of equal keys are re-ordered with the exact comparator (rare)
+//   6. gather: pos_out = pos_in[idx], id_out = idx                           (r4+12 w12+4 B/pt)
+//
+// Key construction.  For coordinates whose largest finite exponent is Lt, each dim contributes a
+// B-bit code (B = 63/dim, i.e. 21 for 3D): the fixed-point bits of |x| at levels Lt-B+1 .. Lt.
+// If any coordinate is negative or infinite, the top 2 bits of each code become a digit ranking
+// {-inf, negative, non-negative, +inf} (digit-interleaved above all magnitude bits) and the
+// magnitude bits of negatives are complemented.  Interleaving these codes (x most significant
+// within a level) gives a key whose order is consistent with the exact Morton order: K(a) < K(b)
+// implies a precedes b.  Equal keys are resolved by the exact comparator, then by input index.
+// NaN rows get the key 2^64-1 and sort last.  The exact comparator is z_pos_less with two quirks
+// of the reference removed (exactly equal coordinates get level -inf instead of e-24, and -0.0 is
+// treated as +0.0); on every pair where the reference's comparator is a strict weak order the two
+// orders agree.
+#include "util.h"
+#include "../../include/jztree_amd.h"
+
+namespace jzt {
+namespace {
+
+constexpr int kSortThreads = 256;  // 4 waves
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kKPT = 12;                          // keys per thread
+constexpr int kTile = kSortThreads * kKPT;        // 3072 keys per tile
+constexpr int kRadix = 256;
+constexpr int kPasses = 8;
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kCountMask = (1u << 30) - 1;
+
+// ---------------------------------------------------------------------------------------------
+// bit spreading (interleave with x most significant within a level)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {  // 21 bits -> every 3rd bit
+  x &= 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__device__ __forceinline__ uint64_t spread2(uint64_t x) {  // 31 bits -> every 2nd bit
+  x &= 0x7fffffffull;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+template <typename T> struct FBits;
+template <> struct FBits<float> {
+  using U = uint32_t;
+  static constexpr int kMant = 23, kBias = 127, kExpMax = 255;
+  static __device__ __forceinline__ U bits(float x) { return __float_as_uint(x); }
+};
+template <> struct FBits<double> {
+  using U = uint64_t;
+  static constexpr int kMant = 52, kBias = 1023, kExpMax = 2047;
+  static __device__ __forceinline__ U bits(double x) { return (uint64_t)__double_as_longlong(x); }
+};
+
+// floor(|x| / 2^(Lt-nb+1)) for finite x with exponent <= Lt: the nb fixed-point bits below Lt
+template <typename T> __device__ __forceinline__ uint64_t mag_code(T x, int Lt, int nb) {
+  using FB = FBits<T>;
+  using U = typename FB::U;
+  U u = FB::bits(x) & ~((U)1 << (sizeof(U) * 8 - 1));
+  int be = (int)(u >> FB::kMant);
+  U mant = u & (((U)1 << FB::kMant) - 1);
+  U M = be ? (mant | ((U)1 << FB::kMant)) : mant;
+  int E = (be ? be : 1) - FB::kBias;
+  int s = Lt - E + FB::kMant + 1 - nb;
+  if (s >= (int)(sizeof(U) * 8)) return 0;
+  if (s < 0) return (uint64_t)M << (-s);  // more code bits than mantissa bits (2D float32)
+  return (uint64_t)(M >> s);
+}
+
+// top digit: -inf 0, negative finite 1, non-negative finite (incl. -0.0) 2, +inf 3
+template <typename T> __device__ __forceinline__ int top_digit(T x) {
+  if (__builtin_isinf(x)) return x < T(0) ? 0 : 3;
+  return x < T(0) ? 1 : 2;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ uint64_t morton_key(const T* x, int Lt, bool general) {
+  bool isnan_row = false;
+#pragma unroll
+  for (int d = 0; d < D; d++) isnan_row |= __builtin_isnan(x[d]);
+  if (isnan_row) return ~0ull;
+  constexpr int B = 63 / D;
+  uint64_t c[D];
+  uint64_t top = 0;
+  if (!general) {
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] = mag_code<T>(x[d], Lt, B);
+  } else {
+    constexpr int nb = B - 2;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      int t = top_digit<T>(x[d]);
+      uint64_t m = 0;
+      if (t == 1) m = ((1ull << nb) - 1) - mag_code<T>(x[d], Lt, nb);
+      if (t == 2) m = mag_code<T>(x[d], Lt, nb);
+      c[d] = m;
+      top = (top << 2) | (uint64_t)t;
+    }
+  }
+  uint64_t key = 0;
+  if constexpr (D == 3) {
+    key = (spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]);
+  } else if constexpr (D == 2) {
+    key = (spread2(c[0]) << 1) | spread2(c[1]);
+  } else {
+    for (int b = B - 1; b >= 0; b--)
+      for (int d = 0; d < D; d++) key = (key << 1) | ((c[d] >> b) & 1ull);
+  }
+  if (general) key |= top << (D * (B - 2));
+  return key;
+}
+
+// ---------------------------------------------------------------------------------------------
+// exact comparator (device copy of the rule described above); -1/0/+1
+// ---------------------------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ int exact_level(T a, T b) {
+  using FB = FBits<T>;
+  using U = typename FB::U;
+  int ta = top_digit<T>(a), tb = top_digit<T>(b);
+  if (ta != tb) return 1 << 20;
+  U ab = FB::bits(a) & ~((U)1 << (sizeof(U) * 8 - 1));
+  U bb = FB::bits(b) & ~((U)1 << (sizeof(U) * 8 - 1));
+  if (ab == bb) return -(1 << 20);
+  int ae = (int)(ab >> FB::kMant) - FB::kBias, be = (int)(bb >> FB::kMant) - FB::kBias;
+  if (ae == be) {
+    int lz = (sizeof(U) == 4) ? __clz((uint32_t)(ab ^ bb)) : __clzll((uint64_t)(ab ^ bb));
+    return max(ae, 1 - FB::kBias) + ((int)(sizeof(U) * 8) - FB::kMant - 1 - lz);
+  }
+  return max(ae, be);
+}
+
+template <int D, typename T> __device__ int exact_cmp(const T* p1, const T* p2) {
+  bool n1 = false, n2 = false;
+  for (int d = 0; d < D; d++) {
+    n1 |= __builtin_isnan(p1[d]);
+    n2 |= __builtin_isnan(p2[d]);
+  }
+  if (n1 || n2) return (int)n1 - (int)n2;
+  int md = 0, m = exact_level<T>(p1[0], p2[0]);
+  for (int d = 1; d < D; d++) {
+    int l = exact_level<T>(p1[d], p2[d]);
+    if (l > m) {
+      m = l;
+      md = d;
+    }
+  }
+  if (m == -(1 << 20)) return 0;
+  T a = p1[md], b = p2[md];
+  if (m == (1 << 20)) return top_digit<T>(a) < top_digit<T>(b) ? -1 : 1;
+  return a < b ? -1 : 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 1. bounds
+// ---------------------------------------------------------------------------------------------
+// meta[0] = max biased exponent of finite nonzero coordinates, meta[1] = any negative, meta[2] = any inf
+template <typename T>
+__global__ __launch_bounds__(256) void k_bounds(const T* __restrict__ pos, int64_t nval, uint32_t* meta) {
+  using FB = FBits<T>;
+  using U = typename FB::U;
+  uint32_t emax = 0, neg = 0, inf = 0;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nval; i += st) {
+    T x = pos[i];
+    U u = FB::bits(x) & ~((U)1 << (sizeof(U) * 8 - 1));
+    uint32_t be = (uint32_t)(u >> FB::kMant);
+    bool fin = be != (uint32_t)FB::kExpMax;
+    if (fin && u != 0) emax = max(emax, be ? be : 1u);
+    neg |= (x < T(0)) ? 1u : 0u;
+    inf |= (!fin && (u & (((U)1 << FB::kMant) - 1)) == 0) ? 1u : 0u;
+  }
+  // wave reductions then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) {
+    emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
+    neg |= (uint32_t)__shfl_xor((int)neg, o, 64);
+    inf |= (uint32_t)__shfl_xor((int)inf, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&meta[0], emax);
+    if (neg) atomicOr(&meta[1], 1u);
+    if (inf) atomicOr(&meta[2], 1u);
+  }
+}
+
+template <typename T> __device__ __forceinline__ int top_level_from_meta(const uint32_t* meta) {
+  using FB = FBits<T>;
+  uint32_t be = meta[0];
+  return (int)(be ? be : 1u) - FB::kBias;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2. keygen + digit histograms
+// ---------------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_keygen(const T* __restrict__ pos, int64_t n, const uint32_t* __restrict__ meta,
+                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                uint32_t* __restrict__ hist) {
+  __shared__ uint32_t sh[kPasses][kRadix];
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&sh[0][0])[i] = 0;
+  __syncthreads();
+  int Lt = top_level_from_meta<T>(meta);
+  bool general = (meta[1] | meta[2]) != 0;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    T x[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) x[d] = pos[i * D + d];
+    uint64_t k = morton_key<D, T>(x, Lt, general);
+    keys[i] = k;
+    vals[i] = (uint32_t)i;
+#pragma unroll
+    for (int p = 0; p < kPasses; p++) atomicAdd(&sh[p][(k >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) {
+    uint32_t c = (&sh[0][0])[j];
+    if (c) atomicAdd(&hist[j], c);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 3. digit offsets
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
+  __shared__ int sbuf[4];
+  for (int p = 0; p < kPasses; p++) {
+    int v = (int)hist[p * kRadix + threadIdx.x];
+    int tot;
+    int ex = block_excl_scan<256>(v, sbuf, &tot);
+    hist[p * kRadix + threadIdx.x] = (uint32_t)ex;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 4. onesweep pass
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           int64_t n, int shift,
+                                                           const uint32_t* __restrict__ bucket_off,
+                                                           uint32_t* lookback, uint32_t* tile_counter) {
+  __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
+  __shared__ uint32_t s_lstart[kRadix];
+  __shared__ uint32_t s_gbase[kRadix];
+  __shared__ uint64_t s_keys[kTile];
+  __shared__ uint8_t s_dig[kTile];
+  __shared__ int s_scan[kSortThreads / 64];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t base = (int64_t)tile * kTile;
+  const int64_t wbase = base + (int64_t)w * 64 * kKPT;
+
+  uint64_t key[kKPT];
+  uint32_t dig[kKPT];
+  uint32_t rank[kKPT];
+#pragma unroll
+  for (int r = 0; r < kKPT; r++) {
+    int64_t idx = wbase + r * 64 + lane;
+    key[r] = idx < n ? kin[idx] : ~0ull;
+    dig[r] = (uint32_t)(key[r] >> shift) & 255u;
+  }
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < kKPT; r++) {
+    uint32_t d = dig[r];
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    uint32_t pre = s_wcnt[w][d];
+    uint32_t below = (uint32_t)__popcll(m & lt_mask);
+    rank[r] = pre + below;
+    if (below == 0) s_wcnt[w][d] = pre + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+
+  // per-digit tile totals and per-wave exclusive offsets (thread tid owns digit tid)
+  const int d = tid;
+  uint32_t total = 0;
+#pragma unroll
+  for (int ww = 0; ww < kSortWaves; ww++) {
+    uint32_t c = s_wcnt[ww][d];
+    s_wcnt[ww][d] = total;
+    total += c;
+  }
+  // decoupled look-back across tiles, one digit per thread
+  uint32_t* lb = lookback + (int64_t)tile * kRadix;
+  uint32_t excl = 0;
+  if (tile == 0) {
+    st_rel(&lb[d], kFlagInc | total);
+  } else {
+    st_rel(&lb[d], kFlagAgg | total);
+    int64_t j = (int64_t)tile - 1;
+    while (true) {
+      uint32_t v = ld_acq(&lookback[j * kRadix + d]);
+      uint32_t f = v & ~kCountMask;
+      if (f == 0) continue;
+      excl += v & kCountMask;
+      if (f == kFlagInc) break;
+      j--;
+    }
+    st_rel(&lb[d], kFlagInc | (excl + total));
+  }
+  int tot;
+  uint32_t lstart = (uint32_t)block_excl_scan<kSortThreads>((int)total, s_scan, &tot);
+  s_lstart[d] = lstart;
+  s_gbase[d] = bucket_off[d] + excl - lstart;
+  __syncthreads();
+
+  // scatter keys into LDS in tile-local digit order
+#pragma unroll
+  for (int r = 0; r < kKPT; r++) {
+    uint32_t dd = dig[r];
+    uint32_t p = s_lstart[dd] + s_wcnt[w][dd] + rank[r];
+    s_keys[p] = key[r];
+    s_dig[p] = (uint8_t)dd;
+  }
+  __syncthreads();
+  const int tile_n = (int)((n - base) < kTile ? (n - base) : kTile);
+  for (int i = tid; i < tile_n; i += kSortThreads) {
+    const uint32_t g = s_gbase[s_dig[i]] + (uint32_t)i;
+    if (g < (uint64_t)n) kout[g] = s_keys[i];
+  }
+  __syncthreads();
+  uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+  for (int r = 0; r < kKPT; r++) {
+    int64_t idx = wbase + r * 64 + lane;
+    uint32_t v = idx < n ? vin[idx] : 0u;
+    uint32_t dd = dig[r];
+    s_vals[s_lstart[dd] + s_wcnt[w][dd] + rank[r]] = v;
+  }
+  __syncthreads();
+  for (int i = tid; i < tile_n; i += kSortThreads) {
+    const uint32_t g = s_gbase[s_dig[i]] + (uint32_t)i;
+    if (g < (uint64_t)n) vout[g] = s_vals[i];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 5. fix-up of equal-key runs (exact comparator, then index)
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_run_flags(const uint64_t* __restrict__ keys, int64_t n,
+                                                   int8_t* __restrict__ flags) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  flags[i] = (i + 1 < n && keys[i + 1] == k && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ bool idx_less(const T* __restrict__ pos, uint32_t a, uint32_t b) {
+  int c = exact_cmp<D, T>(pos + (int64_t)a * D, pos + (int64_t)b * D);
+  return (c < 0) || (c == 0 && a < b);
+}
+
+// One workgroup per run of equal keys (grid-stride over the run list).  The run's end is found by
+// a strided parallel search, sortedness by (exact order, index) is checked pairwise, and unsorted
+// runs are put in order by a bitonic network (ascending comparators only, so any length works)
+// applied in place to the run's indices in global memory.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_fix_runs(const uint64_t* __restrict__ keys, uint32_t* vals, int64_t n,
+                                                  const T* __restrict__ pos, const int32_t* __restrict__ runs,
+                                                  const int32_t* __restrict__ nruns_p, int64_t max_runs) {
+  __shared__ unsigned long long s_end;
+  const int64_t nruns = min((int64_t)nruns_p[0], max_runs);
+  for (int64_t r = blockIdx.x; r < nruns; r += gridDim.x) {
+    const int64_t start = runs[r];
+    const uint64_t k = keys[start];
+    if (threadIdx.x == 0) s_end = (unsigned long long)n;
+    __syncthreads();
+    for (int64_t j = start + 1 + threadIdx.x; j < n; j += blockDim.x) {
+      if (keys[j] != k) {
+        atomicMin(&s_end, (unsigned long long)j);
+        break;
+      }
+    }
+    __syncthreads();
+    const int64_t end = (int64_t)s_end;
+    bool bad = false;
+    for (int64_t j = start + threadIdx.x; j + 1 < end; j += blockDim.x)
+      bad |= !idx_less<D, T>(pos, vals[j], vals[j + 1]);
+    if (__syncthreads_or(bad)) {
+      const int64_t len = end - start;
+      uint32_t* a = vals + start;
+      int64_t np2 = 1;
+      while (np2 < len) np2 <<= 1;
+      for (int64_t size = 2; size <= np2; size <<= 1) {
+        for (int64_t t = threadIdx.x; t < np2 / 2; t += blockDim.x) {
+          int64_t blk = t / (size / 2), off = t % (size / 2);
+          int64_t i = blk * size + off, j = blk * size + size - 1 - off;
+          if (j < len) {
+            uint32_t x = a[i], y = a[j];
+            if (idx_less<D, T>(pos, y, x)) {
+              a[i] = y;
+              a[j] = x;
+            }
+          }
+        }
+        __syncthreads();
+        for (int64_t stride = size / 4; stride > 0; stride >>= 1) {
+          for (int64_t t = threadIdx.x; t < np2 / 2; t += blockDim.x) {
+            int64_t blk = t / stride, off = t % stride;
+            int64_t i = blk * 2 * stride + off, j = i + stride;
+            if (j < len) {
+              uint32_t x = a[i], y = a[j];
+              if (idx_less<D, T>(pos, y, x)) {
+                a[i] = y;
+                a[j] = x;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 6. gather
+// ---------------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ pos, const uint32_t* __restrict__ vals, int64_t n,
+                                                T* __restrict__ pos_out, typename IntOf<T>::type* __restrict__ id_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t v = vals[i];
+  if (v >= (uint64_t)n) v = 0;  // unreachable for a valid permutation; keeps a logic error in bounds
+#pragma unroll
+  for (int d = 0; d < D; d++) pos_out[i * D + d] = pos[(int64_t)v * D + d];
+  id_out[i] = (typename IntOf<T>::type)v;
+}
+
+struct SortLayout {
+  size_t keys0, vals0, keys1, vals1, hist, meta, lookback, poscopy, runflags, runtmp, runs, nruns, total;
+  int64_t ntiles;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+SortLayout sort_layout(int64_t n, int dim, int dtype) {
+  SortLayout L;
+  size_t o = 0;
+  L.ntiles = div_up(n > 0 ? n : 1, kTile);
+  L.keys0 = o; o = align256(o + 8 * (size_t)n);
+  L.vals0 = o; o = align256(o + 4 * (size_t)n);
+  L.keys1 = o; o = align256(o + 8 * (size_t)n);
+  L.vals1 = o; o = align256(o + 4 * (size_t)n);
+  L.hist = o; o += 4 * (size_t)kPasses * kRadix;
+  L.meta = o; o += 4 * 16;
+  L.lookback = o; o = align256(o + 4 * (size_t)kPasses * L.ntiles * kRadix);
+  L.poscopy = o; o = align256(o + (size_t)n * dim * (dtype == JZT_F64 ? 8 : 4));
+  L.runflags = o; o = align256(o + (size_t)n);
+  L.runtmp = o; o = align256(o + 4 * (size_t)(scan_tmp_words(n) + n));
+  L.runs = o; o = align256(o + 4 * (size_t)(n / 2 + 1));
+  L.nruns = o; o = align256(o + 4);
+  L.total = o;
+  return L;
+}
+
+template <int D, typename T>
+int zsort_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::type* id_out, char* tmp, int64_t n) {
+  SortLayout L = sort_layout(n, D, std::is_same<T, double>::value ? JZT_F64 : JZT_F32);
+  uint64_t* k0 = (uint64_t*)(tmp + L.keys0);
+  uint64_t* k1 = (uint64_t*)(tmp + L.keys1);
+  uint32_t* v0 = (uint32_t*)(tmp + L.vals0);
+  uint32_t* v1 = (uint32_t*)(tmp + L.vals1);
+  uint32_t* hist = (uint32_t*)(tmp + L.hist);
+  uint32_t* meta = (uint32_t*)(tmp + L.meta);
+  uint32_t* lookback = (uint32_t*)(tmp + L.lookback);
+  // zero histograms, meta (incl. tile counters) and look-back state in one fill
+  if (fill_u32(s, hist, 0u, (int64_t)((L.poscopy - L.hist) / 4))) return 1;
+  const T* src = pos_in;
+  if ((const void*)pos_in == (const void*)pos_out) {
+    T* cp = (T*)(tmp + L.poscopy);
+    if (hipMemcpyAsync(cp, pos_in, (size_t)n * D * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail("zsort: copy of aliased input failed");
+    src = cp;
+  }
+  int64_t blocks = div_up(n * D, 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((k_bounds<T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n * D, meta);
+  blocks = div_up(n, 256);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL((k_keygen<D, T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n, meta, k0, v0, hist);
+  hipLaunchKernelGGL(k_hist_scan, dim3(1), dim3(256), 0, s, hist);
+  if (check_launch("zsort keygen")) return 1;
+  for (int p = 0; p < kPasses; p++) {
+    const uint64_t* ki = (p & 1) ? k1 : k0;
+    const uint32_t* vi = (p & 1) ? v1 : v0;
+    uint64_t* ko = (p & 1) ? k0 : k1;
+    uint32_t* vo = (p & 1) ? v0 : v1;
+    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n, 8 * p,
+                       hist + p * kRadix, lookback + (int64_t)p * L.ntiles * kRadix, meta + 8 + p);
+  }
+  if (check_launch("zsort onesweep")) return 1;
+  // fix-up of equal-key runs
+  int8_t* rflags = (int8_t*)(tmp + L.runflags);
+  int32_t* runs = (int32_t*)(tmp + L.runs);
+  int32_t* nruns = (int32_t*)(tmp + L.nruns);
+  const int64_t max_runs = n / 2 + 1;
+  hipLaunchKernelGGL(k_run_flags, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, k0, n, rflags);
+  if (where_flags_i8(s, rflags, n, runs, max_runs, 0, nullptr, nruns, (int32_t*)(tmp + L.runtmp))) return 1;
+  hipLaunchKernelGGL((k_fix_runs<D, T>), dim3(1024), dim3(256), 0, s, k0, v0, n, src, runs, nruns, max_runs);
+  hipLaunchKernelGGL((k_gather<D, T>), dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, src, v0, n, pos_out, id_out);
+  return check_launch("zsort gather");
+}
+
+}  // namespace
+}  // namespace jzt
+
+using namespace jzt;
+
+extern "C" size_t jzt_zsort_tmp_bytes(int64_t n, int dim, int dtype) { return sort_layout(n, dim, dtype).total; }
+
+extern "C" int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out, void* tmp,
+                                   size_t tmp_bytes, int64_t n, int dim, int dtype) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0) return fail("jzt_pos_zorder_sort: negative n");
+  if (n == 0) return 0;
+  if (n >= (int64_t)kCountMask) return fail("jzt_pos_zorder_sort: n must be < 2^30");
+  size_t need = sort_layout(n, dim, dtype).total;
+  if (tmp_bytes < need)
+    return fail("jzt_pos_zorder_sort: tmp buffer too small (have " + std::to_string(tmp_bytes) + ", need " +
+                std::to_string(need) + ")");
+  char* t = (char*)tmp;
+  if (dtype == JZT_F32) {
+    if (dim == 3) return zsort_impl<3, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n);
+    if (dim == 2) return zsort_impl<2, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n);
+  } else if (dtype == JZT_F64) {
+    if (dim == 3) return zsort_impl<3, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
+    if (dim == 2) return zsort_impl<2, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
+  }
+  return fail("jzt_pos_zorder_sort: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
+}

@@ -1,0 +1,328 @@
+// Leaf and node construction from z-sorted positions (replaces jztree's tree.cuh kernels).
+//
+//  FlagLeafBoundaries  tree.cuh:13-98    boundary b (between particles b-1 and b) is a leaf split if
+//                                        the binary Morton cell it separates holds > max_size
+//                                        particles (counted in an LDS window of scan_size boundaries)
+//  FindNodeBoundaries  tree.cuh:104-199  per boundary of the leaf representatives: its level and the
+//                                        [lbound, rbound) extent of the node it splits (binary search)
+//  GetNodeGeometry     tree.cuh:312-372  level, centre and extent of the smallest cell holding a range
+//
+// All integer/bit exact.  Launch: 256 threads (4 waves) per workgroup, one boundary/node per lane.
+#include "util.h"
+#include "../../include/jztree_amd.h"
+
+namespace jzt {
+namespace {
+
+constexpr int kTreeThreads = 256;
+
+template <int D, typename T>
+__global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict__ posz, const uint8_t* __restrict__ ptype,
+                                                            const int32_t* __restrict__ lvl_bound,
+                                                            const int32_t* __restrict__ npart_p,
+                                                            int8_t* __restrict__ split_flags, int32_t* __restrict__ lvl,
+                                                            int64_t size_part, int max_size, int scan_size,
+                                                            int num_types) {
+  extern __shared__ int32_t s_level[];
+  const int nload = kTreeThreads + 2 * scan_size + 1;
+  uint8_t* s_types = reinterpret_cast<uint8_t*>(s_level + nload);
+  const int64_t nump = npart_p[0];
+  const int lb_l = lvl_bound[0], lb_r = lvl_bound[1];
+  const int64_t node_idx = (int64_t)blockIdx.x * kTreeThreads + threadIdx.x;
+  const int64_t ioff = (int64_t)blockIdx.x * kTreeThreads - scan_size - 1;
+
+  // s_level[i] = level of boundary ioff+i+1, i.e. between particles ioff+i and ioff+i+1
+  for (int i = threadIdx.x; i < nload - 1; i += kTreeThreads) {
+    int64_t ip = ioff + i;
+    int L;
+    uint8_t ty = 0;
+    if (ip < 0) {
+      L = lb_l;
+    } else if (ip + 1 >= nump) {
+      L = lb_r;
+    } else {
+      L = msb_diff_level<D, T>(load_vec<D, T>(posz, ip), load_vec<D, T>(posz, ip + 1));
+      if (num_types > 1) ty = ptype[ip];
+    }
+    s_level[i] = L;
+    s_types[i] = ty;
+  }
+  __syncthreads();
+
+  const int idx = threadIdx.x + scan_size;
+  const int my = s_level[idx];
+  bool is_split = false;
+  for (int type = 0; type < num_types; type++) {
+    int lsize = 0;
+    for (int i = idx - scan_size; i < idx; i++) lsize = (s_level[i] < my ? lsize : 0) + (s_types[i] == type ? 1 : 0);
+    int rsize = 0;
+    for (int i = idx + scan_size; i > idx; i--) rsize = (s_level[i] < my ? rsize : 0) + (s_types[i] == type ? 1 : 0);
+    is_split |= (lsize + rsize > max_size);
+    is_split |= (my > lb_l) && (node_idx - lsize <= 0);
+    is_split |= (my > lb_r) && (node_idx + rsize >= nump);
+  }
+  is_split &= node_idx <= nump;
+  is_split |= node_idx == 0;
+  is_split |= node_idx == nump;
+  if (node_idx < size_part + 1) {
+    split_flags[node_idx] = (int8_t)is_split;
+    lvl[node_idx] = my;
+  }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(kTreeThreads) void k_find_node_bounds(const T* __restrict__ pos_in,
+                                                                   const T* __restrict__ pos_boundary,
+                                                                   const int32_t* __restrict__ nleaves_p,
+                                                                   int32_t* __restrict__ levels, int32_t* __restrict__ lbound,
+                                                                   int32_t* __restrict__ rbound, int64_t size_nodes,
+                                                                   int lvl_invalid) {
+  int64_t idx = (int64_t)blockIdx.x * kTreeThreads + threadIdx.x;
+  const int n = nleaves_p[0];
+  const int nnodes = n + 1;
+  if (idx >= size_nodes) return;
+  if (idx >= nnodes) {
+    levels[idx] = lvl_invalid;
+    lbound[idx] = n;
+    rbound[idx] = n;
+    return;
+  }
+  Vec<D, T> p1 = (idx == 0) ? load_vec<D, T>(pos_boundary, 0) : load_vec<D, T>(pos_in, idx - 1);
+  Vec<D, T> p2 = (idx == nnodes - 1) ? load_vec<D, T>(pos_boundary, 1) : load_vec<D, T>(pos_in, idx);
+  const int target = msb_diff_level<D, T>(p1, p2);
+  if (n <= 0) {  // no leaves: the single boundary spans nothing (the reference would read pos_in[-1])
+    levels[idx] = target;
+    lbound[idx] = 0;
+    rbound[idx] = 0;
+    return;
+  }
+  int lb, rb;
+  if (msb_diff_level<D, T>(load_vec<D, T>(pos_in, 0), p2) <= target) {
+    lb = 0;
+  } else {
+    int ibefore = 0, iinside = (int)idx;
+    while (ibefore + 1 < iinside) {
+      int it = (ibefore + iinside) / 2;
+      if (msb_diff_level<D, T>(load_vec<D, T>(pos_in, it), p2) > target) ibefore = it;
+      else iinside = it;
+    }
+    lb = iinside;
+  }
+  if (msb_diff_level<D, T>(p1, load_vec<D, T>(pos_in, n - 1)) <= target) {
+    rb = n;
+  } else {
+    int iinside = (int)idx - 1, iafter = n - 1;
+    while (iinside + 1 < iafter) {
+      int it = (iinside + iafter) / 2;
+      if (msb_diff_level<D, T>(p1, load_vec<D, T>(pos_in, it)) > target) iafter = it;
+      else iinside = it;
+    }
+    rb = iafter;
+  }
+  levels[idx] = target;
+  lbound[idx] = lb;
+  rbound[idx] = rb;
+}
+
+template <typename T> __device__ __forceinline__ T quiet_nan();
+template <> __device__ __forceinline__ float quiet_nan<float>() { return __uint_as_float(0x7fc00000u); }
+template <> __device__ __forceinline__ double quiet_nan<double>() { return __longlong_as_double(0x7ff8000000000000ll); }
+
+template <int D, typename T>
+__global__ __launch_bounds__(kTreeThreads) void k_node_geometry(const T* __restrict__ pos, const int32_t* __restrict__ lbound,
+                                                                const int32_t* __restrict__ rbound,
+                                                                const int32_t* __restrict__ nnodes_p,
+                                                                int32_t* __restrict__ level, T* __restrict__ center,
+                                                                T* __restrict__ extent, int64_t size_nodes,
+                                                                int64_t size_part, int lvl_invalid, uint32_t mode,
+                                                                int upper) {
+  int64_t idx = (int64_t)blockIdx.x * kTreeThreads + threadIdx.x;
+  if (idx >= size_nodes) return;
+  if (idx >= nnodes_p[0]) {
+    if (mode & 1u) level[idx] = lvl_invalid;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (mode & 2u) center[idx * D + d] = quiet_nan<T>();
+      if (mode & 4u) extent[idx * D + d] = T(0);
+    }
+    return;
+  }
+  auto clip = [&](int64_t a) { return a < 0 ? (int64_t)0 : (a > size_part - 1 ? size_part - 1 : a); };
+  const int64_t lb = lbound[idx], rb = rbound[idx];
+  Vec<D, T> x0b = load_vec<D, T>(pos, clip(lb));
+  Vec<D, T> x1a = load_vec<D, T>(pos, clip(rb - 1));
+  int lvl;
+  if (!upper) {
+    lvl = msb_diff_level<D, T>(x0b, x1a);
+  } else {
+    Vec<D, T> x0a = load_vec<D, T>(pos, clip(lb - 1));
+    Vec<D, T> x1b = load_vec<D, T>(pos, clip(rb));
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (lb - 1 < 0) x0a[d] = -INFINITY;
+      if (rb >= size_part) x1b[d] = INFINITY;
+    }
+    lvl = min(msb_diff_level<D, T>(x0a, x0b), msb_diff_level<D, T>(x1a, x1b)) - 1;
+  }
+  int l[D];
+  lvl_vec<D>(lvl, l);
+  if (mode & 1u) level[idx] = lvl;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    if (mode & 2u) center[idx * D + d] = round_pow2_cent(x0b[d], l[d]);
+    if (mode & 4u) extent[idx * D + d] = pow2<T>(T(1), l[d]);
+  }
+}
+
+template <int D, typename T>
+int flag_leaf_impl(hipStream_t s, const void* posz, const uint8_t* ptype, const int32_t* lvl_bound, const int32_t* npart,
+                   int8_t* flags, int32_t* lvl, int64_t size_part, int max_size, int scan_size, int num_types) {
+  int64_t nb = size_part + 1;
+  int nload = kTreeThreads + 2 * scan_size + 1;
+  size_t shmem = (size_t)nload * (sizeof(int32_t) + 1);
+  if (shmem > 60000) return fail("FlagLeafBoundaries: scan_size too large");
+  hipLaunchKernelGGL((k_flag_leaf<D, T>), dim3((unsigned)div_up(nb, kTreeThreads)), dim3(kTreeThreads), shmem, s,
+                     (const T*)posz, ptype, lvl_bound, npart, flags, lvl, size_part, max_size, scan_size, num_types);
+  return check_launch("FlagLeafBoundaries");
+}
+
+template <int D, typename T>
+int find_node_impl(hipStream_t s, const void* pos_in, const void* pos_boundary, const int32_t* nleaves, int32_t* lv,
+                   int32_t* lb, int32_t* rb, int64_t size_nodes, int lvl_invalid) {
+  hipLaunchKernelGGL((k_find_node_bounds<D, T>), dim3((unsigned)div_up(size_nodes, kTreeThreads)), dim3(kTreeThreads),
+                     0, s, (const T*)pos_in, (const T*)pos_boundary, nleaves, lv, lb, rb, size_nodes, lvl_invalid);
+  return check_launch("FindNodeBoundaries");
+}
+
+template <int D, typename T>
+int geometry_impl(hipStream_t s, const void* pos, const int32_t* lbound, const int32_t* rbound, const int32_t* nnodes,
+                  int32_t* level, void* center, void* extent, int64_t size_nodes, int64_t size_part, int lvl_invalid,
+                  uint32_t mode, int upper) {
+  hipLaunchKernelGGL((k_node_geometry<D, T>), dim3((unsigned)div_up(size_nodes, kTreeThreads)), dim3(kTreeThreads), 0,
+                     s, (const T*)pos, lbound, rbound, nnodes, level, (T*)center, (T*)extent, size_nodes, size_part,
+                     lvl_invalid, mode, upper);
+  return check_launch("GetNodeGeometry");
+}
+
+// ------------------------------------------------------------------------------------------------
+// regularization level (tree.py:55-68 _level_histogram, :160-187 find_regularization_level)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_reg_hist(const int32_t* __restrict__ lvl, const int32_t* __restrict__ w, int64_t n, int min_lvl,
+                           int nlvl, unsigned long long* __restrict__ hist) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b = lvl[i] - min_lvl;
+  b = b < 0 ? 0 : b;  // jnp.bincount clips below and drops above
+  int64_t wi = w ? (int64_t)w[i] : 1;
+  if (b < nlvl && wi != 0) atomicAdd(&hist[b], (unsigned long long)wi);
+}
+
+// FP32 evaluation in ascending level order by one thread (the reference reduces under XLA's
+// order; only borderline ties can differ).
+__global__ void k_reg_level(const unsigned long long* __restrict__ hist, int nlvl, int min_lvl, int max_lvl, float fac,
+                            float frac, int32_t* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float total = 0.f;
+  for (int b = 0; b < nlvl; b++) total += (float)(int64_t)hist[b];
+  const float thresh = frac * total;
+  float csum = 0.f;
+  int ref = max_lvl;
+  for (int b = 0; b < nlvl; b++) {
+    csum += (float)(int64_t)hist[b];
+    if (csum >= thresh) {
+      ref = b + min_lvl;
+      break;
+    }
+  }
+  float vol_sum = 0.f, cts_sum = 0.f;
+  for (int b = 0; b < nlvl && b + min_lvl <= ref; b++) {
+    float c = (float)(int64_t)hist[b];
+    float v = ldexpf(1.0f, b + min_lvl - ref);
+    vol_sum += v * c;
+    cts_sum += c;
+  }
+  const float avg = vol_sum / cts_sum;
+  const float lim = fac * avg;
+  int res = max_lvl;
+  csum = 0.f;
+  for (int b = 0; b < nlvl; b++) {
+    csum += (float)(int64_t)hist[b];
+    float v = ldexpf(1.0f, b + min_lvl - ref);
+    if (csum >= thresh && v >= lim) {
+      res = b + min_lvl;
+      break;
+    }
+  }
+  out[0] = res;
+}
+
+}  // namespace
+}  // namespace jzt
+
+using namespace jzt;
+
+extern "C" size_t jzt_regularization_tmp_bytes(int min_lvl, int max_lvl) {
+  return (size_t)(max_lvl - min_lvl + 1) * 8;
+}
+
+extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl, const int32_t* weights, int64_t n,
+                                        int32_t* out_level, int min_lvl, int max_lvl, float max_volume_fac,
+                                        float min_percentile, void* tmp, size_t tmp_bytes) {
+  hipStream_t s = (hipStream_t)stream;
+  const int nlvl = max_lvl - min_lvl + 1;
+  if (tmp_bytes < jzt_regularization_tmp_bytes(min_lvl, max_lvl)) return fail("regularization: tmp too small");
+  if (fill_u32(s, tmp, 0u, (int64_t)nlvl * 2)) return 1;
+  if (n > 0)
+    hipLaunchKernelGGL(k_reg_hist, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, lvl, weights, n, min_lvl, nlvl,
+                       (unsigned long long*)tmp);
+  // 0.01*percentile is formed in double and rounded to float, as the reference's weak typing does
+  const float frac = (float)(0.01 * (double)min_percentile);
+  hipLaunchKernelGGL(k_reg_level, dim3(1), dim3(64), 0, s, (const unsigned long long*)tmp, nlvl, min_lvl, max_lvl,
+                     max_volume_fac, frac, out_level);
+  return check_launch("regularization level");
+}
+
+#define JZT_DISPATCH(fn, ...)                                                     \
+  do {                                                                            \
+    if (dtype == JZT_F32 && dim == 3) return fn<3, float>(__VA_ARGS__);           \
+    if (dtype == JZT_F32 && dim == 2) return fn<2, float>(__VA_ARGS__);           \
+    if (dtype == JZT_F64 && dim == 3) return fn<3, double>(__VA_ARGS__);          \
+    if (dtype == JZT_F64 && dim == 2) return fn<2, double>(__VA_ARGS__);          \
+    return fail(std::string(#fn) + ": unsupported dim/dtype");                    \
+  } while (0)
+
+extern "C" int jzt_flag_leaf_boundaries(jzt_stream_t stream, const void* posz, const uint8_t* ptype,
+                                        const int32_t* lvl_bound, const int32_t* npart, int8_t* split_flags,
+                                        int32_t* lvl, int64_t size_part, int dim, int dtype, int max_size,
+                                        int scan_size, int num_types) {
+  if (num_types > 1 && ptype == nullptr) return fail("FlagLeafBoundaries: ptype required for num_types > 1");
+  JZT_DISPATCH(flag_leaf_impl, (hipStream_t)stream, posz, ptype, lvl_bound, npart, split_flags, lvl, size_part,
+               max_size, scan_size, num_types);
+}
+
+extern "C" int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in, const void* pos_boundary,
+                                        const int32_t* nleaves, int32_t* nodes_levels, int32_t* nodes_lbound,
+                                        int32_t* nodes_rbound, int64_t size_nodes, int dim, int dtype, int lvl_max,
+                                        int lvl_invalid) {
+  (void)lvl_max;  // the reference passes it but its kernel never reads it outside dead stores (tree.cuh:144-196)
+  if (size_nodes <= 0) return 0;
+  JZT_DISPATCH(find_node_impl, (hipStream_t)stream, pos_in, pos_boundary, nleaves, nodes_levels, nodes_lbound,
+               nodes_rbound, size_nodes, lvl_invalid);
+}
+
+extern "C" int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* lbound, const int32_t* rbound,
+                                     const int32_t* nnodes, int32_t* level, void* center, void* extent,
+                                     int64_t size_nodes, int64_t size_part, int dim, int dtype, int lvl_invalid,
+                                     uint32_t mode_flags, int upper_extent) {
+  if (size_nodes <= 0) return 0;
+  JZT_DISPATCH(geometry_impl, (hipStream_t)stream, pos, lbound, rbound, nnodes, level, center, extent, size_nodes,
+               size_part, lvl_invalid, mode_flags, upper_extent);
+}
+
+extern "C" size_t jzt_where_tmp_bytes(int64_t n) { return (size_t)(scan_tmp_words(n) + n) * 4; }
+
+extern "C" int jzt_where_flags(jzt_stream_t stream, const int8_t* flags, int64_t n, int32_t* out, int64_t size,
+                               int32_t fill, const int32_t* fill_dev, int32_t* count, void* tmp, size_t tmp_bytes) {
+  if (tmp_bytes < jzt_where_tmp_bytes(n)) return fail("jzt_where_flags: tmp too small");
+  return where_flags_i8((hipStream_t)stream, flags, n, out, size, fill, fill_dev, count, (int32_t*)tmp);
+}

@@ -1,0 +1,137 @@
+// Device-wide scan / fill / compaction and the library's error state.
+#include <mutex>
+#include <string>
+
+#include "util.h"
+#include "../../include/jztree_amd.h"
+
+namespace jzt {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(const std::string& msg) {
+  g_err = msg;
+  return 1;
+}
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+
+int fill_u32(hipStream_t s, void* p, uint32_t v, int64_t n) {
+  if (n <= 0) return 0;
+  int64_t blocks = div_up(n, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)p, v, n);
+  return check_launch("fill_u32");
+}
+
+// pass 1: per-tile sums
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                              int32_t* __restrict__ tile_sums) {
+  __shared__ int sbuf[kScanThreads / 64];
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int v = 0;
+#pragma unroll
+  for (int r = 0; r < kScanItems; r++) {
+    int64_t i = base + (int64_t)r * kScanThreads + threadIdx.x;
+    v += (i < n) ? in[i] : 0;
+  }
+  int tot;
+  block_excl_scan<kScanThreads>(v, sbuf, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// pass 2: exclusive scan of the tile sums by one workgroup, in chunks with a running carry
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int32_t* __restrict__ tile_sums, int64_t ntiles) {
+  __shared__ int sbuf[kScanThreads / 64];
+  int carry = 0;
+  for (int64_t c = 0; c < ntiles; c += kScanThreads) {
+    int64_t i = c + threadIdx.x;
+    int v = (i < ntiles) ? tile_sums[i] : 0;
+    int tot;
+    int ex = block_excl_scan<kScanThreads>(v, sbuf, &tot);
+    if (i < ntiles) tile_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) tile_sums[ntiles] = carry;
+}
+
+// pass 3: scan within each tile (blocked per thread) and add the tile offset
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int32_t* in, int32_t* out, int64_t n,
+                                                             const int32_t* __restrict__ tile_off, int inclusive) {
+  __shared__ int sbuf[kScanThreads / 64];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int x[kScanItems];
+  int v = 0;
+#pragma unroll
+  for (int r = 0; r < kScanItems; r++) {
+    int64_t i = base + r;
+    x[r] = (i < n) ? in[i] : 0;
+    v += x[r];
+  }
+  int tot;
+  int ex = block_excl_scan<kScanThreads>(v, sbuf, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < kScanItems; r++) {
+    int64_t i = base + r;
+    int nv = ex + x[r];
+    if (i < n) out[i] = inclusive ? nv : ex;
+    ex = nv;
+  }
+}
+
+int scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, bool inclusive, int32_t* tmp) {
+  if (n <= 0) return 0;
+  int64_t nt = div_up(n, kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, n, tmp);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, tmp, nt);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, out, n, tmp,
+                     inclusive ? 1 : 0);
+  return check_launch("scan_i32");
+}
+
+__global__ void k_flags_to_i32(const int8_t* __restrict__ f, int32_t* __restrict__ o, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = f[i] != 0;
+}
+
+__global__ void k_where_scatter(const int8_t* __restrict__ f, const int32_t* __restrict__ ex, int64_t n,
+                                int32_t* __restrict__ out, int64_t size, int32_t fill, const int32_t* fill_dev,
+                                int32_t* count_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fill_dev) fill = fill_dev[0];
+  int32_t total = ex[n - 1] + (f[n - 1] != 0);
+  if (i < n && f[i] != 0) {
+    int32_t j = ex[i];
+    if (j < size) out[j] = (int32_t)i;
+  }
+  // the remaining slots get the fill value
+  if (i < size && i >= total) out[i] = fill;
+  if (i == 0 && count_out) *count_out = total;
+}
+
+int where_flags_i8(hipStream_t s, const int8_t* flags, int64_t n, int32_t* out, int64_t size, int32_t fill,
+                   const int32_t* fill_dev, int32_t* count_out, int32_t* tmp) {
+  if (n <= 0) return fail("where_flags_i8: empty input");
+  int32_t* ex = tmp + scan_tmp_words(n);
+  hipLaunchKernelGGL(k_flags_to_i32, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, flags, ex, n);
+  if (scan_i32(s, ex, ex, n, false, tmp)) return 1;
+  int64_t m = n > size ? n : size;
+  hipLaunchKernelGGL(k_where_scatter, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, flags, ex, n, out, size,
+                     fill, fill_dev, count_out);
+  return check_launch("where_flags_i8");
+}
+
+}  // namespace jzt
+
+extern "C" const char* jzt_last_error(void) { return jzt::g_err.c_str(); }
+extern "C" int jzt_abi_version(void) { return JZT_ABI_VERSION; }

@@ -1,0 +1,294 @@
+"""Particle and tree data structures (torch tensors on the GPU).
+
+Mirrors src/jztree/data.py: the particle dataclasses (:25-63), the accessors (:69-94),
+PackedArray (:251-345), LevelInfo (:352-379), TreeHierarchy (:381-452), InteractionList
+(:458-514) and RankIdx (:681-686).  Array shapes and contents are those of the reference; the
+containers hold torch tensors instead of jax arrays.  Sizes that JAX keeps static are Python
+ints here; counts that JAX keeps traced stay device tensors (no host synchronisation).
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+from typing import Any, List
+
+import numpy as np
+import torch
+
+
+def same_width_int(dtype: torch.dtype) -> torch.dtype:
+    """data.py:13-19"""
+    size = torch.empty((), dtype=dtype).element_size()
+    if size == 4:
+        return torch.int32
+    if size == 8:
+        return torch.int64
+    raise ValueError(f"unsupported type {dtype}")
+
+
+# ------------------------------------------------------------------------------------------------
+# user-facing particle containers
+# ------------------------------------------------------------------------------------------------
+
+@dataclass(kw_only=True)
+class Pos:
+    """Positions (size, dim); num/num_total only matter for padded or distributed data."""
+    pos: torch.Tensor
+    num: Any = None
+    num_total: int | None = None
+
+
+@dataclass(kw_only=True)
+class PosMass:
+    pos: torch.Tensor
+    mass: torch.Tensor
+    num: Any = None
+    num_total: int | None = None
+
+
+@dataclass(kw_only=True)
+class ParticleData:
+    pos: torch.Tensor
+    mass: Any = None
+    vel: torch.Tensor | None = None
+    id: torch.Tensor | None = None
+    num: Any = None
+    num_total: int | None = None
+
+
+@dataclass
+class RankIdx:
+    """Rank and index of a particle's origin (data.py:681-686)."""
+    rank: torch.Tensor | None
+    idx: torch.Tensor
+
+
+def get_pos(part) -> torch.Tensor:
+    if torch.is_tensor(part):
+        assert part.shape[-1] <= 10 and part.ndim == 2
+        return part
+    if hasattr(part, "pos"):
+        assert part.pos.ndim == 2
+        return part.pos
+    raise ValueError("Invalid input particles")
+
+
+def get_num(part, default_to_length=False, default_to_nancount=True):
+    if getattr(part, "num", None) is None:
+        if default_to_length:
+            return len(get_pos(part))
+        if default_to_nancount:
+            return torch.sum(~torch.isnan(get_pos(part)[:, 0])).to(torch.int32)
+        raise ValueError("Need .num attribute in particle structure for distributed mode")
+    num = part.num
+    return torch.sum(num) if torch.is_tensor(num) else num
+
+
+def get_num_total(part, default_to_length=False):
+    num = getattr(part, "num_total", None)
+    if num is not None:
+        return num
+    if default_to_length:
+        return len(get_pos(part))
+    raise ValueError("Need num_total attribute on particle data structure (for distributed mode)")
+
+
+def tree_map_by_len(fn, tree, length: int):
+    """Apply fn to every tensor leaf whose leading dimension equals `length` (jax_ext.py:58-71)."""
+    if tree is None:
+        return None
+    if torch.is_tensor(tree):
+        return fn(tree) if (tree.ndim > 0 and tree.shape[0] == length) else tree
+    if isinstance(tree, (tuple, list)):
+        return type(tree)(tree_map_by_len(fn, t, length) for t in tree)
+    if isinstance(tree, dict):
+        return {k: tree_map_by_len(fn, v, length) for k, v in tree.items()}
+    if dataclasses.is_dataclass(tree):
+        vals = {f.name: tree_map_by_len(fn, getattr(tree, f.name), length) for f in dataclasses.fields(tree)}
+        return dataclasses.replace(tree, **vals)
+    return tree
+
+
+# ------------------------------------------------------------------------------------------------
+# PackedArray
+# ------------------------------------------------------------------------------------------------
+
+@dataclass
+class PackedArray:
+    """Several per-plane arrays stacked in one buffer (data.py:251-345).
+
+    data: the buffer, ispl: (levels+1,) device start offsets, fill_values: (levels,) device values
+    returned for out-of-range reads."""
+    data: torch.Tensor
+    ispl: torch.Tensor
+    fill_values: torch.Tensor | None = None
+    levels_filled: torch.Tensor | None = None
+
+    @classmethod
+    def from_data(cls, data, ispl, fill_values=None, levels_filled=None):
+        if levels_filled is None:
+            levels_filled = torch.full((1,), len(ispl) - 1, dtype=torch.int32, device=data.device)
+        if fill_values is not None and not torch.is_tensor(fill_values):
+            fill_values = torch.full((len(ispl) - 1,), fill_values, dtype=data.dtype, device=data.device)
+        elif fill_values is not None:
+            fill_values = fill_values.to(data.dtype)
+        return cls(data, ispl, fill_values, levels_filled)
+
+    def get(self, level, size=None, fill_value=None):
+        if size is None:
+            size = self.size()
+        idx = torch.arange(size, dtype=torch.int64, device=self.data.device) + self.ispl[level]
+        valid = idx < self.ispl[level + 1]
+        vals = self.data[torch.clamp(idx, max=len(self.data) - 1)]
+        if fill_value is None:
+            fill = self.fill_values[level]
+        else:
+            fill = torch.as_tensor(fill_value, dtype=self.data.dtype, device=self.data.device)
+        if self.data.ndim > 1:
+            valid = valid.reshape((-1,) + (1,) * (self.data.ndim - 1))
+        return torch.where(valid, vals, fill)
+
+    def set(self, level, values, num=None, fill_value=None):
+        """data.py:300-307 (values written at ispl[level], later levels start after them)."""
+        if num is None:
+            num = values.shape[0]
+        lvl_idx = torch.arange(len(self.ispl), device=self.ispl.device)
+        start = self.ispl[level]
+        new_spl = torch.where(lvl_idx <= level, self.ispl, start + num).to(self.ispl.dtype)
+        new_data = self.data.clone()
+        n = values.shape[0]
+        pos = start + torch.arange(n, device=self.data.device)
+        keep = torch.arange(n, device=self.data.device) < num
+        dump = torch.cat([new_data, new_data[:1]])  # one spare slot for masked writes
+        tgt = torch.where(keep & (pos < len(self.data)), pos, torch.full_like(pos, len(self.data)))
+        dump[tgt] = values.to(self.data.dtype)
+        new_data = dump[:len(self.data)]
+        fv = self.fill_values
+        if fill_value is not None:
+            fv = fv.clone()
+            fv[level] = torch.as_tensor(fill_value).to(fv.dtype)
+        lf = torch.full((1,), level + 1, dtype=torch.int32, device=self.data.device)
+        return PackedArray(new_data, new_spl, fv, lf)
+
+    def append(self, values, num=None, fill_value=None, resize=False):
+        arr = self.resize_levels(self.nlevels() + 1) if resize else self
+        return arr.set(self.nlevels(), values, num, fill_value)
+
+    def resize_levels(self, levels):
+        ispl_new = torch.full((levels + 1,), 0, dtype=self.ispl.dtype, device=self.ispl.device)
+        ispl_new[:] = self.ispl[-1]
+        ispl_new[:len(self.ispl)] = self.ispl
+        fv = self.fill_values
+        if fv is not None:
+            reps = int(np.ceil(levels / max(len(fv), 1)))
+            fv = fv.repeat(reps)[:levels]  # jnp.resize semantics
+        return PackedArray(self.data, ispl_new, fv, self.levels_filled)
+
+    def size(self):
+        return len(self.data)
+
+    def num(self, level):
+        return self.ispl[level + 1] - self.ispl[level]
+
+    def nfilled(self):
+        return self.ispl[-1]
+
+    def nlevels(self):
+        return len(self.ispl) - 1
+
+
+# ------------------------------------------------------------------------------------------------
+# tree structure
+# ------------------------------------------------------------------------------------------------
+
+def _min_max_msb_diff(dtype):
+    if dtype == torch.float32:
+        return -150, 128
+    if dtype == torch.float64:
+        return -1075, 1024
+    if dtype == torch.int32:
+        return -1, 32
+    if dtype == torch.int64:
+        return -1, 64
+    raise ValueError(dtype)
+
+
+@dataclass(frozen=True)
+class LevelInfo():
+    """Minimum and maximum Morton level of a tree (data.py:352-379)."""
+    dim: int
+    dtype: torch.dtype
+
+    def min_lvl(self) -> int:
+        return self.dim * _min_max_msb_diff(self.dtype)[0]
+
+    def max_lvl(self) -> int:
+        return self.dim * (_min_max_msb_diff(self.dtype)[1] + 1)
+
+
+@dataclass
+class TreeHierarchy():
+    """The tree-plane hierarchy (data.py:381-452); see the reference for the field meanings."""
+    size_leaves: int
+    ispl_n2n: PackedArray
+    ispl_n2l: PackedArray
+    ispl_l2p_per_type: torch.Tensor
+    lvl: PackedArray
+    geom_cent: PackedArray
+    mass: PackedArray | None = None
+    mass_cent: PackedArray | None = None
+
+    def splits_leaf_to_part(self, ptype: int = 0, size: int | None = None) -> torch.Tensor:
+        if size is None:
+            size = self.size() + 1
+        return self.ispl_l2p_per_type[ptype][:size]
+
+    def npart(self, level: int, ptype: int = 0, size=None) -> torch.Tensor:
+        if size is None:
+            size = self.size()
+        spl = self.splits_leaf_to_part(ptype)
+        n2l = self.ispl_n2l.get(level, size + 1).long()
+        n2p = spl[torch.clamp(n2l, max=len(spl) - 1)]
+        return n2p[1:] - n2p[:-1]
+
+    def center(self) -> PackedArray:
+        return self.mass_cent if self.mass_cent is not None else self.geom_cent
+
+    def num_planes(self) -> int:
+        return len(self.ispl_n2l.ispl) - 1
+
+    def num(self, level):
+        return self.lvl.num(level)
+
+    def size(self) -> int:
+        return self.ispl_n2n.size() - 1
+
+    def info(self) -> LevelInfo:
+        return LevelInfo(self.geom_cent.data.shape[-1], self.geom_cent.data.dtype)
+
+
+@dataclass
+class InteractionList:
+    """Receiver i interacts with sources isrc[ispl[i]:ispl[i+1]] (data.py:458-507)."""
+    ispl: torch.Tensor
+    isrc: torch.Tensor
+    rad2: torch.Tensor | None = None
+    ids: torch.Tensor | None = None
+    dev_spl: torch.Tensor | None = None
+
+    def nfilled(self):
+        return self.ispl[-1]
+
+    def size(self):
+        return self.isrc.numel()
+
+    def dtype(self):
+        return self.isrc.dtype
+
+
+def verify_ilist(ilist: InteractionList) -> InteractionList:
+    """data.py:509-514: int32 interaction lists must stay below 2**31 entries."""
+    if ilist.isrc.numel() >= 2 ** 31:
+        raise ValueError("Intercation list allocation too large and may produce integer overflows",
+                         "Hint: Use more GPUs or decrease alloc_fac_ilist (if possible)")
+    return ilist

@@ -1,0 +1,166 @@
+"""Friends-of-friends labels (the API of src/jztree/fof.py, single GPU).
+
+Pipeline (fof.py:772-816): z-sort + tree -> dual walk over the planes (NodeToChildLabel,
+FofNode2Node: guaranteed node links + lists of uncertain node pairs) -> particle level
+(NodeToChildLabel, FofLeaf2Leaf).  Labels in z-order are the minimum z-index of each group.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import checks
+from .checks import raise_if
+from .config import FofConfig
+from .data import InteractionList, PackedArray, TreeHierarchy, get_pos, verify_ilist
+from .tree import build_tree_hierarchy, grouped_dense_interaction_list, zsort
+
+
+@dataclass
+class PosLvl:
+    """Node positions and levels (data.py:199-212)."""
+    pos: torch.Tensor
+    lvl: torch.Tensor
+
+    def pos_lvl(self) -> torch.Tensor:
+        if self.pos.dtype == torch.float32:
+            lvl_t = self.lvl.to(torch.int32).view(torch.float32)
+        else:
+            lvl_t = self.lvl.to(torch.int64).view(torch.float64)
+        return torch.cat([self.pos, lvl_t[:, None]], dim=-1).contiguous()
+
+
+@dataclass
+class FofNodeData:
+    """data.py:566-570"""
+    lvl: torch.Tensor
+    igroup: torch.Tensor
+    spl: torch.Tensor
+
+
+def _r2link(rlink: float) -> float:
+    """fof.py:50, :101, :165: np.float32(rlink*rlink), the product formed in float64."""
+    return float(np.float32(rlink * rlink))
+
+
+def _fof_node2node(ilist: InteractionList, spl_parent, node_data: PosLvl, node_igroup, rlink: float,
+                   boxsize: float = 0., block_size: int = 32):
+    """fof.py:31-70 -> FofNode2Node.  Returns (node_igroup, node_ilist)."""
+    assert ilist.ispl.shape[0] == spl_parent.shape[0]
+    assert len(node_data.lvl) == len(node_igroup)
+    dev = spl_parent.device
+    size = len(node_data.pos)
+    ig = node_igroup.to(torch.int32).contiguous()
+    il_spl = torch.empty(size + 1, dtype=torch.int32, device=dev)
+    il = torch.empty(ilist.size(), dtype=torch.int32, device=dev)
+    L.call("jzt_fof_node2node", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(spl_parent.contiguous()),
+           L.ptr(node_data.pos_lvl()), L.ptr(ig), L.ptr(ig), L.ptr(il_spl), L.ptr(il), spl_parent.shape[0] - 1,
+           size, ilist.size(), ilist.size(), _r2link(rlink), float(boxsize or 0.), node_data.pos.shape[-1],
+           L.dtype_code(node_data.pos.dtype))
+    node_ilist = verify_ilist(InteractionList(il_spl, il))
+    raise_if(node_ilist.nfilled() > il.numel(),
+             "The interaction list allocation is too small. (need: {n1} have: {n2})\n"
+             "Hint: Increase alloc_fac_ilist", n1=node_ilist.nfilled(), n2=il.numel())
+    return ig, node_ilist
+
+
+def _node_to_child_label(igroup, lvl, spl, rlink: float, dim: int, num=None, size_child: int | None = None,
+                         flag_local=None, block_size: int = 64, dtype=torch.float32):
+    """fof.py:85-109 -> NodeToChildLabel."""
+    assert len(spl) == len(igroup) + 1 == len(lvl) + 1
+    dev = igroup.device
+    if size_child is None:
+        size_child = igroup.numel()
+    out = torch.empty(size_child, dtype=torch.int32, device=dev)
+    loc = None if flag_local is None else flag_local.to(torch.uint8).contiguous()
+    L.call("jzt_node_to_child_label", L.stream(), L.ptr(igroup.to(torch.int32).contiguous()), L.ptr(loc),
+           L.ptr(lvl.to(torch.int32).contiguous()), L.ptr(spl.to(torch.int32).contiguous()), L.ptr(out),
+           igroup.numel(), size_child, _r2link(rlink), int(dim), L.dtype_code(dtype))
+    if num is not None:
+        out = torch.where(torch.arange(size_child, device=dev) < num, out, torch.zeros_like(out))
+    return out
+
+
+def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_fac_ilist: float = 128):
+    """fof.py:111-151: plane-by-plane node linking and interaction lists."""
+    nplanes = th.num_planes()
+    size = th.size()
+    spl, ilist, nsup = grouped_dense_interaction_list(th.lvl.num(nplanes - 1),
+                                                      int(th.size_leaves * alloc_fac_ilist), ngroup=32,
+                                                      size_super=size)
+    dev = spl.device
+    igroup = torch.arange(size, dtype=torch.int32, device=dev)
+    spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
+    raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
+    max_lvl = th.info().max_lvl()
+    nlvl = th.lvl.append(torch.full_like(igroup, max_lvl), nsup, max_lvl, resize=True)
+    dim = th.geom_cent.data.shape[-1]
+    for i in range(nplanes):
+        level = nplanes - 1 - i
+        parent_spl = spl_n2n.get(level + 1, size + 1)
+        igroup = _node_to_child_label(igroup, nlvl.get(level + 1, size), parent_spl, num=th.num(level), rlink=rlink,
+                                      dim=dim, dtype=th.geom_cent.data.dtype)
+        node_data = PosLvl(pos=th.geom_cent.get(level, size), lvl=nlvl.get(level, size))
+        igroup, ilist = _fof_node2node(ilist, parent_spl, node_data, igroup, rlink=rlink, boxsize=boxsize)
+    node_data = FofNodeData(nlvl.get(0, size), igroup, spl_n2n.get(0, size + 1))
+    return node_data, ilist
+
+
+def _fof_leaf2leaf(leaf_data: FofNodeData, ilist: InteractionList, posz, rlink: float, boxsize: float = 0.,
+                   block_size=32):
+    """fof.py:153-169 -> NodeToChildLabel (leaf -> particle) + FofLeaf2Leaf."""
+    assert len(leaf_data.spl) == len(ilist.ispl) == len(leaf_data.igroup) + 1 == len(leaf_data.lvl) + 1
+    dev = posz.device
+    part_igroup = _node_to_child_label(leaf_data.igroup, leaf_data.lvl, leaf_data.spl, rlink=rlink,
+                                       size_child=len(posz), num=leaf_data.spl[-1], dim=posz.shape[-1],
+                                       dtype=posz.dtype)
+    out = torch.empty(len(posz), dtype=torch.int32, device=dev)
+    L.call("jzt_fof_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
+           L.ptr(leaf_data.spl.contiguous()),
+           L.ptr(posz.contiguous()), L.ptr(part_igroup), L.ptr(out), len(leaf_data.spl) - 1, len(posz),
+           _r2link(rlink), float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    return out
+
+
+def fof_is_superset(igroup_sup, igroup, mask=None):
+    """fof.py:203-218: every group of igroup_sup is a union of groups of igroup."""
+    label_map = torch.zeros(len(igroup), dtype=torch.int64, device=igroup.device)
+    label_map[igroup.long()] = igroup_sup.long()
+    ok = igroup_sup.long() == label_map[igroup.long()]
+    if mask is not None:
+        ok = ok | ~mask
+    return bool(torch.all(ok))
+
+
+def fof_labels(part, rlink: float, boxsize: float = 0., cfg: FofConfig = FofConfig(), th: TreeHierarchy | None = None,
+               output_order: str = "z"):
+    """Friends-of-friends groups on one GPU (fof.py:772-816).
+
+    Returns (part, igroup): with output_order="z" the z-ordered particles and labels pointing to
+    the first z-order particle of each group; with "input" the input particles and labels in
+    input order (the input index of the group's first z-order member)."""
+    try:
+        if th is None:
+            partz, idz = zsort(part)
+            th = build_tree_hierarchy(partz, cfg_tree=cfg.tree)
+        else:
+            partz, idz = part, None
+        node_data, ilist = _fof_dual_walk(th, rlink=rlink, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
+        igroupz = _fof_leaf2leaf(node_data, ilist, get_pos(partz), rlink=rlink, boxsize=boxsize)
+        checks.flush()
+    except BaseException:
+        checks.clear()
+        raise
+    if output_order == "input" and idz is not None:
+        idl = idz.long()
+        igroup = torch.empty_like(igroupz)
+        igroup[idl] = idz[igroupz.long()].to(igroupz.dtype)
+        return part, igroup
+    return partz, igroupz
+
+
+fof_labels.jit = fof_labels

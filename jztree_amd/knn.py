@@ -1,0 +1,195 @@
+"""Exact k-nearest-neighbour search (the API of src/jztree/knn.py, single GPU).
+
+Pipeline (knn.py:198-356): z-sort + tree (tree.py) -> dual tree walk that refines a dense
+top-level interaction list plane by plane into leaf-leaf lists (KnnNode2Node) -> leaf-vs-leaf
+top-k (KnnLeaf2Leaf).  Neighbours come out ascending by (canonical FP32 r2, reported index).
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass
+from typing import Any, Callable, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import checks
+from .checks import raise_if
+from .config import KNNConfig
+from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
+from .tools import masked_inverse
+from .tree import _tmp, grouped_dense_interaction_list, zsort_and_tree
+
+
+@dataclass
+class PosLvlNum:
+    """Node data for the kNN walk (data.py:214-239)."""
+    pos: torch.Tensor
+    lvl: torch.Tensor
+    npart: torch.Tensor
+
+    def pos_lvl(self) -> torch.Tensor:
+        """(size, dim+1): positions with the level bit-cast into the last column."""
+        if self.pos.dtype == torch.float32:
+            lvl_t = self.lvl.to(torch.int32).view(torch.float32)
+        else:
+            lvl_t = self.lvl.to(torch.int64).view(torch.float64)
+        return torch.cat([self.pos, lvl_t[:, None]], dim=-1).contiguous()
+
+
+def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, boxsize=0., src_ids=None):
+    """knn.py:28-60 -> KnnLeaf2Leaf.  Returns (rnn (nq,k), inn (nq,k) int32).
+
+    src_ids (extension): id reported for each source particle; default its z-index."""
+    if xQ is None:
+        xQ = xT
+    if splQ is None:
+        splQ = splT
+    assert ilist.rad2.shape == ilist.isrc.shape
+    assert xT.dtype == xQ.dtype and xT.shape[-1] == xQ.shape[-1]
+    assert splT.dtype == splQ.dtype == torch.int32
+    assert len(ilist.ispl) == len(splQ)
+    dev = xQ.device
+    nq = xQ.shape[0]
+    dt = L.dtype_code(xT.dtype)
+    rnn = torch.empty((nq, k), dtype=xT.dtype, device=dev)
+    inn = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    nb = L.LIB.jzt_knn_leaf2leaf_tmp_bytes(nq, k, dt)
+    tmp = _tmp(nb, dev)
+    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+    L.call("jzt_knn_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2), ilist.size(),
+           L.ptr(splT),
+           L.ptr(xT.contiguous()), L.ptr(splQ), L.ptr(xQ.contiguous()), L.ptr(sid), L.ptr(rnn), L.ptr(inn),
+           L.ptr(tmp), nb, len(splQ) - 1, nq, int(k), float(boxsize or 0.), xT.shape[-1], dt)
+    return rnn, inn
+
+
+_knn_leaf2leaf.jit = _knn_leaf2leaf
+
+
+def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNum, k: int = 32,
+                         boxsize: float = 0.) -> InteractionList:
+    """knn.py:62-110 -> KnnNode2Node."""
+    boxsize = 0. if boxsize is None else boxsize
+    assert ilist.ispl.shape[0] == spl_parent.shape[0]
+    assert ilist.isrc.shape == ilist.rad2.shape
+    dev = spl_parent.device
+    size = len(node_data.pos)
+    poslvl = node_data.pos_lvl()
+    radii = torch.empty(size, dtype=torch.float32, device=dev)
+    ispl = torch.empty(size + 1, dtype=torch.int32, device=dev)
+    il = torch.empty(ilist.size(), dtype=torch.int32, device=dev)
+    ilr2 = torch.empty(ilist.size(), dtype=torch.float32, device=dev)
+    npart = node_data.npart.to(torch.int32).contiguous()
+    L.call("jzt_knn_node2node", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2),
+           L.ptr(spl_parent.contiguous()), L.ptr(poslvl), L.ptr(npart), L.ptr(radii), L.ptr(ispl), L.ptr(il),
+           L.ptr(ilr2), spl_parent.shape[0] - 1, size, ilist.size(), ilist.size(), int(k), float(boxsize),
+           node_data.pos.shape[-1], L.dtype_code(node_data.pos.dtype))
+    raise_if(ispl[-1] > il.numel(),
+             "The interaction list allocation is too small. (need: {n1}, have: {n2})\n"
+             "Hint: increase alloc_fac_ilist", n1=ispl[-1], n2=il.numel())
+    return InteractionList(ispl, il, rad2=ilr2)
+
+
+def _knn_dual_walk(th: TreeHierarchy, k: int, boxsize: float | None = None,
+                   alloc_fac_ilist: float = 32.) -> InteractionList:
+    """knn.py:132-193 (single GPU): from the dense top-plane list down to leaf-leaf lists."""
+    nlevels = th.num_planes()
+    size = th.size()
+    spl, ilist, nsup = grouped_dense_interaction_list(
+        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size)
+    ilist.rad2 = torch.zeros(ilist.isrc.shape, dtype=torch.float32, device=ilist.isrc.device)
+    spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
+    raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
+    for i in range(nlevels):
+        level = nlevels - i - 1
+        parent_spl = spl_n2n.get(level + 1, size + 1)
+        node_data = PosLvlNum(pos=th.geom_cent.get(level, size), lvl=th.lvl.get(level, size),
+                              npart=th.npart(level, size=size, ptype=0))
+        ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize)
+    return ilist
+
+
+def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,
+        result: str = "rad_globalidx", reduce_func: Callable | None = None, output_order: str = "input",
+        cfg: KNNConfig = KNNConfig()):
+    """Exact k nearest neighbours (knn.py:198-347).
+
+    Returns (rnn, inn) by default: radii and indices of the k nearest neighbours of every particle,
+    ascending by (distance, index).  result: "_"-separated keys among rad, drad, rankidx,
+    globalidx, part, reduce, or a particle attribute name.  output_order: "input" or "z".
+    """
+    assert output_order in ("z", "input")
+    if part_query is not None:
+        raise NotImplementedError("separate query sets (multi-type trees) are not part of this build")
+    pos = get_pos(part)
+    dev = pos.device
+    size = len(pos)
+    try:
+        origin = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
+        num_origin = get_num(part, default_to_length=True)
+
+        if th is not None:
+            partz = part
+            originz = origin
+        else:
+            partz, originz, th = zsort_and_tree(part, cfg.tree, data=origin)
+        if output_order == "z":
+            originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
+            num_origin = len(get_pos(partz))
+
+        ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
+        spl = th.splits_leaf_to_part(ptype=0)
+        posz = get_pos(partz)
+        # neighbours are reported (and tie-broken) by their origin index directly
+        rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,
+                                     src_ids=originz.idx)
+
+        res = []
+        zidx_cache = {}
+
+        def innz():
+            if "z" not in zidx_cache:
+                inv = torch.zeros(size + 1, dtype=torch.int64, device=dev)
+                inv[originz.idx.long()] = torch.arange(size, device=dev)
+                zidx_cache["z"] = inv[inn_g.long().clamp(min=0, max=size)]
+            return zidx_cache["z"]
+
+        for key in result.split("_"):
+            if key == "rad":
+                res.append(rnnz)
+            elif key == "drad":
+                xz = posz
+                res.append(torch.linalg.norm(xz[:, None] - xz[innz()], dim=-1))
+            elif key == "rankidx":
+                res.append(RankIdx(rank=None, idx=inn_g))
+            elif key == "globalidx":
+                if get_num_total(part, default_to_length=True) >= 2 ** 31:
+                    raise ValueError("I have > 2**31 particles, globalidx will overflow. Use rankidx instead!")
+                res.append(inn_g)
+            elif key == "part":
+                res.append(tree_map_by_len(lambda v: v[innz()], partz, size))
+            elif key == "reduce":
+                assert reduce_func is not None, "Please provide reduce_func if using 'reduce'"
+                x = reduce_func(part=partz, rnn=rnnz, inn=innz(), origin=originz)
+                res.append(x)
+            elif hasattr(partz, key):
+                arr = getattr(partz, key)
+                res.append(tree_map_by_len(lambda v: v[innz()], arr, size))
+            else:
+                raise ValueError(f"Invalid result key {key}")
+        res = tuple(res) if len(res) > 1 else res[0]
+
+        if output_order == "input":
+            idx = originz.idx
+            inverse = masked_inverse(idx, mask=torch.arange(len(idx), device=dev) < num_origin)
+            res = tree_map_by_len(lambda v: v[inverse.long().clamp(max=size - 1)], res, size)
+        checks.flush()
+    except BaseException:
+        checks.clear()
+        raise
+    return res
+
+
+knn.jit = knn

@@ -1,0 +1,407 @@
+"""Z-order sort and tree hierarchy (the API of src/jztree/tree.py, single GPU).
+
+Every device computation of the reference's FFI targets runs in libjztree_amd.so (HIP, gfx950);
+the reference's jnp glue between them (compactions, prefix sums, scatters into PackedArrays) is
+written with torch GPU ops that never synchronise with the host.  Function names, arguments and
+outputs follow the reference; each docstring cites the lines it mirrors.
+"""
+from __future__ import annotations
+
+import math
+from typing import Any, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .checks import raise_if
+from .config import TreeConfig, RegularizationConfig
+from .data import (InteractionList, LevelInfo, PackedArray, Pos, TreeHierarchy, get_num, get_num_total, get_pos,
+                   same_width_int, tree_map_by_len, verify_ilist)
+from .tools import cumsum_starting_with_zero, div_ceil
+
+
+def _i32(x, device) -> torch.Tensor:
+    """A (1,) int32 device tensor holding x (python int or device scalar)."""
+    if torch.is_tensor(x):
+        return x.reshape(1).to(device=device, dtype=torch.int32)
+    return torch.tensor([int(x)], dtype=torch.int32, device=device)
+
+
+def _tmp(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+# ------------------------------------------------------------------------------------------------
+# z-order sort (tree.py:75-141)
+# ------------------------------------------------------------------------------------------------
+
+def _zsort_impl(x: torch.Tensor, block_size: int = 64):
+    """tree.py:75-93 -> PosZorderSort."""
+    assert x.ndim == 2
+    n, dim = x.shape
+    assert dim in (2, 3), "this build sorts 2D and 3D positions"
+    x = x.contiguous()
+    pos = torch.empty_like(x)
+    ids = torch.empty(n, dtype=same_width_int(x.dtype), device=x.device)
+    if n == 0:
+        return pos, ids
+    dt = L.dtype_code(x.dtype)
+    nb = L.LIB.jzt_zsort_tmp_bytes(n, dim, dt)
+    tmp = _tmp(nb, x.device)
+    L.call("jzt_pos_zorder_sort", L.stream(), L.ptr(x), L.ptr(pos), L.ptr(ids), L.ptr(tmp), nb, n, dim, dt)
+    return pos, ids
+
+
+class _ZSort(torch.autograd.Function):
+    """Custom VJP of tree.py:111-138: gradients scatter back through the inverse permutation."""
+
+    @staticmethod
+    def forward(ctx, x):
+        pos, idz = _zsort_impl(x.detach())
+        ctx.save_for_backward(idz)
+        ctx.mark_non_differentiable(idz)
+        return pos, idz
+
+    @staticmethod
+    def backward(ctx, gpos, gidz):
+        (idz,) = ctx.saved_tensors
+        idinv = torch.zeros_like(idz)
+        idinv[idz.long()] = torch.arange(len(idz), dtype=idz.dtype, device=idz.device)
+        return gpos[idinv.long()]
+
+
+def zsort(pos):
+    """Brings position vectors into z-order (tree.py:95-141).
+
+    Args:
+        pos: (N, dim) tensor, or a particle dataclass with a .pos attribute; every tensor field
+            with leading length N is sorted along with it.
+    Returns:
+        (posz, idz) with posz = pos[idz].
+    """
+    assert get_pos(pos).ndim == 2, "positions must have shape (N,3)"
+    if torch.is_tensor(pos):
+        return _ZSort.apply(pos)
+    posz, idz = _ZSort.apply(pos.pos)
+    out = tree_map_by_len(lambda v: v[idz.long()], pos, len(posz))
+    try:
+        out.pos = posz
+    except AttributeError:
+        pass
+    return out, idz
+
+
+zsort.jit = zsort
+
+
+# ------------------------------------------------------------------------------------------------
+# leaves and nodes (tree.py:160-329)
+# ------------------------------------------------------------------------------------------------
+
+def find_regularization_level(lvl: torch.Tensor, linfo: LevelInfo, weights: torch.Tensor | None = None,
+                              max_vol_fac: float = 50., min_percentile: float = 90.) -> torch.Tensor:
+    """tree.py:160-187 (with _level_histogram :55-68) -> device int32 (1,) level."""
+    dev = lvl.device
+    out = torch.empty(1, dtype=torch.int32, device=dev)
+    nb = L.LIB.jzt_regularization_tmp_bytes(linfo.min_lvl(), linfo.max_lvl())
+    tmp = _tmp(nb, dev)
+    lvl = lvl.to(torch.int32).contiguous()
+    w = None if weights is None else weights.to(torch.int32).contiguous()
+    L.call("jzt_regularization_level", L.stream(), L.ptr(lvl), L.ptr(w), lvl.numel(), L.ptr(out),
+           linfo.min_lvl(), linfo.max_lvl(), float(max_vol_fac), float(min_percentile), L.ptr(tmp), nb)
+    return out
+
+
+def _where_flags(flags: torch.Tensor, size: int, fill) -> Tuple[torch.Tensor, torch.Tensor]:
+    """jnp.where(flags, size=size, fill_value=fill)[0] as int32, plus the device count."""
+    dev = flags.device
+    n = flags.numel()
+    out = torch.empty(size, dtype=torch.int32, device=dev)
+    cnt = torch.empty(1, dtype=torch.int32, device=dev)
+    nb = L.LIB.jzt_where_tmp_bytes(n)
+    tmp = _tmp(nb, dev)
+    fill_dev = fill.reshape(1).to(torch.int32) if torch.is_tensor(fill) else None
+    fill_host = 0 if torch.is_tensor(fill) else int(fill)
+    L.call("jzt_where_flags", L.stream(), L.ptr(flags.contiguous()), n, L.ptr(out), size, fill_host,
+           L.ptr(fill_dev), L.ptr(cnt), L.ptr(tmp), nb)
+    return out, cnt[0]
+
+
+def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None, num_types: int | None = None,
+                           leaf_size: int = 32, npart=None, lvl_bound=None, block_size: int = 64,
+                           alloc_size: int | None = None, cfg_reg: RegularizationConfig | None = None) -> torch.Tensor:
+    """Leaf splits of z-sorted positions (tree.py:189-262): leaves are the largest binary Morton
+    cells holding no more than leaf_size particles of any type."""
+    n, dim = posz.shape
+    dev = posz.device
+    if alloc_size is None:
+        alloc_size = int(div_ceil(n, max(leaf_size // 2, 1))) + 1
+    if npart is None:
+        npart = n
+    linfo = LevelInfo(dim, posz.dtype)
+    if lvl_bound is None:
+        lvl_bound = (linfo.max_lvl(), linfo.max_lvl())
+    lvl_bound_t = torch.as_tensor(lvl_bound, dtype=torch.int32, device=dev).reshape(2)
+    npart_t = _i32(npart, dev)
+    if ptype is None:
+        num_types = 1
+        ptype_t = None
+    else:
+        assert num_types is not None, "Please provide num_types if providing ptype"
+        assert len(ptype) == len(posz)
+        ptype_t = ptype.to(torch.uint8).contiguous()
+    flags = torch.empty(n + 1, dtype=torch.int8, device=dev)
+    lvl = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    L.call("jzt_flag_leaf_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(ptype_t), L.ptr(lvl_bound_t),
+           L.ptr(npart_t), L.ptr(flags), L.ptr(lvl), n, dim, L.dtype_code(posz.dtype), int(leaf_size),
+           int((leaf_size + 1) * num_types), int(num_types))
+
+    if cfg_reg is not None:
+        splits, _ = _where_flags(flags, alloc_size, npart_t)
+        sl = splits.long()
+        leaf_npart = splits[1:] - splits[:-1]
+        leaf_lvl = torch.minimum(lvl[sl[1:]], lvl[sl[:-1]]) - 1
+        lvl_max = find_regularization_level(leaf_lvl, linfo, weights=leaf_npart,
+                                            max_vol_fac=cfg_reg.max_volume_fac,
+                                            min_percentile=cfg_reg.regularize_percentile)
+        ar = torch.arange(n + 1, device=dev)
+        flags = (flags.bool() | ((lvl >= lvl_max) & (ar < npart_t + 1))).to(torch.int8)
+
+    splits, nfilled = _where_flags(flags, alloc_size, npart_t)
+    raise_if(nfilled > alloc_size,
+             "Tree-Leaf allocation too small: filled={filled} size={size}.\n"
+             "Hint: Increase alloc_fac_nodes or max_leaf_size", filled=nfilled, size=alloc_size)
+    return splits
+
+
+detect_leaf_boundaries.jit = detect_leaf_boundaries
+
+
+def determine_znode_boundaries(posz: torch.Tensor, block_size: int = 64, nleaves=None):
+    """tree.py:264-284 -> FindNodeBoundaries.  Returns (lvl, lbound, rbound), each (len+1,)."""
+    size, dim = posz.shape
+    dev = posz.device
+    if nleaves is None:
+        nleaves = size
+    nl = torch.clamp(_i32(nleaves, dev), max=size)
+    info = LevelInfo(dim, posz.dtype)
+    pos_bound = torch.full((2, dim), float("nan"), dtype=posz.dtype, device=dev)
+    lvl, lb, rb = (torch.empty(size + 1, dtype=torch.int32, device=dev) for _ in range(3))
+    L.call("jzt_find_node_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(pos_bound), L.ptr(nl),
+           L.ptr(lvl), L.ptr(lb), L.ptr(rb), size + 1, dim, L.dtype_code(posz.dtype), info.max_lvl(), -2000)
+    return lvl, lb, rb
+
+
+determine_znode_boundaries.jit = determine_znode_boundaries
+
+
+def get_node_geometry(posz: torch.Tensor, lbound: torch.Tensor, rbound: torch.Tensor, num=None,
+                      block_size: int = 64, result: str = "lvl_cent_ext", upper_extent: bool = False):
+    """tree.py:286-329 -> GetNodeGeometry."""
+    assert lbound.shape == rbound.shape
+    assert lbound.dtype == rbound.dtype == torch.int32
+    dev = posz.device
+    n, dim = posz.shape
+    size = lbound.shape[0]
+    mode = (1 if "lvl" in result else 0) | (2 if "cent" in result else 0) | (4 if "ext" in result else 0)
+    if num is None:
+        num = size
+    num_t = _i32(num, dev)
+    lvl = torch.empty(size if mode & 1 else 1, dtype=torch.int32, device=dev)
+    cent = torch.empty((size if mode & 2 else 1, dim), dtype=posz.dtype, device=dev)
+    ext = torch.empty((size if mode & 4 else 1, dim), dtype=posz.dtype, device=dev)
+    L.call("jzt_get_node_geometry", L.stream(), L.ptr(posz.contiguous()), L.ptr(lbound.contiguous()),
+           L.ptr(rbound.contiguous()), L.ptr(num_t), L.ptr(lvl), L.ptr(cent), L.ptr(ext), size, n, dim,
+           L.dtype_code(posz.dtype), -2000, mode, int(bool(upper_extent)))
+    rdict = {"lvl": lvl, "cent": cent, "ext": ext}
+    res = [rdict[k] for k in result.split("_")]
+    return res[0] if len(res) == 1 else tuple(res)
+
+
+get_node_geometry.jit = get_node_geometry
+
+
+# ------------------------------------------------------------------------------------------------
+# hierarchy (tree.py:600-829)
+# ------------------------------------------------------------------------------------------------
+
+def _estimate_node_number(npart, max_node_size, alloc_fac=1.):
+    """tree.py:600-609"""
+    return int(div_ceil(npart * alloc_fac, np.maximum(max_node_size // 2, 1))) + 1
+
+
+def _define_tree_level_node_sizes(npart: int, cfg_tree: TreeConfig):
+    """tree.py:611-619"""
+    max_num_leaves = _estimate_node_number(npart, cfg_tree.max_leaf_size)
+    nlevels = np.log(max_num_leaves / cfg_tree.stop_coarsen) / np.log(cfg_tree.coarse_fac)
+    nlevels = np.maximum(int(np.ceil(nlevels)), 1)
+    return [int(cfg_tree.max_leaf_size * (cfg_tree.coarse_fac ** i)) for i in range(0, nlevels)]
+
+
+def _scatter_drop(size: int, idx: torch.Tensor, values: torch.Tensor, dtype) -> torch.Tensor:
+    """zeros(size).at[idx].set(values) with out-of-range indices dropped (== size here)."""
+    out = torch.zeros(size + 1, dtype=dtype, device=idx.device)
+    out[torch.clamp(idx, 0, size).reshape(-1)] = values.reshape(-1).to(dtype)
+    return out[:size]
+
+
+def _delete_rows(x: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """jnp.delete(x, idx, axis=0, assume_unique_indices=True) with device indices."""
+    n = x.shape[0]
+    m = idx.numel()
+    mark = torch.zeros(n, dtype=torch.int64, device=x.device)
+    mark[idx.long()] = 1
+    dest = torch.arange(n, device=x.device) - (torch.cumsum(mark, 0) - mark)
+    dest = torch.where(mark.bool(), torch.full_like(dest, n - m), dest)
+    out = torch.empty((n - m + 1,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)
+    out[dest] = x
+    return out[:n - m]
+
+
+def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None, num_types=None, lvl_bound=None,
+                            cfg_reg: RegularizationConfig | None = None):
+    """tree.py:628-734: leaf splits, then node activity per plane packed into PackedArrays."""
+    if ptype is not None:
+        raise NotImplementedError("multi-type trees are not part of this build")
+    n, dim = posz.shape
+    dev = posz.device
+    nlevels = len(node_sizes)
+    linfo = LevelInfo(dim, posz.dtype)
+    if lvl_bound is None:
+        lvl_bound = (linfo.max_lvl(), linfo.max_lvl())
+
+    ispl = detect_leaf_boundaries(posz, leaf_size=node_sizes[0], npart=npart, alloc_size=alloc_size,
+                                  lvl_bound=lvl_bound, cfg_reg=cfg_reg)
+    nleaves = torch.argmax(ispl).to(torch.int32)
+    reps = posz[torch.clamp(ispl[:-1].long(), max=n - 1)]
+    lvl, lbound, rbound = determine_znode_boundaries(reps, nleaves=nleaves)
+    lbl, rbl = lbound.long(), rbound.long()
+    npins = ispl[rbl] - ispl[lbl]
+    ispl_per_type = ispl.reshape((1,) + ispl.shape)
+
+    sizes = torch.tensor(node_sizes, dtype=torch.int32, device=dev)
+    isl = npins[None, :] > sizes[:, None]
+    isl = isl | ((lbound <= 0) & (lvl > lvl_bound[0]))[None, :]
+    isl = isl | ((rbound >= nleaves) & (lvl > lvl_bound[1]))[None, :]
+
+    if cfg_reg is not None:
+        new = [(npins > 0) | isl[0]]
+        lvl_up = torch.minimum(lvl[lbl], lvl[rbl]) - 1
+        ar = torch.arange(len(lvl), device=dev)
+        for i in range(1, nlevels):
+            is_node = (~isl[i]) & isl[i, lbl] & isl[i, rbl]
+            is_node = is_node & (ar < nleaves)
+            lvl_max = find_regularization_level(lvl_up, linfo, weights=torch.where(is_node, npins, 0),
+                                                max_vol_fac=cfg_reg.max_volume_fac,
+                                                min_percentile=cfg_reg.regularize_percentile)
+            new.append(isl[i] | ((lvl > lvl_max) & (ar <= nleaves)))
+        isl = torch.stack(new)
+
+    offsets = torch.cumsum(isl.reshape(-1).to(torch.int64), 0).reshape(isl.shape)
+    level_spl = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), offsets[:, -1]]).to(torch.int32)
+    nnodes_on_level = level_spl[1:] - level_spl[:-1] - 1
+    raise_if(level_spl[-1] > alloc_size,
+             "Tree-node allocation too small: filled={filled} size={size}.\n"
+             "Hint: Increase alloc_fac_nodes, max_leaf_size or coarse_fac", filled=level_spl[-1], size=alloc_size)
+    offsets = torch.where(isl, offsets - isl.to(torch.int64), torch.full_like(offsets, alloc_size))
+
+    ispl_n2l = _scatter_drop(alloc_size, offsets, offsets[0:1, :].expand_as(offsets), torch.int32)
+    ispl_n2l = PackedArray.from_data(ispl_n2l, level_spl, fill_values=nnodes_on_level[0].expand(nlevels))
+
+    value = torch.empty_like(offsets)
+    value[0] = ispl.to(torch.int64)
+    for li in range(1, nlevels):
+        value[li] = offsets[li - 1] - level_spl[li - 1]
+    ispl_n2n = _scatter_drop(alloc_size, offsets, value, torch.int32)
+    fill_val = torch.cat([ispl[-1:], nnodes_on_level[:-1]])
+    ispl_n2n = PackedArray.from_data(ispl_n2n, level_spl, fill_values=fill_val)
+    return ispl, ispl_per_type, ispl_n2l, ispl_n2n
+
+
+def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None, num_types=None) -> TreeHierarchy:
+    """Builds a tree hierarchy from z-order positions (tree.py:756-829).
+
+    Plane 0 holds leaves of at most cfg_tree.max_leaf_size particles; plane p nodes hold at most
+    max_leaf_size * coarse_fac**p particles.  Geometric centres only (mass-centred trees are not
+    part of this build)."""
+    if cfg_tree.mass_centered:
+        raise NotImplementedError("mass-centred trees are not part of this build")
+    posz = get_pos(partz)
+    npart_tot = get_num_total(partz, default_to_length=True)
+    np_per_dev = npart_tot
+    npart = get_num(partz, default_to_nancount=True)
+    node_sizes = _define_tree_level_node_sizes(npart_tot, cfg_tree)
+    alloc_size = _estimate_node_number(np_per_dev, cfg_tree.max_leaf_size, cfg_tree.alloc_fac_nodes)
+
+    ispl_l2p, ispl_per_type, ispl_n2l, ispl_n2n = _define_split_hierarchy(
+        posz, node_sizes, alloc_size, npart=npart, ptype=ptype, num_types=num_types, lvl_bound=lvl_bound,
+        cfg_reg=cfg_tree.regularization)
+
+    ispl_n2p = ispl_l2p[ispl_n2l.data.long()]
+    lvl, geom_cent = get_node_geometry(posz, ispl_n2p[:-1], ispl_n2p[1:], num=ispl_n2l.nfilled() - 1,
+                                       result="lvl_cent")
+    seams = ispl_n2l.ispl[1:-1] - 1
+    lvl = _delete_rows(lvl, seams)
+    geom_cent = _delete_rows(geom_cent, seams)
+    prop_array_spl = cumsum_starting_with_zero(ispl_n2n.ispl[1:] - ispl_n2n.ispl[:-1] - 1)
+    lvl = PackedArray.from_data(lvl, prop_array_spl, fill_values=-2000)
+    geom_cent = PackedArray.from_data(geom_cent, prop_array_spl, fill_values=float("nan"))
+    size_leaves = _estimate_node_number(np_per_dev, cfg_tree.max_leaf_size)
+    return TreeHierarchy(size_leaves, ispl_n2n, ispl_n2l, ispl_per_type, lvl, geom_cent)
+
+
+build_tree_hierarchy.jit = build_tree_hierarchy
+
+
+def zsort_and_tree(part, cfg_tree: TreeConfig = TreeConfig(), data: Any | None = None, ptype=None,
+                   num_types=None, shrink: bool = True):
+    """tree.py:363-421 (single GPU): z-sort (carrying `data`) and build the hierarchy.
+
+    Returns (partz, th), or (partz, dataz, th) when data is given."""
+    if ptype is not None:
+        raise NotImplementedError("multi-type trees are not part of this build")
+    partz, isort = zsort(part)
+    dtz = tree_map_by_len(lambda v: v[isort.long()], (data, ptype), len(get_pos(partz)))
+    th = build_tree_hierarchy(partz, cfg_tree)
+    out = [partz]
+    if (data is not None) or not shrink:
+        out.append(dtz[0])
+    if not shrink:
+        out.append(dtz[1])
+    return (*out, th)
+
+
+zsort_and_tree.jit = zsort_and_tree
+
+
+# ------------------------------------------------------------------------------------------------
+# interaction-list initialisation (tree.py:868-930)
+# ------------------------------------------------------------------------------------------------
+
+def grouped_dense_interaction_list(nnodes, size_ilist: int, ngroup: int = 32, size_super: int | None = None,
+                                   node_range=None, dtype=torch.int32):
+    """All-to-all interaction list over super-nodes of `ngroup` top-level nodes (tree.py:868-930).
+
+    Returns (spl_super, ilist, nsuper_nodes)."""
+    if node_range is not None:
+        raise NotImplementedError("node_range is a multi-GPU feature")
+    dev = nnodes.device if torch.is_tensor(nnodes) else torch.device("cuda")
+    nn = _i32(nnodes, dev)[0]
+    if size_super is None:
+        size_super = int(np.ceil(np.sqrt(size_ilist))) + 2
+    nsuper = (nn + ngroup - 1) // ngroup
+    nint = nsuper * nsuper
+    ar = torch.arange(size_super + 1, dtype=torch.int64, device=dev)
+    spl_super = torch.minimum(ar * ngroup, nn.to(torch.int64)).to(dtype)
+    ispl = torch.minimum(ar * nsuper.to(torch.int64), nint.to(torch.int64)).to(dtype)
+    raise_if(nint.to(torch.int64) > size_ilist,
+             "Cannot fit {n} interactions into interaction list with size {size}\nHint: Increase alloc_fac_ilist",
+             n=nint, size=size_ilist)
+    idx = torch.arange(size_ilist, dtype=torch.int64, device=dev)
+    ilist = torch.where(idx < nint.to(torch.int64), idx % torch.clamp(nsuper, min=1).to(torch.int64),
+                        torch.zeros_like(idx)).to(dtype)
+    return spl_super, verify_ilist(InteractionList(ispl=ispl, isrc=ilist)), nsuper.to(torch.int32)
+
+
+grouped_dense_interaction_list.jit = grouped_dense_interaction_list

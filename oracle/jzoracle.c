@@ -1,0 +1,635 @@
+/*
+ * jzoracle.c -- CPU restatement of jztree's tree hot path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the checker: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  The product (jztree_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates (paths relative to
+ * the jztree repository).  Where this build deliberately differs from the reference (kNN tie
+ * order, z-order quirk for exactly equal coordinates) the difference is stated at the function.
+ *
+ * Build: oracle/Makefile  (gcc -O2 -fopenmp -ffp-contract=off -shared -fPIC)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/*                                    bit helpers                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+static inline uint32_t f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static inline uint64_t d2u(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static inline float u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+static inline double u2d(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+static inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+static inline int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* ------------------------------------------------------------------------------------------ */
+/*                     z-order level math  (src/jztree_cuda/common/math.cuh)                 */
+/* ------------------------------------------------------------------------------------------ */
+
+/* math.cuh:241-269 msb_xor_float, literal (including the equal-bits quirk: clz(0)=32). */
+int32_t orc_msb_xor_f32(float a, float b) {
+    if (isnan(a) || isnan(b) || (signbit(a) != signbit(b))) return 128;
+    uint32_t ab = f2u(fabsf(a)), bb = f2u(fabsf(b));
+    int32_t ae = (int32_t)(ab >> 23) - 127, be = (int32_t)(bb >> 23) - 127;
+    if (ae == be) {
+        ae = imax(ae, -126);
+        return ae + (8 - clz32(ab ^ bb));
+    }
+    return imax(ae, be);
+}
+
+/* math.cuh:271-290 msb_xor_double, literal.  (The reference has no NaN test here.) */
+int32_t orc_msb_xor_f64(double a, double b) {
+    if (signbit(a) != signbit(b)) return 1024;
+    uint64_t ab = d2u(fabs(a)), bb = d2u(fabs(b));
+    int32_t ae = (int32_t)(ab >> 52) - 1023, be = (int32_t)(bb >> 52) - 1023;
+    if (ae == be) {
+        ae = imax(ae, -1022);
+        return ae + (11 - clz64(ab ^ bb));
+    }
+    return imax(ae, be);
+}
+
+/* math.cuh:313-327 msb_diff_level: (msb+1)*dim - msb_dim, highest level wins, ties -> lowest dim */
+int32_t orc_msb_diff_level_f32(const float* p1, const float* p2, int dim) {
+    int msb_dim = 0, msb = orc_msb_xor_f32(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = orc_msb_xor_f32(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    return (msb + 1) * dim - msb_dim;
+}
+int32_t orc_msb_diff_level_f64(const double* p1, const double* p2, int dim) {
+    int msb_dim = 0, msb = orc_msb_xor_f64(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = orc_msb_xor_f64(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    return (msb + 1) * dim - msb_dim;
+}
+
+/* math.cuh:365-402 z_pos_less, literal (NaN rows last; compare along the dim of highest level). */
+int orc_z_pos_less_f32(const float* p1, const float* p2, int dim) {
+    int n1 = 0, n2 = 0;
+    for (int i = 0; i < dim; i++) { n1 |= isnan(p1[i]); n2 |= isnan(p2[i]); }
+    if (n1) return 0;
+    if (n2) return 1;
+    int msb_dim = 0, msb = orc_msb_xor_f32(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = orc_msb_xor_f32(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    return p1[msb_dim] < p2[msb_dim];
+}
+int orc_z_pos_less_f64(const double* p1, const double* p2, int dim) {
+    int n1 = 0, n2 = 0;
+    for (int i = 0; i < dim; i++) { n1 |= isnan(p1[i]); n2 |= isnan(p2[i]); }
+    if (n1) return 0;
+    if (n2) return 1;
+    int msb_dim = 0, msb = orc_msb_xor_f64(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = orc_msb_xor_f64(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    return p1[msb_dim] < p2[msb_dim];
+}
+
+/*
+ * Exact Morton order used by this build (a strict total order once the index breaks ties).
+ *
+ * It is z_pos_less (math.cuh:381-402) with two reference quirks removed:
+ *  - exactly equal coordinates have no differing bit (level -inf) instead of level e-24
+ *    (math.cuh:263-264, clz(0)=32);
+ *  - -0.0 is treated as +0.0 (the reference gives the pair level 128 and calls it neither less
+ *    nor greater).
+ * Per dim the "top digit" ranks {-inf, negative finite, non-negative finite, +inf}; a differing
+ * top digit is level 129 (above every finite magnitude bit, like the reference's 128 for sign
+ * or inf differences).  Returns -1, 0, +1.
+ */
+static inline int topdig_f32(float x) {
+    if (isinf(x)) return x < 0 ? 0 : 3;
+    return (x < 0.0f) ? 1 : 2;
+}
+static inline int topdig_f64(double x) {
+    if (isinf(x)) return x < 0 ? 0 : 3;
+    return (x < 0.0) ? 1 : 2;
+}
+static inline int exact_level_f32(float a, float b) {
+    int ta = topdig_f32(a), tb = topdig_f32(b);
+    if (ta != tb) return 129;
+    uint32_t ab = f2u(fabsf(a)), bb = f2u(fabsf(b));
+    if (ab == bb) return -100000;
+    int32_t ae = (int32_t)(ab >> 23) - 127, be = (int32_t)(bb >> 23) - 127;
+    if (ae == be) return imax(ae, -126) + (8 - clz32(ab ^ bb));
+    return imax(ae, be);
+}
+static inline int exact_level_f64(double a, double b) {
+    int ta = topdig_f64(a), tb = topdig_f64(b);
+    if (ta != tb) return 1025;
+    uint64_t ab = d2u(fabs(a)), bb = d2u(fabs(b));
+    if (ab == bb) return -100000;
+    int32_t ae = (int32_t)(ab >> 52) - 1023, be = (int32_t)(bb >> 52) - 1023;
+    if (ae == be) return imax(ae, -1022) + (11 - clz64(ab ^ bb));
+    return imax(ae, be);
+}
+int orc_z_exact_cmp_f32(const float* p1, const float* p2, int dim) {
+    int n1 = 0, n2 = 0;
+    for (int i = 0; i < dim; i++) { n1 |= isnan(p1[i]); n2 |= isnan(p2[i]); }
+    if (n1 || n2) return n1 - n2;
+    int msb_dim = 0, msb = exact_level_f32(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = exact_level_f32(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    if (msb == -100000) return 0;
+    float a = p1[msb_dim], b = p2[msb_dim];
+    if (msb == 129) return topdig_f32(a) < topdig_f32(b) ? -1 : 1;
+    return a < b ? -1 : 1;
+}
+int orc_z_exact_cmp_f64(const double* p1, const double* p2, int dim) {
+    int n1 = 0, n2 = 0;
+    for (int i = 0; i < dim; i++) { n1 |= isnan(p1[i]); n2 |= isnan(p2[i]); }
+    if (n1 || n2) return n1 - n2;
+    int msb_dim = 0, msb = exact_level_f64(p1[0], p2[0]);
+    for (int i = 1; i < dim; i++) {
+        int m = exact_level_f64(p1[i], p2[i]);
+        if (m > msb) { msb = m; msb_dim = i; }
+    }
+    if (msb == -100000) return 0;
+    double a = p1[msb_dim], b = p2[msb_dim];
+    if (msb == 1025) return topdig_f64(a) < topdig_f64(b) ? -1 : 1;
+    return a < b ? -1 : 1;
+}
+
+/* Stable z-order sort: (exact Morton order, original index).  Restates PosZorderSort
+ * (sort.cuh:40-76: CUB merge sort of (pos, iota) with z_pos_less); the reference leaves the
+ * order of comparator-equivalent points to CUB, this build fixes it to the input index. */
+static const float* g_sort_pos32;
+static const double* g_sort_pos64;
+static int g_sort_dim;
+static int cmp_idx_f32(const void* a, const void* b) {
+    int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+    int c = orc_z_exact_cmp_f32(g_sort_pos32 + i * g_sort_dim, g_sort_pos32 + j * g_sort_dim, g_sort_dim);
+    if (c) return c;
+    return (i > j) - (i < j);
+}
+static int cmp_idx_f64(const void* a, const void* b) {
+    int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+    int c = orc_z_exact_cmp_f64(g_sort_pos64 + i * g_sort_dim, g_sort_pos64 + j * g_sort_dim, g_sort_dim);
+    if (c) return c;
+    return (i > j) - (i < j);
+}
+void orc_zsort_f32(const float* pos, int64_t n, int dim, int64_t* idz_out) {
+    for (int64_t i = 0; i < n; i++) idz_out[i] = i;
+    g_sort_pos32 = pos; g_sort_dim = dim;
+    qsort(idz_out, (size_t)n, sizeof(int64_t), cmp_idx_f32);
+}
+void orc_zsort_f64(const double* pos, int64_t n, int dim, int64_t* idz_out) {
+    for (int64_t i = 0; i < n; i++) idz_out[i] = i;
+    g_sort_pos64 = pos; g_sort_dim = dim;
+    qsort(idz_out, (size_t)n, sizeof(int64_t), cmp_idx_f64);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/*                                tree kernels (tree.cuh)                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+/*
+ * tree.cuh:13-98 FlagLeafBoundaries, one boundary at a time.  Boundary b separates particles
+ * b-1 and b.  level[b] = msb_diff_level(posz[b-1], posz[b]) with lvl_bound at the ends.
+ * lsize/rsize count the particles of the binary Morton cell on each side (runs of boundaries
+ * with a smaller level, window scan_size), ptype handling as in the reference (the type of the
+ * particle left of each scanned boundary, tree.cuh:67-75).
+ */
+void orc_flag_leaf_boundaries_f32(const float* posz, const uint8_t* ptype, const int32_t* lvl_bound,
+                                  int32_t npart, int64_t size_part, int dim, int max_size,
+                                  int scan_size, int num_types, int8_t* split_flags, int32_t* lvl) {
+    int64_t nb = size_part + 1;
+    /* level of boundary b (b in [0, size_part]) */
+    int32_t* L = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nb));
+    uint8_t* T = (uint8_t*)malloc((size_t)nb);
+    for (int64_t b = 0; b < nb; b++) {
+        int64_t ipart = b - 1;  /* particle left of boundary b */
+        if (ipart < 0) { L[b] = lvl_bound[0]; T[b] = 0; }
+        else if (ipart + 1 >= npart) { L[b] = lvl_bound[1]; T[b] = 0; }
+        else {
+            L[b] = orc_msb_diff_level_f32(posz + ipart * dim, posz + (ipart + 1) * dim, dim);
+            T[b] = (num_types > 1) ? ptype[ipart] : 0;
+        }
+    }
+    for (int64_t b = 0; b < nb; b++) {
+        int my = L[b];
+        int is_split = 0;
+        for (int type = 0; type < num_types; type++) {
+            int lsize = 0, rsize = 0;
+            for (int64_t i = b - scan_size; i < b; i++) {
+                int li = (i < 0) ? lvl_bound[0] : L[i];
+                int ti = (i < 0) ? 0 : T[i];
+                lsize = (li < my ? lsize : 0) + (ti == type);
+            }
+            for (int64_t i = b + scan_size; i > b; i--) {
+                int li, ti;
+                if (i >= nb) { li = lvl_bound[1]; ti = 0; }
+                else { li = L[i]; ti = T[i]; }
+                /* the reference's LDS window only holds boundaries <= size_part; beyond the
+                 * particle range it stores lbound_r (ipart+1 >= nump), identical to L[i] here */
+                rsize = (li < my ? rsize : 0) + (ti == type);
+            }
+            is_split |= (lsize + rsize > max_size);
+            is_split |= ((my > lvl_bound[0]) && (b - lsize <= 0));
+            is_split |= ((my > lvl_bound[1]) && (b + rsize >= npart));
+        }
+        is_split &= (b <= npart);
+        is_split |= (b == 0);
+        is_split |= (b == npart);
+        split_flags[b] = (int8_t)is_split;
+        lvl[b] = my;
+    }
+    free(L); free(T);
+}
+
+/* tree.cuh:104-199 FindNodeBoundaries, literal (binary searches with msb_diff_level). */
+void orc_find_node_boundaries_f32(const float* pos_in, const float* pos_boundary, int32_t nleaves,
+                                  int64_t size_nodes, int dim, int lvl_max, int lvl_invalid,
+                                  int32_t* nodes_levels, int32_t* nodes_lbound, int32_t* nodes_rbound) {
+    int n = nleaves, nnodes = n + 1;
+    for (int64_t idx = 0; idx < size_nodes; idx++) {
+        if (idx >= nnodes) {
+            nodes_levels[idx] = lvl_invalid; nodes_lbound[idx] = n; nodes_rbound[idx] = n;
+            continue;
+        }
+        const float* p1 = (idx == 0) ? pos_boundary : pos_in + (idx - 1) * dim;
+        const float* p2 = (idx == nnodes - 1) ? pos_boundary + dim : pos_in + idx * dim;
+        int target = orc_msb_diff_level_f32(p1, p2, dim);
+        int lbound, rbound;
+        if (orc_msb_diff_level_f32(pos_in, p2, dim) <= target) {
+            lbound = 0;
+        } else {
+            int ibefore = 0, iinside = (int)idx;
+            while (ibefore + 1 < iinside) {
+                int itest = (ibefore + iinside) / 2;
+                int l = orc_msb_diff_level_f32(pos_in + (int64_t)itest * dim, p2, dim);
+                if (l > target) ibefore = itest; else iinside = itest;
+            }
+            lbound = iinside;
+        }
+        if (orc_msb_diff_level_f32(p1, pos_in + (int64_t)(n - 1) * dim, dim) <= target) {
+            rbound = n;
+        } else {
+            int iinside = (int)idx - 1, iafter = n - 1;
+            while (iinside + 1 < iafter) {
+                int itest = (iinside + iafter) / 2;
+                int l = orc_msb_diff_level_f32(p1, pos_in + (int64_t)itest * dim, dim);
+                if (l > target) iafter = itest; else iinside = itest;
+            }
+            rbound = iafter;
+        }
+        nodes_levels[idx] = target; nodes_lbound[idx] = lbound; nodes_rbound[idx] = rbound;
+    }
+}
+
+/* math.cuh:524-541 lvl_vec: python floor division of the binary level over dims */
+static void lvl_vec(int level, int dim, int* lvec) {
+    int olvl = (level + 2000 * dim) / dim - 2000;
+    int omod = level - olvl * dim;
+    for (int i = 0; i < dim; i++) lvec[i] = olvl + (omod >= (dim - i));
+}
+
+/* math.cuh:404-437 round_float_pow2_cent, literal */
+static float round_f32_pow2_cent(float x, int level) {
+    int32_t xb = (int32_t)f2u(fabsf(x));
+    int32_t xe = (xb >> 23) - 127;
+    if (level >= 128) return 0.f;
+    if (level > xe) return ldexpf(signbit(x) ? -0.5f : 0.5f, level);
+    int32_t keep = imax(xe, -126) - level;
+    if (keep >= 23) return x;
+    uint32_t mask = 0xFFFFFFFFu << (23 - keep);
+    uint32_t nb = ((uint32_t)xb & mask) | (1u << (22 - keep));
+    return signbit(x) ? -u2f(nb) : u2f(nb);
+}
+
+/* tree.cuh:312-372 GetNodeGeometry (lower- and upper-extent modes), literal. */
+void orc_get_node_geometry_f32(const float* pos, const int32_t* lbound, const int32_t* rbound,
+                               int32_t nnodes, int64_t size_nodes, int64_t size_part, int dim,
+                               int lvl_invalid, uint32_t mode_flags, int upper_extent,
+                               int32_t* level, float* center, float* extent) {
+    for (int64_t idx = 0; idx < size_nodes; idx++) {
+        if (idx >= nnodes) {
+            if (mode_flags & 1u) level[idx] = lvl_invalid;
+            if (mode_flags & 2u) for (int d = 0; d < dim; d++) center[idx * dim + d] = u2f(0x7fc00000u);
+            if (mode_flags & 4u) for (int d = 0; d < dim; d++) extent[idx * dim + d] = 0.f;
+            continue;
+        }
+        int lb = lbound[idx], rb = rbound[idx];
+        int64_t i0 = imin(imax(lb, 0), (int)size_part - 1), i1 = imin(imax(rb - 1, 0), (int)size_part - 1);
+        const float* x0b = pos + i0 * dim;
+        const float* x1a = pos + i1 * dim;
+        int lvl;
+        if (!upper_extent) {
+            lvl = orc_msb_diff_level_f32(x0b, x1a, dim);
+        } else {
+            float x0a[16], x1b[16];
+            int64_t ia = imin(imax(lb - 1, 0), (int)size_part - 1), ib = imin(imax(rb, 0), (int)size_part - 1);
+            for (int d = 0; d < dim; d++) {
+                x0a[d] = (lb - 1 < 0) ? -INFINITY : pos[ia * dim + d];
+                x1b[d] = (rb >= size_part) ? INFINITY : pos[ib * dim + d];
+            }
+            int ll = orc_msb_diff_level_f32(x0a, x0b, dim);
+            int lr = orc_msb_diff_level_f32(x1a, x1b, dim);
+            lvl = imin(ll, lr) - 1;
+        }
+        int lv[16];
+        lvl_vec(lvl, dim, lv);
+        if (mode_flags & 1u) level[idx] = lvl;
+        if (mode_flags & 2u) for (int d = 0; d < dim; d++) center[idx * dim + d] = round_f32_pow2_cent(x0b[d], lv[d]);
+        if (mode_flags & 4u) for (int d = 0; d < dim; d++) extent[idx * dim + d] = ldexpf(1.0f, lv[d]);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/*                              canonical distance (math.cuh:471-522)                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* wrap_dx (math.cuh:471-488): into [-L/2, L/2) by one add/subtract */
+static inline float wrap_f32(float dx, float boxsize) {
+    if (boxsize > 0.f) {
+        float bh = 0.5f * boxsize;
+        dx = dx < -bh ? dx + boxsize : dx;
+        dx = dx >= bh ? dx - boxsize : dx;
+    }
+    return dx;
+}
+/* distance_squared (math.cuh:511-522) under nvcc's default contraction: an FMA chain
+ * r2 = fma(dz,dz, fma(dy,dy, dx*dx)).  This is the canonical FP32 r^2 of this build. */
+static inline float r2_f32(const float* a, const float* b, int dim, float boxsize) {
+    float dx = wrap_f32(a[0] - b[0], boxsize);
+    float r2 = dx * dx;
+    for (int i = 1; i < dim; i++) {
+        float d = wrap_f32(a[i] - b[i], boxsize);
+        r2 = fmaf(d, d, r2);
+    }
+    return r2;
+}
+float orc_r2_f32(const float* a, const float* b, int dim, float boxsize) { return r2_f32(a, b, dim, boxsize); }
+
+/* ------------------------------------------------------------------------------------------ */
+/*                                    exact kNN                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Ordering of this build (differs from knn.cuh:84-102, which inserts with strict '<' in visit
+ * order): neighbours ascending by (canonical r2, reported id). */
+static inline int pair_less(float r2a, int64_t ia, float r2b, int64_t ib) {
+    return (r2a < r2b) || (r2a == r2b && ia < ib);
+}
+
+static void topk_insert(float* r2s, int64_t* ids, int k, float r2, int64_t id) {
+    if (!pair_less(r2, id, r2s[k - 1], ids[k - 1])) return;
+    int i = k - 1;
+    while (i > 0 && pair_less(r2, id, r2s[i - 1], ids[i - 1])) { r2s[i] = r2s[i - 1]; ids[i] = ids[i - 1]; i--; }
+    r2s[i] = r2; ids[i] = id;
+}
+
+/* Brute force O(Nq*Ns): for every query, the k smallest (r2, id).  Output r2 (not sqrt). */
+void orc_knn_brute_f32(const float* xs, const int64_t* ids, int64_t ns, const float* xq, int64_t nq,
+                       int dim, int k, float boxsize, float* r2_out, int64_t* id_out) {
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < nq; q++) {
+        float* r2s = r2_out + q * k;
+        int64_t* ii = id_out + q * k;
+        for (int j = 0; j < k; j++) { r2s[j] = INFINITY; ii[j] = INT64_MAX; }
+        for (int64_t s = 0; s < ns; s++) {
+            float r2 = r2_f32(xq + q * dim, xs + s * dim, dim, boxsize);
+            topk_insert(r2s, ii, k, r2, ids ? ids[s] : s);
+        }
+    }
+}
+
+/* Uniform-grid cell list over the bounding box (or the periodic box). */
+typedef struct {
+    int dim, nc[3];
+    double lo[3], h[3];
+    int64_t* cell_start;   /* ncells+1 */
+    int64_t* order;        /* points sorted by cell */
+    int periodic;
+} grid_t;
+
+static int64_t cell_of(const grid_t* g, const float* x, int c[3]) {
+    int64_t lin = 0;
+    for (int d = 0; d < g->dim; d++) {
+        int cd = (int)floor(((double)x[d] - g->lo[d]) / g->h[d]);
+        if (cd < 0) cd = 0;
+        if (cd >= g->nc[d]) cd = g->nc[d] - 1;
+        c[d] = cd;
+        lin = lin * g->nc[d] + cd;
+    }
+    return lin;
+}
+
+static void grid_build(grid_t* g, const float* xs, int64_t ns, int dim, double target_h, float boxsize) {
+    g->dim = dim;
+    g->periodic = boxsize > 0.f;
+    for (int d = 0; d < dim; d++) {
+        double lo = INFINITY, hi = -INFINITY;
+        if (g->periodic) { lo = 0.0; hi = boxsize; }
+        else for (int64_t i = 0; i < ns; i++) { double v = xs[i * dim + d]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+        if (!(hi > lo)) hi = lo + 1.0;
+        double span = hi - lo;
+        int nc = (int)floor(span / target_h);
+        if (nc < 1) nc = 1;
+        if (nc > 1024) nc = 1024;
+        g->nc[d] = nc; g->lo[d] = lo; g->h[d] = span / nc;
+    }
+    int64_t ncell = 1;
+    for (int d = 0; d < dim; d++) ncell *= g->nc[d];
+    g->cell_start = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+    g->order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ns > 0 ? ns : 1));
+    int64_t* cid = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ns > 0 ? ns : 1));
+    int c[3];
+    for (int64_t i = 0; i < ns; i++) { cid[i] = cell_of(g, xs + i * dim, c); g->cell_start[cid[i] + 1]++; }
+    for (int64_t i = 0; i < ncell; i++) g->cell_start[i + 1] += g->cell_start[i];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+    memcpy(fill, g->cell_start, sizeof(int64_t) * (size_t)ncell);
+    for (int64_t i = 0; i < ns; i++) g->order[fill[cid[i]]++] = i;
+    free(fill); free(cid);
+}
+static void grid_free(grid_t* g) { free(g->cell_start); free(g->order); }
+
+/*
+ * Exact kNN by expanding Chebyshev shells of grid cells.  A shell s (cells at Chebyshev
+ * distance s from the query's cell) is only skipped once the lower bound (s-1)*h_min of its
+ * distance, squared and with a 1e-5 relative margin for FP32 rounding of r2, exceeds the
+ * current k-th r2.  Results equal orc_knn_brute_f32.
+ */
+void orc_knn_grid_f32(const float* xs, const int64_t* ids, int64_t ns, const float* xq, int64_t nq,
+                      int dim, int k, float boxsize, float* r2_out, int64_t* id_out) {
+    grid_t g;
+    double vol = 1.0;
+    for (int d = 0; d < dim; d++) {
+        double lo = INFINITY, hi = -INFINITY;
+        if (boxsize > 0.f) { lo = 0; hi = boxsize; }
+        else for (int64_t i = 0; i < ns; i++) { double v = xs[i * dim + d]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+        vol *= (hi > lo ? hi - lo : 1.0);
+    }
+    double per = (double)(k > 4 ? k : 4) * 0.5;
+    double h = pow(vol * per / (double)(ns > 0 ? ns : 1), 1.0 / dim);
+    grid_build(&g, xs, ns, dim, h, boxsize);
+    double hmin = g.h[0];
+    for (int d = 1; d < dim; d++) hmin = g.h[d] < hmin ? g.h[d] : hmin;
+    int maxnc = 0;
+    for (int d = 0; d < dim; d++) maxnc = imax(maxnc, g.nc[d]);
+
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t q = 0; q < nq; q++) {
+        float* r2s = r2_out + q * k;
+        int64_t* ii = id_out + q * k;
+        for (int j = 0; j < k; j++) { r2s[j] = INFINITY; ii[j] = INT64_MAX; }
+        int c0[3] = {0, 0, 0};
+        cell_of(&g, xq + q * dim, c0);
+        for (int s = 0; s <= maxnc; s++) {
+            if (s >= 1) {
+                double lb = (double)(s - 1) * hmin;
+                if (lb * lb > (double)r2s[k - 1] * (1.0 + 1e-5)) break;
+            }
+            /* enumerate cells with Chebyshev distance exactly s */
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; d++) { lo[d] = 0; hi[d] = 0; }
+            for (int d = 0; d < dim; d++) { lo[d] = c0[d] - s; hi[d] = c0[d] + s; }
+            int a[3];
+            for (a[0] = lo[0]; a[0] <= hi[0]; a[0]++)
+            for (a[1] = lo[1]; a[1] <= hi[1]; a[1]++)
+            for (a[2] = lo[2]; a[2] <= hi[2]; a[2]++) {
+                int cheb = 0;
+                for (int d = 0; d < dim; d++) { int dd = abs(a[d] - c0[d]); cheb = imax(cheb, dd); }
+                if (cheb != s) continue;
+                int64_t lin = 0, skip = 0;
+                for (int d = 0; d < dim; d++) {
+                    int cd = a[d];
+                    if (g.periodic) {
+                        if (2 * s + 1 > g.nc[d] && (a[d] - c0[d] > g.nc[d] / 2 || a[d] - c0[d] < -((g.nc[d] - 1) / 2))) { skip = 1; break; }
+                        cd = ((cd % g.nc[d]) + g.nc[d]) % g.nc[d];
+                    } else if (cd < 0 || cd >= g.nc[d]) { skip = 1; break; }
+                    lin = lin * g.nc[d] + cd;
+                }
+                if (skip) continue;
+                for (int64_t t = g.cell_start[lin]; t < g.cell_start[lin + 1]; t++) {
+                    int64_t si = g.order[t];
+                    float r2 = r2_f32(xq + q * dim, xs + si * dim, dim, boxsize);
+                    topk_insert(r2s, ii, k, r2, ids ? ids[si] : si);
+                }
+            }
+        }
+    }
+    grid_free(&g);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/*                                  friends-of-friends                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+static int64_t uf_find(int64_t* p, int64_t x) {
+    while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+    return x;
+}
+static void uf_union(int64_t* p, int64_t a, int64_t b) {
+    a = uf_find(p, a); b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) p[b] = a; else p[a] = b;
+}
+
+/*
+ * FoF groups of the predicate canonical r2(i,j) <= r2link (fof.cuh:334-343 restated on all
+ * pairs, no tree).  Output label[i] = min over i's group of key[j] (key = input index gives the
+ * canonical labels; key = z-index gives the reference's z-order labels, fof.cuh:22-36).
+ * The cell list uses cells >= rlink in every dim, so neighbours lie in adjacent cells.
+ */
+void orc_fof_f32(const float* x, int64_t n, int dim, float r2link, float boxsize,
+                 const int64_t* key, int64_t* label_out) {
+    int64_t* par = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) par[i] = i;
+    double rl = sqrt((double)r2link) * (1.0 + 1e-6);
+    grid_t g;
+    grid_build(&g, x, n, dim, rl, boxsize);
+    /* make sure every cell width is >= rlink (grid_build floors nc, so h >= target) */
+    int64_t ncell = 1;
+    for (int d = 0; d < dim; d++) ncell *= g.nc[d];
+    int nnb = 1;
+    for (int d = 0; d < dim; d++) nnb *= 3;
+    for (int64_t cl = 0; cl < ncell; cl++) {
+        int c[3] = {0, 0, 0};
+        int64_t rem = cl;
+        for (int d = dim - 1; d >= 0; d--) { c[d] = (int)(rem % g.nc[d]); rem /= g.nc[d]; }
+        int64_t seen[27];
+        int nseen = 0;
+        for (int o = 0; o < nnb; o++) {
+            int off[3] = {0, 0, 0}, oo = o, skip = 0;
+            int64_t lin = 0;
+            for (int d = 0; d < dim; d++) { off[d] = oo % 3 - 1; oo /= 3; }
+            for (int d = 0; d < dim; d++) {
+                int cd = c[d] + off[d];
+                if (g.periodic) cd = ((cd % g.nc[d]) + g.nc[d]) % g.nc[d];
+                else if (cd < 0 || cd >= g.nc[d]) { skip = 1; break; }
+                lin = lin * g.nc[d] + cd;
+            }
+            if (skip || lin < cl) continue;
+            int dup = 0;
+            for (int s = 0; s < nseen; s++) dup |= (seen[s] == lin);
+            if (dup) continue;
+            seen[nseen++] = lin;
+            for (int64_t a = g.cell_start[cl]; a < g.cell_start[cl + 1]; a++) {
+                int64_t i = g.order[a];
+                int64_t bstart = (lin == cl) ? a + 1 : g.cell_start[lin];
+                for (int64_t b = bstart; b < g.cell_start[lin + 1]; b++) {
+                    int64_t j = g.order[b];
+                    if (r2_f32(x + i * dim, x + j * dim, dim, boxsize) <= r2link) uf_union(par, i, j);
+                }
+            }
+        }
+    }
+    grid_free(&g);
+    int64_t* mn = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) mn[i] = INT64_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = uf_find(par, i);
+        int64_t kk = key ? key[i] : i;
+        if (kk < mn[r]) mn[r] = kk;
+    }
+    for (int64_t i = 0; i < n; i++) label_out[i] = mn[uf_find(par, i)];
+    free(mn); free(par);
+}
+
+/* brute force FoF (tiny n only): same output convention as orc_fof_f32 */
+void orc_fof_brute_f32(const float* x, int64_t n, int dim, float r2link, float boxsize,
+                       const int64_t* key, int64_t* label_out) {
+    int64_t* par = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) par[i] = i;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++)
+            if (r2_f32(x + i * dim, x + j * dim, dim, boxsize) <= r2link) uf_union(par, i, j);
+    int64_t* mn = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) mn[i] = INT64_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = uf_find(par, i);
+        int64_t kk = key ? key[i] : i;
+        if (kk < mn[r]) mn[r] = kk;
+    }
+    for (int64_t i = 0; i < n; i++) label_out[i] = mn[uf_find(par, i)];
+    free(mn); free(par);
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

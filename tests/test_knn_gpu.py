@@ -1,0 +1,96 @@
+"""Exact kNN on the GPU vs the oracle: neighbour ids bit-exact (ties by index), r2 bit-exact,
+radii = correctly rounded sqrt of r2 and within 2 ulp of the float64 distance."""
+import numpy as np
+import pytest
+import torch
+
+from gen import clustered, uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _knn(x, k, dev, **kw):
+    import jztree_amd as jz
+    print(f"[knn] n={len(x)} k={k} {kw}", flush=True)
+    t = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    rnn, inn = jz.knn.knn(t, k=k, **kw)
+    torch.cuda.synchronize()
+    return rnn.cpu().numpy(), inn.cpu().numpy()
+
+
+def _check(x, k, rnn, inn, oracle, boxsize=0.0):
+    r2o, io = oracle.knn(x, k, boxsize=boxsize)
+    assert np.array_equal(inn.astype(np.int64), io), f"{np.sum(inn != io)} ids differ"
+    assert np.array_equal(rnn, np.sqrt(r2o).astype(np.float32))
+
+
+@pytest.mark.parametrize("k", [1, 4, 8, 12, 16, 32, 53, 64, 133])
+def test_knn_k_sweep(dev, oracle, k):
+    """checks/tests/test_knn.py:62-67 sizes: k from 1 to 133 (several passes above 64)."""
+    x = uniform(1 << 14, seed=k)
+    rnn, inn = _knn(x, k, dev)
+    _check(x, k, rnn, inn, oracle)
+
+
+def test_knn_config1(dev, oracle):
+    """Plain kNN k=16 on 2^16 uniform points in the unit cube, non-periodic."""
+    x = uniform(1 << 16, seed=0)
+    rnn, inn = _knn(x, 16, dev)
+    _check(x, 16, rnn, inn, oracle)
+
+
+@pytest.mark.parametrize("boxsize", [0.03, 1.0, 170.0])
+def test_knn_periodic(dev, oracle, boxsize):
+    """checks/tests/test_knn.py:83-88"""
+    x = uniform(1 << 14, seed=3, lo=0.0, hi=boxsize)
+    x[x >= np.float32(boxsize)] = 0.0
+    rnn, inn = _knn(x, 16, dev, boxsize=boxsize)
+    _check(x, 16, rnn, inn, oracle, boxsize=boxsize)
+
+
+@pytest.mark.parametrize("lohi", [(-0.3, 0.7), (0.1, 0.4), (-1.0, 0.0), (0.0, 1e6), (-1.0, 1.0)])
+def test_knn_domains(dev, oracle, lohi):
+    """checks/tests/test_knn.py:55-60"""
+    x = uniform(1 << 14, seed=5, lo=lohi[0], hi=lohi[1])
+    rnn, inn = _knn(x, 16, dev)
+    _check(x, 16, rnn, inn, oracle)
+
+
+def _clustered_cfg():
+    """The default KNNConfig leaf allocation overflows on strongly clustered sets (as in the
+    reference); a larger alloc_fac_nodes is the documented remedy."""
+    import dataclasses
+    from jztree_amd.config import KNNConfig
+    c = KNNConfig()
+    return dataclasses.replace(c, tree=dataclasses.replace(c.tree, alloc_fac_nodes=2.0))
+
+
+def test_knn_clustered_and_duplicates(dev, oracle):
+    x = clustered(1 << 16, nhalos=128, seed=4)
+    x = np.concatenate([x, x[:4000]])  # exact duplicates: ties broken by index
+    rnn, inn = _knn(x, 32, dev, cfg=_clustered_cfg())
+    _check(x, 32, rnn, inn, oracle)
+
+
+def test_knn_output_orders(dev):
+    """checks/tests/test_knn.py:102-116: radii depend on the output order only; ids also on the input."""
+    import jztree_amd as jz
+    from jztree_amd.config import KNNConfig
+    x = torch.from_numpy(uniform(1 << 15, seed=6)).to(dev)
+    partz, idz, th = jz.tree.zsort_and_tree(x, KNNConfig().tree, data=torch.arange(len(x), device=dev))
+    rnn00, inn00 = jz.knn.knn(x, k=16)
+    rnn0z, inn0z = jz.knn.knn(x, k=16, output_order="z")
+    rnnzz, innzz = jz.knn.knn(partz, k=16, th=th)
+    idz = idz.long()
+    assert torch.equal(rnn00[idz], rnn0z)
+    assert torch.equal(rnn0z, rnnzz)
+    assert torch.equal(inn00[idz], idz[innzz.long()].to(inn00.dtype))
+    assert torch.equal(inn0z, innzz)
+
+
+def test_knn_distance_ulp(dev):
+    """Radii within 2 ulp of the float64 distance to the same neighbour."""
+    x = clustered(1 << 15, nhalos=64, seed=8)
+    rnn, inn = _knn(x, 16, dev, cfg=_clustered_cfg())
+    d64 = np.linalg.norm(x.astype(np.float64)[:, None, :] - x.astype(np.float64)[inn], axis=-1)
+    assert np.all(np.abs(rnn.astype(np.float64) - d64) <= 2 * np.spacing(rnn))

@@ -1,0 +1,254 @@
+"""Pins the CPU oracle (tests run on the CPU; no GPU needed).
+
+Each check below compares the oracle with something it does not share code with: the
+reference's literal golden vectors and documented known answers, an independent big-integer
+formulation of the Morton order, brute force, float64 brute force and scipy's cKDTree.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from conftest import (COSMO2D_LABELS40, COSMO2D_NGROUPS_GE10, ZSORT_GOLDEN_IN, ZSORT_GOLDEN_OUT, cosmo2d)
+from gen import clustered, uniform
+
+
+# ------------------------------------------------------------------------------------------------
+# z-order
+# ------------------------------------------------------------------------------------------------
+
+def test_zsort_golden(oracle):
+    """checks/tests/test_tree.py:85-108 (literal)."""
+    xin = np.array(ZSORT_GOLDEN_IN, np.float32)
+    xout = np.array(ZSORT_GOLDEN_OUT, np.float32)
+    pz, _ = oracle.zsort(xin)
+    assert np.array_equal(pz[:-1], xout[:-1])
+    assert np.all(np.isnan(pz[-1]))
+
+
+W = 280  # fixed-point bits per dim: covers 2^-149 .. 2^130
+
+
+def _bigint_key(p):
+    """Exact Morton key by a different route: per dim a top digit (-inf, <0, >=0, +inf) and the
+    fixed-point integer of |x| (complemented for negatives), digits first, then the magnitude bits
+    interleaved from the most significant level down with x first within a level."""
+    dim = len(p)
+    tops, mags = [], []
+    for v in p:
+        v = float(v)
+        if np.isinf(v):
+            t, m = (0 if v < 0 else 3), 0
+        else:
+            f = int(abs(v) * (2.0 ** 149))  # exact: every float32 is a multiple of 2^-149
+            t, m = (1, (1 << W) - 1 - f) if v < 0 else (2, f)
+        tops.append(t)
+        mags.append(m)
+    key = 0
+    for t in tops:
+        key = key * 4 + t
+    for b in range(W - 1, -1, -1):
+        for m in mags:
+            key = (key << 1) | ((m >> b) & 1)
+    return key
+
+
+@pytest.mark.parametrize("case", ["uniform", "dups", "negative", "specials", "2d"])
+def test_zsort_vs_bigint(oracle, case):
+    rng = np.random.default_rng(7)
+    if case == "uniform":
+        x = uniform(3000, seed=1)
+    elif case == "dups":
+        x = uniform(500, seed=2)
+        x = np.concatenate([x, x[:200], x[:100]])
+        x[::7, 1] = x[0, 1]  # exactly equal coordinates across points
+        x[::11, 0] = 0.0
+        x[::13, 2] = np.float32(1e-30)
+    elif case == "negative":
+        x = (rng.random((2000, 3)) * 4 - 2).astype(np.float32)
+        x[::17] = -x[::17]
+    elif case == "specials":
+        x = (rng.random((600, 3)) * 2 - 1).astype(np.float32)
+        x[::9, 0] = np.inf
+        x[::10, 1] = -np.inf
+        x[::23, 2] = np.float32(1e-42)  # denormal
+        x[::29, 1] = -0.0
+    else:
+        x = uniform(3000, dim=2, seed=3)
+    pz, idz = oracle.zsort(x)
+    keys = [(_bigint_key(x[i]), i) for i in range(len(x))]
+    ref = [i for _, i in sorted(keys)]
+    if case == "specials":
+        # -0.0 is ordered as +0.0 by this build; the big-int route treats -0.0 as non-negative too
+        pass
+    assert list(idz) == ref
+
+
+def test_zsort_agrees_with_reference_comparator(oracle):
+    """No adjacent pair of the oracle order is inverted by the literal z_pos_less."""
+    x = clustered(20000, nhalos=64, seed=5)
+    pz, _ = oracle.zsort(x)
+    bad = sum(oracle.z_pos_less(pz[i + 1], pz[i]) for i in range(len(pz) - 1))
+    assert bad == 0
+
+
+def test_msb_xor_literal_cases(oracle):
+    # closed forms of math.cuh:241-269
+    assert oracle.msb_xor(0.5, 0.75) == -2          # 0.5=1.0b-1, 0.75=1.1b-1: differ at 2^-2
+    assert oracle.msb_xor(1.0, 2.0) == 1            # exponents differ: larger exponent
+    assert oracle.msb_xor(-1.0, 1.0) == 128         # sign difference
+    assert oracle.msb_xor(np.nan, 1.0) == 128
+    assert oracle.msb_xor(0.5, 0.5) == -1 - 24      # equal bits: e + 8 - clz(0)
+    assert oracle.msb_xor(0.0, 0.0) == -126 - 24    # zero clamps to -126
+
+
+# ------------------------------------------------------------------------------------------------
+# tree
+# ------------------------------------------------------------------------------------------------
+
+def test_leaves_are_maximal_cells(oracle):
+    """Every leaf holds <= leaf_size particles and every leaf boundary splits a binary Morton cell
+    with more than leaf_size particles (counted by brute force from pairwise levels)."""
+    x = clustered(4000, nhalos=16, seed=11)
+    pz, _ = oracle.zsort(x)
+    leaf = 16
+    flags, lvl = oracle.flag_leaf_boundaries(pz, leaf_size=leaf)
+    spl = np.nonzero(flags)[0]
+    assert spl[0] == 0 and spl[-1] == len(pz)
+    assert np.all(np.diff(spl) <= leaf)
+    for b in spl[1:-1]:
+        L = oracle.msb_diff_level(pz[b - 1], pz[b])
+        # the binary cell split by boundary b: the contiguous run of points that differ from pz[b]
+        # at most at level L
+        lo = b - 1
+        while lo - 1 >= 0 and oracle.msb_diff_level(pz[lo - 1], pz[b]) <= L:
+            lo -= 1
+        hi = b
+        while hi + 1 < len(pz) and oracle.msb_diff_level(pz[hi + 1], pz[b]) <= L:
+            hi += 1
+        assert hi - lo + 1 > leaf
+
+
+def test_tree_hierarchy_invariants(oracle):
+    """checks/tests/test_tree.py:11-29 and :31-50 on the oracle's hierarchy."""
+    x = uniform(1 << 15, seed=4)
+    pz, _ = oracle.zsort(x)
+    th = oracle.build_tree_hierarchy(pz, 32, 6.0, 1024, 1.0)
+    spl0 = th.ispl_n2l.get(0)
+    nleaves = int(np.argmax(spl0))
+    for lv in range(1, th.num_planes()):
+        in2l = th.ispl_n2l.get(lv)
+        in2n = th.ispl_n2n.get(lv)
+        assert np.all(in2l[1:] >= in2l[:-1]) and np.all(in2n[1:] >= in2n[:-1])
+        assert np.max(in2l) == nleaves
+        assert np.max(in2n) == np.argmax(th.ispl_n2n.get(lv - 1))
+    npart = np.sum(th.npart(0))
+    assert npart == len(pz)
+    for i in range(th.num_planes()):
+        assert np.sum(th.npart(i)) == npart
+        lv = th.lvl.get(i, th.num(i))
+        assert np.all((lv >= -450) & (lv < 100))
+    # node boxes contain their particles and are minimal
+    ispl = np.sort(np.random.default_rng(42).integers(0, len(pz), 50))
+    lvl, cent, ext = oracle.get_node_geometry(pz, ispl[:-1], ispl[1:])
+    for i in range(len(ispl) - 1):
+        if ispl[i + 1] - ispl[i] <= 1:
+            continue
+        p = pz[ispl[i]:ispl[i + 1]]
+        assert np.all(cent[i] - ext[i] * 0.5 < p.min(0)) and np.all(p.max(0) < cent[i] + ext[i] * 0.5)
+        assert np.any(cent[i] - ext[i] * 0.25 >= p.min(0)) or np.any(cent[i] + ext[i] * 0.25 <= p.max(0))
+
+
+def test_regularization_level_closed_form(oracle):
+    """tree.py:160-187 by hand.  All weight at level -60: reference level -60, mean volume 1, the
+    first level with volume >= 20 is -60+5.  Half the weight at -40: the 90th percentile is -40,
+    mean volume (1000*2^-20 + 1000)/2000 ~ 0.5, first level with 2^(l+40) >= 10 is -36."""
+    lvl = np.full(1000, -60, np.int32)
+    w = np.ones(1000, np.int64)
+    assert oracle.find_regularization_level(lvl, w, 3) == -55
+    lvl2 = np.concatenate([lvl, np.full(5, -40, np.int32)])
+    w2 = np.concatenate([w, np.full(5, 200, np.int64)])
+    assert oracle.find_regularization_level(lvl2, w2, 3) == -36
+
+
+# ------------------------------------------------------------------------------------------------
+# kNN
+# ------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_knn_grid_equals_brute(oracle, periodic):
+    x = clustered(6000, nhalos=8, seed=9) if not periodic else uniform(6000, seed=9)
+    box = 1.0 if periodic else 0.0
+    r2a, ia = oracle.knn(x, 20, boxsize=box)
+    r2b, ib = oracle.knn(x, 20, boxsize=box, method="brute")
+    assert np.array_equal(r2a, r2b) and np.array_equal(ia, ib)
+
+
+def test_knn_canonical_r2_closed_form(oracle):
+    a = np.array([0.1, 0.2, 0.3], np.float32)
+    b = np.array([0.4, 0.25, 0.7], np.float32)
+    d = (a - b).astype(np.float32)
+    # fma(dz,dz, fma(dy,dy, dx*dx)) evaluated exactly in float64 then rounded once per step
+    r = np.float32(d[0] * d[0])
+    r = np.float32(np.float64(d[1]) * d[1] + np.float64(r))
+    r = np.float32(np.float64(d[2]) * d[2] + np.float64(r))
+    assert oracle.r2(a, b) == r
+    # periodic wrap of a difference beyond half the box
+    assert oracle.r2(np.array([0.05, 0, 0], np.float32), np.array([0.95, 0, 0], np.float32), 1.0) == \
+        np.float32(np.float32(0.05) - np.float32(0.95) + np.float32(1.0)) ** 2
+
+
+def test_knn_vs_float64_and_ckdtree(oracle):
+    """Radii match cKDTree (float64) to rtol 1e-4 and ids differ only at (near-)ties, as in
+    checks/tests/test_knn.py:35-52; neighbour sets agree with float64 brute force except where a
+    float64 distance gap below 1e-6 relative makes the order ambiguous."""
+    x = uniform(1 << 14, seed=1)
+    k = 16
+    r2, ii = oracle.knn(x, k)
+    rnn = np.sqrt(r2)
+    tree = cKDTree(x.astype(np.float64))
+    r64, i64 = tree.query(x.astype(np.float64), k=k)
+    assert np.all(rnn[:, 1:] >= rnn[:, :-1])
+    assert np.allclose(rnn, r64, rtol=1e-4)
+    assert np.sum(i64[:, :-1] != ii[:, :-1]) / len(x) <= 1e-2
+    # distances within 2 ulp of the float64 truth
+    ulp = np.spacing(rnn.astype(np.float32))
+    assert np.all(np.abs(rnn.astype(np.float64) - r64) <= 2 * ulp + 1e-30)
+
+
+# ------------------------------------------------------------------------------------------------
+# FoF
+# ------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fof_grid_equals_brute(oracle, periodic):
+    x = clustered(3000, nhalos=10, seed=3)
+    rl = 0.7 * len(x) ** (-1 / 3)
+    a = oracle.fof(x, rl, boxsize=1.0 if periodic else 0.0)
+    b = oracle.fof(x, rl, boxsize=1.0 if periodic else 0.0, method="brute")
+    assert np.array_equal(a, b)
+
+
+def test_fof_vs_float64_components(oracle):
+    """Groups equal the connected components of the float64 pair graph (no pair within 1e-6 of
+    the threshold in this sample), checked with fof_is_superset both ways (checks/tests/test_fof.py)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    x = uniform(20000, seed=12)
+    rl = 0.8 * len(x) ** (-1 / 3)
+    lab = oracle.fof(x, rl, boxsize=1.0)
+    tree = cKDTree(x.astype(np.float64), boxsize=1.0)
+    pairs = tree.query_pairs(rl, output_type="ndarray")
+    d = np.linalg.norm(((x[pairs[:, 0]] - x[pairs[:, 1]] + 0.5) % 1.0) - 0.5, axis=1)
+    assert np.sum(np.abs(d - rl) < 1e-6 * rl) == 0
+    g = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(len(x), len(x)))
+    _, comp = connected_components(g, directed=False)
+    assert oracle.fof_is_superset(lab, comp) and oracle.fof_is_superset(comp, lab)
+
+
+def test_cosmo2d_known_answer(oracle):
+    """docs/quickstart.md:115-131 (labels[0:40] in z-order) and :219-251 (164 groups >= 10)."""
+    pos = cosmo2d()
+    pz, _ = oracle.zsort(pos)
+    lab = oracle.fof(pz, 0.2 * 100. / 128, boxsize=100.0, key=np.arange(len(pz)))
+    assert list(lab[:40]) == COSMO2D_LABELS40
+    assert np.sum(np.bincount(lab, minlength=len(lab)) >= 10) == COSMO2D_NGROUPS_GE10

@@ -1,0 +1,105 @@
+"""Tree kernels and hierarchy on the GPU vs the oracle's restatement (bit-exact)."""
+import numpy as np
+import pytest
+import torch
+
+from gen import clustered, uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+@pytest.mark.parametrize("leaf", [16, 32, 48])
+def test_flag_leaf_boundaries(dev, oracle, leaf):
+    import jztree_amd as jz
+    x = clustered(1 << 16, nhalos=64, seed=leaf)
+    pz, _ = oracle.zsort(x)
+    print(f"[flag_leaf] leaf={leaf}", flush=True)
+    spl = jz.tree.detect_leaf_boundaries(_t(pz, dev), leaf_size=leaf)
+    torch.cuda.synchronize()
+    flags, lvl = oracle.flag_leaf_boundaries(pz, leaf_size=leaf)
+    ref = np.nonzero(flags)[0]
+    got = spl.cpu().numpy()
+    assert np.array_equal(got[:len(ref)], ref)
+    assert np.all(got[len(ref):] == len(pz))
+
+
+def test_node_boundaries_and_geometry(dev, oracle):
+    import jztree_amd as jz
+    x = uniform(1 << 15, seed=3)
+    pz, _ = oracle.zsort(x)
+    flags, _ = oracle.flag_leaf_boundaries(pz, leaf_size=32)
+    spl = np.nonzero(flags)[0]
+    reps = pz[spl[:-1]]
+    print("[find_node_boundaries]", flush=True)
+    lv, lb, rb = jz.tree.determine_znode_boundaries(_t(reps, dev), nleaves=len(spl) - 1)
+    olv, olb, orb = oracle.find_node_boundaries(reps, len(spl) - 1)
+    assert np.array_equal(lv.cpu().numpy(), olv)
+    assert np.array_equal(lb.cpu().numpy(), olb)
+    assert np.array_equal(rb.cpu().numpy(), orb)
+    ispl = np.sort(np.random.default_rng(42).integers(0, len(pz), 300)).astype(np.int32)
+    for upper in (False, True):
+        print(f"[get_node_geometry] upper={upper}", flush=True)
+        g = jz.tree.get_node_geometry(_t(pz, dev), _t(ispl[:-1], dev), _t(ispl[1:], dev), upper_extent=upper)
+        o = oracle.get_node_geometry(pz, ispl[:-1], ispl[1:], upper_extent=upper)
+        for a, b in zip(g, o):
+            assert np.array_equal(a.cpu().numpy(), b, equal_nan=True)
+
+
+@pytest.mark.parametrize("cfg_name", ["knn", "fof", "default"])
+def test_build_tree_hierarchy(dev, oracle, cfg_name):
+    import jztree_amd as jz
+    from jztree_amd.config import FofConfig, KNNConfig, TreeConfig
+    import dataclasses
+    cfg = {"knn": KNNConfig().tree, "fof": FofConfig().tree, "default": TreeConfig()}[cfg_name]
+    cfg = dataclasses.replace(cfg, alloc_fac_nodes=1.3)  # strongly clustered input needs more leaves
+    x = clustered(1 << 18, nhalos=512, seed=7)
+    pz, _ = oracle.zsort(x)
+    reg = None if cfg.regularization is None else (cfg.regularization.max_volume_fac,
+                                                   cfg.regularization.regularize_percentile)
+    print(f"[build_tree_hierarchy] {cfg_name}", flush=True)
+    th = jz.tree.build_tree_hierarchy(_t(pz, dev), cfg)
+    jz.checks.flush()
+    oth = oracle.build_tree_hierarchy(pz, cfg.max_leaf_size, cfg.coarse_fac, cfg.stop_coarsen,
+                                      cfg.alloc_fac_nodes, regularize=reg)
+    assert th.size_leaves == oth.size_leaves
+    for name in ("ispl_n2n", "ispl_n2l", "lvl", "geom_cent"):
+        a, b = getattr(th, name), getattr(oth, name)
+        assert np.array_equal(a.ispl.cpu().numpy(), b.ispl), name
+        assert np.array_equal(a.data.cpu().numpy(), b.data, equal_nan=True), name
+        assert np.array_equal(a.fill_values.cpu().numpy(), b.fill_values, equal_nan=True), name
+    assert np.array_equal(th.ispl_l2p_per_type.cpu().numpy(), oth.ispl_l2p_per_type)
+
+
+def test_tree_nan_padding_invariance(dev):
+    """checks/tests/test_tree.py:135-153 (geometric tree): NaN padding leaves the hierarchy unchanged."""
+    import jztree_amd as jz
+    from jztree_amd.config import TreeConfig
+    from jztree_amd.data import Pos
+    x = clustered(1 << 16, nhalos=64, seed=2)
+    pz, _ = jz.tree.zsort(_t(x, dev))
+    pad = torch.cat([pz, torch.full((1356, 3), float("nan"), device=dev)])
+    th1 = jz.tree.build_tree_hierarchy(pz, TreeConfig())
+    th2 = jz.tree.build_tree_hierarchy(Pos(pos=pad, num_total=len(pz)), TreeConfig())
+    for name in ("ispl_n2n", "ispl_n2l", "lvl", "geom_cent"):
+        a, b = getattr(th1, name), getattr(th2, name)
+        assert torch.equal(a.ispl, b.ispl), name
+        assert torch.equal(torch.nan_to_num(a.data, nan=-7.0), torch.nan_to_num(b.data, nan=-7.0)), name
+
+
+def test_leaf_allocation_overflow_matches_oracle(dev, oracle):
+    """tree.py:251-255 error path: the regularised kNN tree of a strongly clustered set overflows the
+    default leaf allocation; the GPU path raises with the same counts as the oracle."""
+    import jztree_amd as jz
+    from jztree_amd.config import KNNConfig
+    x = clustered(1 << 18, nhalos=512, seed=7)
+    pz, _ = oracle.zsort(x)
+    with pytest.raises(RuntimeError) as e_ref:
+        oracle.build_tree_hierarchy(pz, 48, 8.0, 2048, 1.0, regularize=(20.0, 90.0))
+    with pytest.raises(RuntimeError) as e_gpu:
+        jz.tree.build_tree_hierarchy(_t(pz, dev), KNNConfig().tree)
+        jz.checks.flush()
+    assert str(e_ref.value).split(".")[0] in str(e_gpu.value)
