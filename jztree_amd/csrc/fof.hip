@@ -3,8 +3,10 @@
 // Union-find over int32 labels with parent[x] <= x.  Hooking is lock-free with atomicMin: to join
 // roots hi > lo, atomicMin(&parent[hi], lo) returns the old parent; if hi was no longer a root the
 // displaced link (old) is re-joined with lo, so no link is lost (the reference retries a CAS,
-// fof.cuh:22-36; both leave the group minimum as the root).  Pointer reads are coherent relaxed
-// loads (L2), so a retry always observes progress.  Contraction = pointer jumping to the root.
+// fof.cuh:22-36; both leave the group minimum as the root).  Pointer walks use ordinary (cached)
+// loads: a stale value is still an ancestor (parents only decrease), and every retry restarts from
+// the parent the atomic returned, so the larger root of the pair strictly decreases and the loop
+// ends.  Contraction = pointer jumping to the root.
 //
 //  NodeToChildLabel  fof.cuh:42-71     children of an unlinked, too-large node label themselves,
 //                                      all others inherit the first child of the parent's root
@@ -25,7 +27,7 @@ namespace {
 constexpr float kSlackHi = 1.0f + kBoundSlack;  // inflate for "may link" tests
 constexpr float kSlackLo = 1.0f - kBoundSlack;  // deflate for "must link" tests
 
-__device__ __forceinline__ int uf_parent(const int32_t* p, int x) { return ld_acq(p + x); }
+__device__ __forceinline__ int uf_parent(const int32_t* p, int x) { return p[x]; }
 
 __device__ __forceinline__ int uf_find(const int32_t* p, int x) {
   while (true) {
@@ -169,6 +171,7 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
     const bool valid = qoff + lane < q1;
     Vec<D, T> xq = load_vec<D, T>(pos, iq);
     const int gq = ig[iq];
+    int glast = -1;
     for (int e = e0; e < e1; e++) {
       const int lt = il[e];
       if (lt < lq) continue;  // each leaf pair once
@@ -183,8 +186,15 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
         const int nl = min(t1 - toff, 64);
         if (valid) {
           for (int j = 0; j < nl; j++) {
+            const int gj = s_g[j];
+            // labels are snapshots of set members; a label equal to q's own, or to one q was
+            // already joined with, belongs to q's set by now
+            if (gj == gq || gj == glast) continue;
             T r2 = dist2<D, T>(xq, s_x[j], boxsize);
-            if (gq != s_g[j] && r2 <= (T)r2link) uf_link(ig, iq, toff + j);
+            if (r2 <= (T)r2link) {
+              uf_link(ig, iq, toff + j);
+              glast = gj;
+            }
           }
         }
         __syncthreads();
