@@ -133,6 +133,22 @@ int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32
                       void* tmp, size_t tmp_bytes, int64_t size_leaves_query, int64_t size_part_query, int k,
                       float boxsize, int dim, int dtype);
 
+/* KnnFused -- no single reference counterpart: replaces the leaf-plane KnnNode2Node together with
+ * KnnLeaf2Leaf (knn.cuh:309-356, :602-694; callers knn.py:186 and :325) by one kernel that walks
+ * the plane-1 lists (parent_ilist_*, r2-sorted, as produced by jzt_knn_node2node for plane 1) and
+ * evaluates child leaves on the fly.  parent_child_spl: plane-1 node -> leaf splits; leaf_parent:
+ * leaf -> plane-1 node; leaf_nodes: (size_leaves, dim+1) leaf centre + level bits; leaf_spl: leaf ->
+ * particle splits; xz: z-ordered positions (sources = queries).  src_ids (optional): reported id
+ * per particle; dst_rows (optional): output row per query.  init_r2 (optional, per leaf): starting
+ * bound; queries left with < k neighbours inside it are flagged in `retry` (n int32, required with
+ * init_r2) and recomputed without bound before return.  k <= 64.  Results equal the unfused ops. */
+int jzt_knn_fused(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                  const float* parent_ilist_r2, int64_t parent_ilist_size, const int32_t* parent_child_spl,
+                  const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* leaf_spl, const void* xz,
+                  const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2, int32_t* retry,
+                  void* knn_rad, int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize,
+                  int dim, int dtype);
+
 /* ------------------------------------------------------------------------------------------ */
 /* friends-of-friends                                                                           */
 /* ------------------------------------------------------------------------------------------ */

@@ -40,6 +40,8 @@ _SIGNATURES = {
     "jzt_knn_leaf2leaf_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_knn_leaf2leaf": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _i64, _i64, _i32, _f32,
                            _i32, _i32], _i32),
+    "jzt_knn_fused": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32,
+                       _i32, _i32], _i32),
     "jzt_node_to_child_label": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
     "jzt_fof_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
     "jzt_fof_leaf2leaf": ([_p, _p, _p, _i64, _p, _p, _p, _p, _i64, _i64, _f32, _f32, _i32, _i32], _i32),

@@ -69,7 +69,7 @@ _knn_leaf2leaf.jit = _knn_leaf2leaf
 
 
 def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNum, k: int = 32,
-                         boxsize: float = 0.) -> InteractionList:
+                         boxsize: float = 0., with_radii: bool = False):
     """knn.py:62-110 -> KnnNode2Node."""
     boxsize = 0. if boxsize is None else boxsize
     assert ilist.ispl.shape[0] == spl_parent.shape[0]
@@ -89,12 +89,18 @@ def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNu
     raise_if(ispl[-1] > il.numel(),
              "The interaction list allocation is too small. (need: {n1}, have: {n2})\n"
              "Hint: increase alloc_fac_ilist", n1=ispl[-1], n2=il.numel())
+    if with_radii:
+        return InteractionList(ispl, il, rad2=ilr2), radii
     return InteractionList(ispl, il, rad2=ilr2)
 
 
 def _knn_dual_walk(th: TreeHierarchy, k: int, boxsize: float | None = None,
-                   alloc_fac_ilist: float = 32.) -> InteractionList:
-    """knn.py:132-193 (single GPU): from the dense top-plane list down to leaf-leaf lists."""
+                   alloc_fac_ilist: float = 32., stop_level: int = 0, return_parents: bool = False):
+    """knn.py:132-193 (single GPU): from the dense top-plane list down to the lists of plane
+    `stop_level` (0 = leaf-leaf lists, as in the reference).
+
+    With return_parents, also returns (rmax2 of the plane's nodes or None at the top, the
+    node-to-child splits of that plane)."""
     nlevels = th.num_planes()
     size = th.size()
     spl, ilist, nsup = grouped_dense_interaction_list(
@@ -102,23 +108,69 @@ def _knn_dual_walk(th: TreeHierarchy, k: int, boxsize: float | None = None,
     ilist.rad2 = torch.zeros(ilist.isrc.shape, dtype=torch.float32, device=ilist.isrc.device)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
-    for i in range(nlevels):
+    rmax2 = None
+    for i in range(nlevels - stop_level):
         level = nlevels - i - 1
         parent_spl = spl_n2n.get(level + 1, size + 1)
         node_data = PosLvlNum(pos=th.geom_cent.get(level, size), lvl=th.lvl.get(level, size),
                               npart=th.npart(level, size=size, ptype=0))
-        ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize)
+        ilist, rmax2 = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize, with_radii=True)
+    if return_parents:
+        return ilist, rmax2, spl_n2n.get(stop_level, size + 1)
     return ilist
+
+
+def _leaf_parent(n2c: torch.Tensor, size: int) -> torch.Tensor:
+    """Parent slot of every leaf slot.  The padded tail of the splits piles up at the leaf count,
+    so empty leaf slots would map to `size`; they are clamped into range (their kernels exit
+    before reading it)."""
+    from .tools import inverse_of_splits
+    return torch.clamp(inverse_of_splits(n2c, size), 0, size - 1)
+
+
+def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, alloc_fac_ilist: float = 256.,
+               src_ids=None, dst_rows=None, out_rows: int | None = None, init_r2=None):
+    """Leaf plane as one fused kernel (extension; see csrc/knn_fused.hip): walks the plane-1 lists
+    and evaluates child leaves on the fly instead of materialising leaf-leaf lists.
+
+    init_r2: per-leaf starting bound (default: the parent's guaranteed radius).  Queries that hold
+    fewer than k neighbours within it are redone with an unbounded search inside the call.
+    Results are identical to _knn_dual_walk + _knn_leaf2leaf with the same src_ids."""
+    dev = posz.device
+    n = len(posz)
+    size = th.size()
+    nlevels = th.num_planes()
+    ilist_p, rmax_p, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                                          return_parents=True)
+    leaf_parent = _leaf_parent(n2c, size)
+    leaf_nodes = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=leaf_parent).pos_lvl()
+    l2p = th.splits_leaf_to_part()
+    if init_r2 is None and rmax_p is not None:
+        init_r2 = rmax_p[leaf_parent.long()]
+    rows = n if out_rows is None else out_rows
+    rnn = torch.empty((rows, k), dtype=posz.dtype, device=dev)
+    inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
+    retry = torch.empty(n, dtype=torch.int32, device=dev) if init_r2 is not None else None
+    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+    ir2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
+    L.call("jzt_knn_fused", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
+           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p.contiguous()),
+           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(ir2), L.ptr(retry), L.ptr(rnn), L.ptr(inn),
+           size, n, int(k), float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    return rnn, inn, retry
 
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,
         result: str = "rad_globalidx", reduce_func: Callable | None = None, output_order: str = "input",
-        cfg: KNNConfig = KNNConfig()):
+        cfg: KNNConfig = KNNConfig(), fused: bool = False):
     """Exact k nearest neighbours (knn.py:198-347).
 
     Returns (rnn, inn) by default: radii and indices of the k nearest neighbours of every particle,
     ascending by (distance, index).  result: "_"-separated keys among rad, drad, rankidx,
     globalidx, part, reduce, or a particle attribute name.  output_order: "input" or "z".
+    fused (extension): use the fused leaf-plane kernel for k <= 64 (same results; False runs the
+    reference's KnnNode2Node + KnnLeaf2Leaf decomposition).
     """
     assert output_order in ("z", "input")
     if part_query is not None:
@@ -139,12 +191,21 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
             originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
             num_origin = len(get_pos(partz))
 
-        ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
         spl = th.splits_leaf_to_part(ptype=0)
         posz = get_pos(partz)
-        # neighbours are reported (and tie-broken) by their origin index directly
-        rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,
-                                     src_ids=originz.idx)
+        keys = result.split("_")
+        # rows can be written straight to input order when only radii/indices are requested
+        direct = (output_order == "input" and set(keys) <= {"rad", "globalidx"}
+                  and isinstance(num_origin, int) and num_origin == size)
+        if fused and k <= 64:
+            # neighbours are reported (and tie-broken) by their origin index directly
+            rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
+                                        src_ids=originz.idx, dst_rows=originz.idx if direct else None)
+        else:
+            ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
+            rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,
+                                         src_ids=originz.idx)
+            direct = False
 
         res = []
         zidx_cache = {}
@@ -156,7 +217,7 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
                 zidx_cache["z"] = inv[inn_g.long().clamp(min=0, max=size)]
             return zidx_cache["z"]
 
-        for key in result.split("_"):
+        for key in keys:
             if key == "rad":
                 res.append(rnnz)
             elif key == "drad":
@@ -181,7 +242,7 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
                 raise ValueError(f"Invalid result key {key}")
         res = tuple(res) if len(res) > 1 else res[0]
 
-        if output_order == "input":
+        if output_order == "input" and not direct:
             idx = originz.idx
             inverse = masked_inverse(idx, mask=torch.arange(len(idx), device=dev) < num_origin)
             res = tree_map_by_len(lambda v: v[inverse.long().clamp(max=size - 1)], res, size)

@@ -251,7 +251,8 @@ def _delete_rows(x: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
     n = x.shape[0]
     m = idx.numel()
     mark = torch.zeros(n, dtype=torch.int64, device=x.device)
-    mark[idx.long()] = 1
+    # in range even when an (already recorded) allocation overflow made the indices meaningless
+    mark[torch.clamp(idx.long(), 0, n - 1)] = 1
     dest = torch.arange(n, device=x.device) - (torch.cumsum(mark, 0) - mark)
     dest = torch.where(mark.bool(), torch.full_like(dest, n - m), dest)
     out = torch.empty((n - m + 1,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)

@@ -532,6 +532,134 @@ void orc_knn_grid_f32(const float* xs, const int64_t* ids, int64_t ns, const flo
     grid_free(&g);
 }
 
+/*
+ * Exact kNN with a k-d tree (for clustered data, where a uniform grid degenerates).  Nodes split
+ * the widest bounding-box dim at the median (quickselect); leaves hold <= 16 points.  A subtree is
+ * skipped once the float64 lower bound of its box distance, squared and with a 1e-5 relative
+ * margin for FP32 rounding of r2, exceeds the current k-th r2.  Results equal orc_knn_brute_f32.
+ */
+typedef struct { float lo[3], hi[3]; int64_t start, end; int64_t left, right; } kdnode_t;
+typedef struct { kdnode_t* nodes; int64_t nnodes, cap; int64_t* idx; const float* x; int dim; } kdtree_t;
+
+static void kd_box(kdtree_t* t, int64_t s, int64_t e, float* lo, float* hi) {
+    for (int d = 0; d < t->dim; d++) { lo[d] = INFINITY; hi[d] = -INFINITY; }
+    for (int64_t i = s; i < e; i++)
+        for (int d = 0; d < t->dim; d++) {
+            float v = t->x[t->idx[i] * t->dim + d];
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = v > hi[d] ? v : hi[d];
+        }
+}
+static void kd_select(kdtree_t* t, int64_t s, int64_t e, int64_t kth, int d) {
+    int64_t* a = t->idx;
+    const float* x = t->x;
+    int dim = t->dim;
+    while (e - s > 1) {
+        float piv = x[a[s + (e - s) / 2] * dim + d];
+        int64_t i = s, j = e - 1;
+        while (i <= j) {
+            while (x[a[i] * dim + d] < piv) i++;
+            while (x[a[j] * dim + d] > piv) j--;
+            if (i <= j) { int64_t tmp = a[i]; a[i] = a[j]; a[j] = tmp; i++; j--; }
+        }
+        if (kth <= j) e = j + 1;
+        else if (kth >= i) s = i;
+        else return;
+    }
+}
+static int64_t kd_alloc(kdtree_t* t) {
+    int64_t id;
+    #pragma omp critical(kdalloc)
+    { id = t->nnodes++; }
+    return id;
+}
+static void kd_build_rec(kdtree_t* t, int64_t node, int64_t s, int64_t e, int depth) {
+    kdnode_t* nd = &t->nodes[node];
+    kd_box(t, s, e, nd->lo, nd->hi);
+    nd->start = s; nd->end = e; nd->left = nd->right = -1;
+    if (e - s <= 16) return;
+    int bd = 0;
+    float bw = -1.f;
+    for (int d = 0; d < t->dim; d++) { float w = nd->hi[d] - nd->lo[d]; if (w > bw) { bw = w; bd = d; } }
+    if (!(bw > 0.f)) return;  /* all points identical: keep as a (large) leaf */
+    int64_t mid = s + (e - s) / 2;
+    kd_select(t, s, e, mid, bd);
+    int64_t l = kd_alloc(t), r = kd_alloc(t);
+    t->nodes[node].left = l; t->nodes[node].right = r;
+    if (depth < 6) {
+        #pragma omp task
+        kd_build_rec(t, l, s, mid, depth + 1);
+        #pragma omp task
+        kd_build_rec(t, r, mid, e, depth + 1);
+        #pragma omp taskwait
+    } else {
+        kd_build_rec(t, l, s, mid, depth + 1);
+        kd_build_rec(t, r, mid, e, depth + 1);
+    }
+}
+static void kd_build(kdtree_t* t, const float* x, int64_t n, int dim) {
+    t->x = x; t->dim = dim;
+    t->cap = 2 * (n / 8 + 2) + 16;
+    t->nodes = (kdnode_t*)malloc(sizeof(kdnode_t) * (size_t)t->cap);
+    t->idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) t->idx[i] = i;
+    t->nnodes = 1;
+    #pragma omp parallel
+    #pragma omp single
+    kd_build_rec(t, 0, 0, n, 0);
+}
+static double kd_mind2(const kdnode_t* nd, const float* q, int dim, float boxsize) {
+    double r = 0.0;
+    for (int d = 0; d < dim; d++) {
+        double v = q[d], lo = nd->lo[d], hi = nd->hi[d], dd = 0.0;
+        if (v < lo) dd = lo - v; else if (v > hi) dd = v - hi;
+        if (boxsize > 0.f && dd > 0.0) {
+            double L = boxsize;
+            double a = (v + L < lo) ? lo - (v + L) : ((v + L > hi) ? v + L - hi : 0.0);
+            double b = (v - L < lo) ? lo - (v - L) : ((v - L > hi) ? v - L - hi : 0.0);
+            if (a < dd) dd = a;
+            if (b < dd) dd = b;
+        }
+        r += dd * dd;
+    }
+    return r;
+}
+static void kd_query(const kdtree_t* t, int64_t node, const float* q, const int64_t* ids, int k, float boxsize,
+                     float* r2s, int64_t* ii) {
+    const kdnode_t* nd = &t->nodes[node];
+    if (kd_mind2(nd, q, t->dim, boxsize) > (double)r2s[k - 1] * (1.0 + 1e-5)) return;
+    if (nd->left < 0) {
+        for (int64_t a = nd->start; a < nd->end; a++) {
+            int64_t si = t->idx[a];
+            float r2 = r2_f32(q, t->x + si * t->dim, t->dim, boxsize);
+            topk_insert(r2s, ii, k, r2, ids ? ids[si] : si);
+        }
+        return;
+    }
+    double dl = kd_mind2(&t->nodes[nd->left], q, t->dim, boxsize);
+    double dr = kd_mind2(&t->nodes[nd->right], q, t->dim, boxsize);
+    if (dl <= dr) {
+        kd_query(t, nd->left, q, ids, k, boxsize, r2s, ii);
+        kd_query(t, nd->right, q, ids, k, boxsize, r2s, ii);
+    } else {
+        kd_query(t, nd->right, q, ids, k, boxsize, r2s, ii);
+        kd_query(t, nd->left, q, ids, k, boxsize, r2s, ii);
+    }
+}
+void orc_knn_kdtree_f32(const float* xs, const int64_t* ids, int64_t ns, const float* xq, int64_t nq,
+                        int dim, int k, float boxsize, float* r2_out, int64_t* id_out) {
+    kdtree_t t;
+    kd_build(&t, xs, ns, dim);
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t q = 0; q < nq; q++) {
+        float* r2s = r2_out + q * k;
+        int64_t* ii = id_out + q * k;
+        for (int j = 0; j < k; j++) { r2s[j] = INFINITY; ii[j] = INT64_MAX; }
+        if (ns > 0) kd_query(&t, 0, xq + q * dim, ids, k, boxsize, r2s, ii);
+    }
+    free(t.nodes); free(t.idx);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /*                                  friends-of-friends                                        */
 /* ------------------------------------------------------------------------------------------ */

@@ -71,6 +71,7 @@ def lib():
                     _c.c_float, _f32p, _i64p]
         L.orc_knn_brute_f32.argtypes = knn_args
         L.orc_knn_grid_f32.argtypes = knn_args
+        L.orc_knn_kdtree_f32.argtypes = knn_args
         fof_args = [_f32p, _c.c_int64, _c.c_int, _c.c_float, _c.c_float, _c.c_void_p, _i64p]
         L.orc_fof_f32.argtypes = fof_args
         L.orc_fof_brute_f32.argtypes = fof_args
@@ -405,7 +406,7 @@ def r2(a, b, boxsize=0.0) -> np.float32:
     return np.float32(lib().orc_r2_f32(a, b, len(a), float(boxsize)))
 
 
-def knn(xs, k, xq=None, boxsize=0.0, ids=None, method="grid"):
+def knn(xs, k, xq=None, boxsize=0.0, ids=None, method="kdtree"):
     """Exact kNN: per query the k smallest (canonical r2, id).  Returns (r2 (nq,k) f32, id (nq,k) i64).
 
     ids: the id reported (and used for tie-breaks) for each source point; default = its index."""
@@ -419,7 +420,8 @@ def knn(xs, k, xq=None, boxsize=0.0, ids=None, method="grid"):
     if ids is not None:
         ids = np.ascontiguousarray(ids, np.int64)
         idp = ids.ctypes.data
-    fn = lib().orc_knn_brute_f32 if method == "brute" else lib().orc_knn_grid_f32
+    fn = {"brute": lib().orc_knn_brute_f32, "grid": lib().orc_knn_grid_f32,
+          "kdtree": lib().orc_knn_kdtree_f32}[method]
     fn(xs, idp, ns, xq, nq, dim, k, float(boxsize), r2o, io)
     return r2o, io
 

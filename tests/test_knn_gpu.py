@@ -94,3 +94,49 @@ def test_knn_distance_ulp(dev):
     rnn, inn = _knn(x, 16, dev, cfg=_clustered_cfg())
     d64 = np.linalg.norm(x.astype(np.float64)[:, None, :] - x.astype(np.float64)[inn], axis=-1)
     assert np.all(np.abs(rnn.astype(np.float64) - d64) <= 2 * np.spacing(rnn))
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered_periodic", "small_one_plane", "k64", "two_planes"])
+def test_fused_equals_reference_decomposition(dev, oracle, case):
+    """The fused leaf-plane kernel returns exactly what KnnNode2Node + KnnLeaf2Leaf return, and both
+    equal the oracle."""
+    import jztree_amd as jz
+    if case == "uniform":
+        x, k, box = uniform(1 << 17, seed=11), 16, 0.0
+    elif case == "clustered_periodic":
+        x, k, box = clustered(1 << 17, nhalos=256, seed=12), 32, 1.0
+    elif case == "small_one_plane":
+        x, k, box = uniform(3000, seed=13), 8, 0.0
+    elif case == "k64":
+        x, k, box = clustered(1 << 16, nhalos=64, seed=14), 64, 1.0
+    else:  # 2^19 points: two tree planes, so the parent lists and bounds come from KnnNode2Node
+        x, k, box = clustered(1 << 19, nhalos=256, seed=16), 32, 1.0
+    t = torch.from_numpy(x).to(dev)
+    print(f"[fused] {case} n={len(x)} k={k}", flush=True)
+    cfg = _clustered_cfg()
+    r1, i1 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=True)
+    r2_, i2 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=False)
+    torch.cuda.synchronize()
+    assert torch.equal(i1, i2) and torch.equal(r1, r2_)
+    _check(x, k, r1.cpu().numpy(), i1.cpu().numpy(), oracle, boxsize=box)
+
+
+def test_fused_bound_and_retry(dev, oracle):
+    """A starting bound that is too small for some leaves triggers the in-call retry; results are
+    unchanged."""
+    import jztree_amd as jz
+    x = clustered(1 << 16, nhalos=64, seed=15)
+    t = torch.from_numpy(x).to(dev)
+    cfg = _clustered_cfg()
+    partz, originz, th = jz.tree.zsort_and_tree(t, cfg.tree, data=jz.data.RankIdx(
+        rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev)))
+    size = th.size()
+    tiny = torch.full((size,), 1e-12, dtype=torch.float32, device=dev)
+    tiny[::2] = float("inf")
+    r1, i1, retry = jz.knn._knn_fused(th, partz, 16, alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx,
+                                      init_r2=tiny)
+    r2_, i2, _ = jz.knn._knn_fused(th, partz, 16, alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx)
+    jz.checks.flush()
+    torch.cuda.synchronize()
+    assert int(retry.sum()) > 0
+    assert torch.equal(i1, i2) and torch.equal(r1, r2_)

@@ -175,10 +175,12 @@ def test_regularization_level_closed_form(oracle):
 # ------------------------------------------------------------------------------------------------
 
 @pytest.mark.parametrize("periodic", [False, True])
-def test_knn_grid_equals_brute(oracle, periodic):
-    x = clustered(6000, nhalos=8, seed=9) if not periodic else uniform(6000, seed=9)
+@pytest.mark.parametrize("method", ["grid", "kdtree"])
+def test_knn_equals_brute(oracle, periodic, method):
+    x = clustered(6000, nhalos=8, seed=9)
+    x = np.concatenate([x, x[:300]])  # exact duplicates: ties by index
     box = 1.0 if periodic else 0.0
-    r2a, ia = oracle.knn(x, 20, boxsize=box)
+    r2a, ia = oracle.knn(x, 20, boxsize=box, method=method)
     r2b, ib = oracle.knn(x, 20, boxsize=box, method="brute")
     assert np.array_equal(r2a, r2b) and np.array_equal(ia, ib)
 
@@ -252,3 +254,21 @@ def test_cosmo2d_known_answer(oracle):
     lab = oracle.fof(pz, 0.2 * 100. / 128, boxsize=100.0, key=np.arange(len(pz)))
     assert list(lab[:40]) == COSMO2D_LABELS40
     assert np.sum(np.bincount(lab, minlength=len(lab)) >= 10) == COSMO2D_NGROUPS_GE10
+
+
+def test_leaf_parent_in_range(oracle):
+    """The leaf -> parent map used by the fused kNN stays inside the parent arrays for every leaf
+    slot (empty slots included) and is exact for real leaves; checked on a two-plane tree."""
+    import torch
+    from jztree_amd.knn import _leaf_parent
+    x = clustered(1 << 19, nhalos=256, seed=1)
+    pz, _ = oracle.zsort(x)
+    th = oracle.build_tree_hierarchy(pz, 48, 8.0, 2048, 2.0, regularize=(20.0, 90.0))
+    assert th.num_planes() == 2
+    size = len(th.ispl_n2n.data) - 1
+    n2c = th.ispl_n2n.get(1, size + 1).astype(np.int32)
+    lp = _leaf_parent(torch.from_numpy(n2c), size).numpy()
+    assert lp.min() >= 0 and lp.max() < size
+    nl = th.num(0)
+    ref = np.searchsorted(n2c[:th.num(1) + 1], np.arange(nl), side="right") - 1
+    assert np.array_equal(lp[:nl], ref)
