@@ -61,10 +61,12 @@ def main():
     print(f"# generated {n} points in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     x = torch.from_numpy(xh).to(dev)
 
+    # tree settings for this workload: leaves of <= 64 points (one per lane of a wave64) and a
+    # leaf allocation large enough for strongly clustered sets; results do not depend on either
     kcfg = jz.config.KNNConfig()
-    kcfg = dataclasses.replace(kcfg, tree=dataclasses.replace(kcfg.tree, alloc_fac_nodes=2.0))
+    kcfg = dataclasses.replace(kcfg, tree=dataclasses.replace(kcfg.tree, alloc_fac_nodes=2.0, max_leaf_size=64))
     fcfg = jz.config.FofConfig()
-    fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0))
+    fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0, max_leaf_size=64))
     rlink = args.b * n ** (-1.0 / 3.0)
 
     def ev():
