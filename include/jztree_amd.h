@@ -133,6 +133,22 @@ int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32
                       void* tmp, size_t tmp_bytes, int64_t size_leaves_query, int64_t size_part_query, int k,
                       float boxsize, int dim, int dtype);
 
+/* KnnLeaf2Leaf with three optional extensions (no reference counterpart; jzt_knn_leaf2leaf is this
+ * call with all three NULL and out_rows = size_part_query):
+ *   dst_rows (size_part_query int32): output row of each query (rows outside [0, out_rows) are not
+ *     written), so results can land in input order directly;
+ *   init_r2 (size_leaves_query float, k <= 64): starting bound per query leaf.  Only candidates with
+ *     r2 <= init_r2 enter the top-k.  The caller guarantees that each query leaf's list holds every
+ *     leaf within init_r2 of it; then every query that finds k neighbours is exact;
+ *   retry (size_part_query int32, with init_r2): 1 for queries left with fewer than k neighbours
+ *     inside the bound (their rows hold sentinels and must be recomputed), else 0. */
+int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
+                         const float* ilist_r2, int64_t ilist_size, const int32_t* splT, const void* xT,
+                         const int32_t* splQ, const void* xQ, const int32_t* src_ids, const int32_t* dst_rows,
+                         const float* init_r2, int32_t* retry, void* knn_rad, int32_t* knn_id, void* tmp,
+                         size_t tmp_bytes, int64_t size_leaves_query, int64_t size_part_query, int64_t out_rows,
+                         int k, float boxsize, int dim, int dtype);
+
 /* KnnFused -- no single reference counterpart: replaces the leaf-plane KnnNode2Node together with
  * KnnLeaf2Leaf (knn.cuh:309-356, :602-694; callers knn.py:186 and :325) by one kernel that walks
  * the plane-1 lists (parent_ilist_*, r2-sorted, as produced by jzt_knn_node2node for plane 1) and

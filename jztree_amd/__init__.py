@@ -6,7 +6,7 @@ as hand-written HIP kernels for AMD Instinct MI355X (gfx950), behind jztree's Py
     rnn, inn = jz.knn.knn(pos, k=16)               # exact kNN, input order
     posz, igroup = jz.fof.fof_labels(pos, rlink, boxsize=1.0)
 
-Importing requires the built library jztree_amd/libjztree_amd.so (python -m jztree_amd.build);
+Importing requires the built library jztree_amd/libjztree_amd.so (python jztree_amd/build.py);
 there is no CPU fallback.
 """
 from . import config, data, tools  # noqa: F401

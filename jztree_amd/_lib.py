@@ -40,6 +40,8 @@ _SIGNATURES = {
     "jzt_knn_leaf2leaf_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_knn_leaf2leaf": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _i64, _i64, _i32, _f32,
                            _i32, _i32], _i32),
+    "jzt_knn_leaf2leaf_ex": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _i64, _i64,
+                              _i64, _i32, _f32, _i32, _i32], _i32),
     "jzt_knn_fused": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32,
                        _i32, _i32], _i32),
     "jzt_node_to_child_label": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
@@ -51,7 +53,7 @@ _SIGNATURES = {
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m jztree_amd.build` (hipcc, gfx950). "
+            f"{LIB_PATH} is missing: build it with `python jztree_amd/build.py` (hipcc, gfx950). "
             "jztree_amd has no CPU fallback.")
     lib = ctypes.CDLL(LIB_PATH)
     for name, (args, res) in _SIGNATURES.items():

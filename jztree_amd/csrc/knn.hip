@@ -320,21 +320,32 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ float key_r2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
 
-// Insert `key` into the ascending list best[0..K) (caller guarantees key < best[K-1]).  Top-down:
-// best[i] takes best[i-1] while key < best[i-1], takes key at the insertion point, and keeps its
-// value below it.  Every 8 steps the wave leaves the (fully unrolled, statically indexed) network
-// once no active lane still shifts; checking less often keeps the register count low.
+// Insert `key` into the ascending list best[0..K) (caller guarantees key < best[K-1]; keys are
+// unique).  Top-down: slot i takes best[i-1] where key < best[i-1], takes key where
+// best[i-1] <= key < best[i], and keeps its value otherwise; each comparison serves two
+// neighbouring slots.  Every 8 slots the wave leaves the (fully unrolled, statically indexed)
+// network once no active lane still shifts.
 template <int K>
 __device__ __forceinline__ void topk_insert(uint64_t (&best)[K], uint64_t key) {
-  bool hi = true;  // key < best[i] (true for i = K-1 by the caller's test)
+  if constexpr (K == 1) {
+    best[0] = key;
+  } else {
+    bool hi = true;                     // key < best[i]
+    bool lo_next = key < best[K - 2];   // compare issued one step ahead of its selects
 #pragma unroll
-  for (int i = K - 1; i >= 1; i--) {
-    const bool lo = key < best[i - 1];
-    best[i] = hi ? (lo ? best[i - 1] : key) : best[i];
-    hi = lo;
-    if ((i & 7) == 0 && !wave_any(lo)) return;
+    for (int i = K - 1; i >= 1; i--) {
+      const bool lo = lo_next;          // key < best[i-1]
+      if (i >= 2) lo_next = key < best[i - 2];
+      const uint64_t t = hi ? key : best[i];
+      best[i] = lo ? best[i - 1] : t;
+      hi = lo;
+      // materialise the slot before the next exit test: otherwise the selects sink below the
+      // exits and the compare masks of several blocks stay live in (spilled) SGPRs
+      asm volatile("" : "+v"(best[i]));
+      if ((i & 7) == 0 && !wave_any(lo)) return;
+    }
+    best[0] = hi ? key : best[0];
   }
-  best[0] = hi ? key : best[0];
 }
 
 constexpr uint64_t kEmptyKey = (0x7f800000ull << 32) | 0xffffffffull;  // (r2 = +inf, id = -1)
@@ -343,34 +354,89 @@ __device__ __forceinline__ int rdlane(int v, int i) { return __builtin_amdgcn_re
 __device__ __forceinline__ float rdlane(float v, int i) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
 }
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One source particle as staged in LDS: position and reported id in one 16-byte record.
+struct alignas(16) SrcRec {
+  float p[3];
+  uint32_t id;
+};
+
+// Stream n staged source particles into the per-lane top-k.  `wrap` (wave-uniform) applies the
+// periodic wrap; the caller clears it only when no difference of the chunk needs it.
+template <int K, bool PER, bool FLOOR, int D>
+__device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
+                                              int n, bool wrap, float boxsize, float half, uint64_t floor_key) {
+  for (int j = 0; j < n; j++) {
+    const SrcRec c = s_rec[j];
+    Vec<D, float> p;
+#pragma unroll
+    for (int d = 0; d < D; d++) p[d] = c.p[d];
+    float r2;
+    if (PER && wrap) r2 = dist2P<true, D, float>(q, p, boxsize, half);
+    else r2 = dist2P<false, D, float>(q, p, boxsize, half);
+    const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | c.id;
+    if (key < best[K - 1] && (!FLOOR || key > floor_key)) topk_insert<K>(best, key);
+  }
+}
 
 // One wave per query leaf.  Interaction-list metadata (source leaf, r2, particle range) is loaded
 // 64 entries at a time, one per lane, and broadcast with readlane.  Source particles move in
 // chunks of <= 64: the next chunk is loaded into registers (one particle per lane) while the
-// current chunk is consumed from LDS, where every lane reads the same entry (broadcast).
-template <int K, bool PER, int D, typename T>
+// current chunk is consumed from LDS, where every lane reads the same record (broadcast).
+//
+// Periodic boxes: a chunk is streamed without the wrap selects when, for every staged particle p
+// and every dim, fl(qlo - p) >= -half and fl(qhi - p) < half, with [qlo, qhi] the query chunk's
+// bounding box.  Rounding is monotone, so then every difference fl(q - p) lies in [-half, half)
+// and the canonical wrap leaves it unchanged: the distances are bit-identical.
+//
+// Optional start bound (init_r2, per query leaf, K <= 64 single pass): the slots start at
+// (r2_0, id = 2^32-1), so only candidates with r2 <= r2_0 enter.  A query that ends with fewer
+// than K real neighbours is flagged in `retry` (its row holds the sentinel entries); every other
+// query's row is exact, because all its K neighbours lie within r2_0 and the list holds every
+// leaf within r2_0 (the caller's contract).
+template <int K, bool PER, bool FLOOR, int D>
 __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
                                                  const float* __restrict__ il_r2, const int32_t* __restrict__ splT,
-                                                 const T* __restrict__ xT, const int32_t* __restrict__ splQ,
-                                                 const T* __restrict__ xQ, const int32_t* __restrict__ src_ids,
-                                                 T* __restrict__ rad_out, int32_t* __restrict__ id_out,
-                                                 uint64_t* __restrict__ resume, int koff, int ktot, T boxsize,
-                                                 int64_t il_size) {
-  __shared__ Vec<D, T> s_p[64];
-  __shared__ uint32_t s_id[64];
+                                                 const float* __restrict__ xT, const int32_t* __restrict__ splQ,
+                                                 const float* __restrict__ xQ, const int32_t* __restrict__ src_ids,
+                                                 const int32_t* __restrict__ dst_rows,
+                                                 const float* __restrict__ init_r2, int32_t* __restrict__ retry,
+                                                 float* __restrict__ rad_out, int32_t* __restrict__ id_out,
+                                                 uint64_t* __restrict__ resume, int koff, int ktot, float boxsize,
+                                                 int64_t il_size, int64_t out_rows) {
+  __shared__ SrcRec s_rec[64];
   const int lane = threadIdx.x;
   const int lq = blockIdx.x;
   const int q0 = splQ[lq], q1 = splQ[lq + 1];
   const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
-  const T half = (T)(float)(T(0.5) * boxsize);
-  const bool use_floor = resume != nullptr && koff > 0;
+  const float half = 0.5f * boxsize;
+  const uint64_t start_key =
+      init_r2 ? (((uint64_t)__float_as_uint(init_r2[lq]) << 32) | 0xffffffffull) : kEmptyKey;
   for (int qoff = q0; qoff < q1; qoff += 64) {
     const int iq = min(qoff + lane, q1 - 1);
-    const Vec<D, T> q = load_vec<D, T>(xQ, iq);
-    const uint64_t floor_key = use_floor ? resume[iq] : 0ull;
+    const Vec<D, float> q = load_vec<D, float>(xQ, iq);
+    const uint64_t floor_key = FLOOR ? resume[iq] : 0ull;
+    Vec<D, float> qlo, qhi;
+    if constexpr (PER) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        qlo[d] = wave_minf(q[d]);
+        qhi[d] = wave_maxf(q[d]);
+      }
+    }
     uint64_t best[K];
 #pragma unroll
-    for (int i = 0; i < K; i++) best[i] = kEmptyKey;
+    for (int i = 0; i < K; i++) best[i] = start_key;
     bool done = false;
     for (int eb = e0; eb < e1 && !done; eb += 64) {
       const int ne = min(64, e1 - eb);
@@ -383,15 +449,16 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
         m_t1 = splT[src + 1];
       }
       // first chunk of this batch
-      Vec<D, T> cp;
-      uint32_t cid = 0;
+      SrcRec cr;
       int cn;
       {
         const int t0 = rdlane(m_t0, 0), t1 = rdlane(m_t1, 0);
         cn = min(64, t1 - t0);
         if (lane < cn) {
-          cp = load_vec<D, T>(xT, t0 + lane);
-          cid = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+          const Vec<D, float> p = load_vec<D, float>(xT, t0 + lane);
+#pragma unroll
+          for (int d = 0; d < D; d++) cr.p[d] = p[d];
+          cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
         }
       }
       int i = 0, off = 0;
@@ -410,39 +477,46 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
           noff = 0;
         }
         const int n_cur = cn;
-        if (lane < n_cur) {
-          s_p[lane] = cp;
-          s_id[lane] = cid;
+        bool wrap = false;
+        if constexpr (PER) {
+          bool need = false;
+          if (lane < n_cur) {
+#pragma unroll
+            for (int d = 0; d < D; d++) need |= !((qlo[d] - cr.p[d]) >= -half) || !((qhi[d] - cr.p[d]) < half);
+          }
+          wrap = wave_any(need);
         }
+        if (lane < n_cur) s_rec[lane] = cr;
         __syncthreads();
         if (ni < ne) {
           const int t0 = rdlane(m_t0, ni) + noff, t1 = rdlane(m_t1, ni);
           cn = min(64, t1 - t0);
           if (lane < cn) {
-            cp = load_vec<D, T>(xT, t0 + lane);
-            cid = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+            const Vec<D, float> p = load_vec<D, float>(xT, t0 + lane);
+#pragma unroll
+            for (int d = 0; d < D; d++) cr.p[d] = p[d];
+            cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
           }
         }
-        for (int j = 0; j < n_cur; j++) {
-          const Vec<D, T> p = s_p[j];
-          const float r2 = (float)dist2P<PER, D, T>(q, p, boxsize, half);
-          const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | s_id[j];
-          if (key < best[K - 1] && (!use_floor || key > floor_key)) topk_insert<K>(best, key);
-        }
+        consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_cur, wrap, boxsize, half, floor_key);
         __syncthreads();
         i = ni;
         off = noff;
       }
     }
     if (qoff + lane < q1) {
-      const int64_t row = (int64_t)iq * ktot + koff;
+      const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
+      if (orow >= 0 && orow < out_rows) {
+        const int64_t row = orow * ktot + koff;
 #pragma unroll
-      for (int i = 0; i < K; i++) {
-        if (i < ktot - koff) {
-          rad_out[row + i] = (T)sqrtf(key_r2(best[i]));
-          id_out[row + i] = (int32_t)(uint32_t)best[i];
+        for (int i = 0; i < K; i++) {
+          if (i < ktot - koff) {
+            rad_out[row + i] = sqrtf(key_r2(best[i]));
+            id_out[row + i] = (int32_t)(uint32_t)best[i];
+          }
         }
       }
+      if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
       if (resume) {
         const int last = min(K, ktot - koff) - 1;
         uint64_t lk = best[0];
@@ -481,28 +555,38 @@ int node2node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src
   return check_launch("KnnNode2Node sort");
 }
 
-template <int D, typename T>
+template <int D>
 int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, const float* il_r2, const int32_t* splT,
-                   const void* xT, const int32_t* splQ, const void* xQ, const int32_t* src_ids, void* rad, int32_t* ids,
-                   void* tmp, int64_t nleaves_q, int64_t npart_q, int64_t il_size, int k, float boxsize) {
+                   const void* xT, const int32_t* splQ, const void* xQ, const int32_t* src_ids,
+                   const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* rad, int32_t* ids, void* tmp,
+                   int64_t nleaves_q, int64_t npart_q, int64_t out_rows, int64_t il_size, int k, float boxsize) {
   if (nleaves_q <= 0) return 0;
+  if (init_r2 && k > 64) return fail("KnnLeaf2Leaf: a start bound needs k <= 64");
   uint64_t* resume = k > 64 ? (uint64_t*)tmp : nullptr;
   dim3 g((unsigned)nleaves_q), b(64);
+  const float* xt = (const float*)xT;
+  const float* xq = (const float*)xQ;
   for (int koff = 0; koff < k; koff += 64) {
-    int kn = min(k - koff, 64);
-#define JZT_KNN_LAUNCH(KK)                                                                                      \
-  do {                                                                                                          \
-    if (boxsize > 0.f)                                                                                          \
-      hipLaunchKernelGGL((k_knn_leaf<KK, true, D, T>), g, b, 0, s, il_spl, il_src, il_r2, splT, (const T*)xT,   \
-                         splQ, (const T*)xQ, src_ids, (T*)rad, ids, resume, koff, k, (T)boxsize, il_size);               \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_knn_leaf<KK, false, D, T>), g, b, 0, s, il_spl, il_src, il_r2, splT, (const T*)xT,  \
-                         splQ, (const T*)xQ, src_ids, (T*)rad, ids, resume, koff, k, (T)boxsize, il_size);               \
+    const int kn = min(k - koff, 64);
+    const bool floor = resume && koff > 0;
+#define JZT_KNN_LAUNCH(KK, PER, FLOOR)                                                                              \
+  hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
+                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows)
+#define JZT_KNN_LAUNCH_K(KK)                                      \
+  do {                                                            \
+    if (boxsize > 0.f) {                                          \
+      if (floor) JZT_KNN_LAUNCH(KK, true, true);                  \
+      else JZT_KNN_LAUNCH(KK, true, false);                       \
+    } else {                                                      \
+      if (floor) JZT_KNN_LAUNCH(KK, false, true);                 \
+      else JZT_KNN_LAUNCH(KK, false, false);                      \
+    }                                                             \
   } while (0)
-    if (kn <= 8) JZT_KNN_LAUNCH(8);
-    else if (kn <= 16) JZT_KNN_LAUNCH(16);
-    else if (kn <= 32) JZT_KNN_LAUNCH(32);
-    else JZT_KNN_LAUNCH(64);
+    if (kn <= 8) JZT_KNN_LAUNCH_K(8);
+    else if (kn <= 16) JZT_KNN_LAUNCH_K(16);
+    else if (kn <= 32) JZT_KNN_LAUNCH_K(32);
+    else JZT_KNN_LAUNCH_K(64);
+#undef JZT_KNN_LAUNCH_K
 #undef JZT_KNN_LAUNCH
     if (check_launch("KnnLeaf2Leaf")) return 1;
   }
@@ -543,16 +627,31 @@ extern "C" int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, 
                                  const int32_t* splQ, const void* xQ, const int32_t* src_ids, void* knn_rad,
                                  int32_t* knn_id, void* tmp, size_t tmp_bytes, int64_t size_leaves_query,
                                  int64_t size_part_query, int k, float boxsize, int dim, int dtype) {
+  return jzt_knn_leaf2leaf_ex(stream, ilist_spl, ilist_isrc, ilist_r2, ilist_size, splT, xT, splQ, xQ, src_ids,
+                              nullptr, nullptr, nullptr, knn_rad, knn_id, tmp, tmp_bytes, size_leaves_query,
+                              size_part_query, size_part_query, k, boxsize, dim, dtype);
+}
+
+extern "C" int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
+                                    const float* ilist_r2, int64_t ilist_size, const int32_t* splT, const void* xT,
+                                    const int32_t* splQ, const void* xQ, const int32_t* src_ids,
+                                    const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* knn_rad,
+                                    int32_t* knn_id, void* tmp, size_t tmp_bytes, int64_t size_leaves_query,
+                                    int64_t size_part_query, int64_t out_rows, int k, float boxsize, int dim,
+                                    int dtype) {
   hipStream_t s = (hipStream_t)stream;
   if (k < 1) return fail("KnnLeaf2Leaf: k must be >= 1");
   if (tmp_bytes < jzt_knn_leaf2leaf_tmp_bytes(size_part_query, k, dtype)) return fail("KnnLeaf2Leaf: tmp too small");
-  if (dtype == JZT_F32 && dim == 3)
-    return leaf2leaf_impl<3, float>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, knn_rad, knn_id,
-                                    tmp, size_leaves_query, size_part_query, ilist_size, k, boxsize);
-  if (dtype == JZT_F32 && dim == 2)
-    return leaf2leaf_impl<2, float>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, knn_rad, knn_id,
-                                    tmp, size_leaves_query, size_part_query, ilist_size, k, boxsize);
-  return fail("KnnLeaf2Leaf: unsupported dim/dtype");
+  if (dtype != JZT_F32) return fail("KnnLeaf2Leaf: positions must be float32");
+  if (dim == 3)
+    return leaf2leaf_impl<3>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry,
+                             knn_rad, knn_id, tmp, size_leaves_query, size_part_query, out_rows, ilist_size, k,
+                             boxsize);
+  if (dim == 2)
+    return leaf2leaf_impl<2>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry,
+                             knn_rad, knn_id, tmp, size_leaves_query, size_part_query, out_rows, ilist_size, k,
+                             boxsize);
+  return fail("KnnLeaf2Leaf: unsupported dim");
 }
 
 extern "C" int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const float* key, const int32_t* val,

@@ -38,10 +38,15 @@ class PosLvlNum:
         return torch.cat([self.pos, lvl_t[:, None]], dim=-1).contiguous()
 
 
-def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, boxsize=0., src_ids=None):
-    """knn.py:28-60 -> KnnLeaf2Leaf.  Returns (rnn (nq,k), inn (nq,k) int32).
+def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, boxsize=0., src_ids=None,
+                   dst_rows=None, out_rows: int | None = None, init_r2=None):
+    """knn.py:28-60 -> KnnLeaf2Leaf.  Returns (rnn (rows,k), inn (rows,k) int32), plus the retry
+    flags (nq int32) when init_r2 is given.
 
-    src_ids (extension): id reported for each source particle; default its z-index."""
+    Extensions: src_ids = id reported for each source particle (default its z-index); dst_rows =
+    output row of each query (default its z-index; rows = out_rows, default nq); init_r2 = starting
+    bound per query leaf (k <= 64; the lists must hold every leaf within it), queries left with
+    fewer than k neighbours inside it are flagged and their rows must be recomputed."""
     if xQ is None:
         xQ = xT
     if splQ is None:
@@ -52,16 +57,22 @@ def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, b
     assert len(ilist.ispl) == len(splQ)
     dev = xQ.device
     nq = xQ.shape[0]
+    rows = nq if out_rows is None else int(out_rows)
     dt = L.dtype_code(xT.dtype)
-    rnn = torch.empty((nq, k), dtype=xT.dtype, device=dev)
-    inn = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    rnn = torch.empty((rows, k), dtype=xT.dtype, device=dev)
+    inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
     nb = L.LIB.jzt_knn_leaf2leaf_tmp_bytes(nq, k, dt)
     tmp = _tmp(nb, dev)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
-    L.call("jzt_knn_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2), ilist.size(),
-           L.ptr(splT),
-           L.ptr(xT.contiguous()), L.ptr(splQ), L.ptr(xQ.contiguous()), L.ptr(sid), L.ptr(rnn), L.ptr(inn),
-           L.ptr(tmp), nb, len(splQ) - 1, nq, int(k), float(boxsize or 0.), xT.shape[-1], dt)
+    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+    ir2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
+    retry = None if init_r2 is None else torch.empty(nq, dtype=torch.int32, device=dev)
+    L.call("jzt_knn_leaf2leaf_ex", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2),
+           ilist.size(), L.ptr(splT), L.ptr(xT.contiguous()), L.ptr(splQ), L.ptr(xQ.contiguous()), L.ptr(sid),
+           L.ptr(drow), L.ptr(ir2), L.ptr(retry), L.ptr(rnn), L.ptr(inn), L.ptr(tmp), nb, len(splQ) - 1, nq, rows,
+           int(k), float(boxsize or 0.), xT.shape[-1], dt)
+    if init_r2 is not None:
+        return rnn, inn, retry
     return rnn, inn
 
 
@@ -204,8 +215,7 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
         else:
             ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
             rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,
-                                         src_ids=originz.idx)
-            direct = False
+                                         src_ids=originz.idx, dst_rows=originz.idx if direct else None)
 
         res = []
         zidx_cache = {}
