@@ -122,6 +122,19 @@ int jzt_knn_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, cons
                       int32_t* node_ilist_isrc, float* node_ilist_r2, int64_t size_parents, int64_t size_nodes,
                       int64_t parent_ilist_size, int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
 
+/* KnnNode2Node with a caller-given search radius (extension, no reference counterpart;
+ * jzt_knn_node2node is this call with node_r2_in = NULL).  When node_r2_in (size_nodes float) is
+ * non-NULL, the guaranteed-radius search is skipped: node i's list holds every node whose
+ * distance bound is <= node_r2_in[i] * (1 + 2^-16), and node_rmax2 receives that scaled radius.
+ * Used with a proposed radius r2_0 per leaf, so that jzt_knn_leaf2leaf_ex(init_r2 = r2_0) can
+ * rely on the lists holding every leaf within r2_0. */
+int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                         const float* parent_ilist_r2, const int32_t* parent_spl, const void* nodes,
+                         const int32_t* nodes_npart, const float* node_r2_in, float* node_rmax2,
+                         int32_t* node_ilist_spl, int32_t* node_ilist_isrc, float* node_ilist_r2,
+                         int64_t size_parents, int64_t size_nodes, int64_t parent_ilist_size,
+                         int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
+
 /* KnnLeaf2Leaf -- replaces ffi target "KnnLeaf2Leaf" (generated/ffi_knn.cu:31-188, host
  * knn.cuh:309-356, caller knn.py:28-60).  knn_rad (nq, k) of dtype, knn_id (nq, k) int32.
  * Neighbours ascending by (canonical r2, id); id = z-index in xT, or src_ids[z-index] when src_ids
@@ -164,6 +177,58 @@ int jzt_knn_fused(jzt_stream_t stream, const int32_t* parent_ilist_spl, const in
                   const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2, int32_t* retry,
                   void* knn_rad, int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize,
                   int dim, int dtype);
+
+/* KnnFused retry pass (extension): recomputes, without a bound, exactly the queries flagged in
+ * `retry` (size_part int32, input) and writes their rows; other rows are left untouched.  Same
+ * arguments as jzt_knn_fused.  This is the exact fallback after any bounded pass (the fused one or
+ * jzt_knn_leaf2leaf_ex with init_r2). */
+int jzt_knn_fused_retry(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                        const float* parent_ilist_r2, int64_t parent_ilist_size, const int32_t* parent_child_spl,
+                        const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* leaf_spl, const void* xz,
+                        const int32_t* src_ids, const int32_t* dst_rows, const int32_t* retry, void* knn_rad,
+                        int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize, int dim,
+                        int dtype);
+
+/* ------------------------------------------------------------------------------------------ */
+/* learned radius proposer (no reference counterpart)                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* A 12-64-64-64-1 GELU MLP with bf16 weights (fp32 arithmetic) predicts, per leaf, log2 of the
+ * largest k-th neighbour distance among the leaf's particles.  exp2(prediction + margin)^2 is the
+ * starting bound r2_0 of jzt_knn_node2node_ex / jzt_knn_leaf2leaf_ex, whose retry flags feed
+ * jzt_knn_fused_retry: kNN results never depend on the network.  Flat parameter layout
+ * (input-major weights): W1[12][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4. */
+#define JZT_PROPOSER_NFEAT 12
+#define JZT_PROPOSER_NPARAM 9217
+int jzt_proposer_num_params(void);
+
+/* 12 features per leaf slot (feat: size x 12 float; slots >= *nleaves get zeros).  Plane-0 arrays
+ * npart0/lvl0 (size), leaf_parent (size, plane-1 node of each leaf), plane-1 arrays npart1/lvl1,
+ * parent_child_spl (plane-1 -> leaf splits), and optionally parent_parent (plane-1 -> plane-2 node)
+ * with npart2/lvl2 (NULL when the tree has no plane 2).  nleaves: device scalar. */
+int jzt_leaf_features(jzt_stream_t stream, const int32_t* npart0, const int32_t* lvl0, const int32_t* leaf_parent,
+                      const int32_t* npart1, const int32_t* lvl1, const int32_t* parent_child_spl,
+                      const int32_t* parent_parent, const int32_t* npart2, const int32_t* lvl2,
+                      const int32_t* nleaves, int64_t size, float* feat);
+
+/* Inference.  norm (26 float): feature mean[12], feature std[12], target mean, target std (the
+ * network works on standardised features and targets).  Writes, per sample i < n,
+ * out_log2r[i] = prediction (optional) and out_r2[i] = exp2(prediction + margin)^2 (optional; 0
+ * where npart != NULL and npart[i] <= 0). */
+int jzt_mlp_infer(jzt_stream_t stream, const uint16_t* weights, const float* norm, const float* feat,
+                  const int32_t* npart, int64_t n, float margin, float* out_r2, float* out_log2r);
+
+/* Training forward + backward: grad (9217 float) = d/dW of loss = (1/batch) sum (y - t)^2 over
+ * standardised features x (batch x 12) and targets t, taken at the bf16 weights; loss (1 float).
+ * Both are overwritten. */
+int jzt_mlp_grad(jzt_stream_t stream, const uint16_t* weights, const float* x, const float* t, int64_t batch,
+                 float* grad, float* loss);
+
+/* One Adam step on the fp32 master weights w (bias-corrected, step counts from 1), then
+ * wq = bf16(w) rounded to nearest-even.  jzt_to_bf16: wq = bf16(w) only. */
+int jzt_adam_step(jzt_stream_t stream, float* w, uint16_t* wq, float* m, float* v, const float* grad, int64_t n,
+                  float lr, float beta1, float beta2, float eps, int step);
+int jzt_to_bf16(jzt_stream_t stream, const float* w, uint16_t* wq, int64_t n);
 
 /* ------------------------------------------------------------------------------------------ */
 /* friends-of-friends                                                                           */

@@ -37,6 +37,8 @@ _SIGNATURES = {
     "jzt_regularization_level": ([_p, _p, _p, _i64, _p, _i32, _i32, _f32, _f32, _p, _sz], _i32),
     "jzt_knn_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i32, _f32, _i32,
                            _i32], _i32),
+    "jzt_knn_node2node_ex": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i32,
+                              _f32, _i32, _i32], _i32),
     "jzt_knn_leaf2leaf_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_knn_leaf2leaf": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _i64, _i64, _i32, _f32,
                            _i32, _i32], _i32),
@@ -44,6 +46,14 @@ _SIGNATURES = {
                               _i64, _i32, _f32, _i32, _i32], _i32),
     "jzt_knn_fused": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32,
                        _i32, _i32], _i32),
+    "jzt_knn_fused_retry": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32,
+                             _f32, _i32, _i32], _i32),
+    "jzt_proposer_num_params": ([], _i32),
+    "jzt_leaf_features": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p], _i32),
+    "jzt_mlp_infer": ([_p, _p, _p, _p, _p, _i64, _f32, _p, _p], _i32),
+    "jzt_mlp_grad": ([_p, _p, _p, _p, _i64, _p, _p], _i32),
+    "jzt_adam_step": ([_p, _p, _p, _p, _p, _p, _i64, _f32, _f32, _f32, _f32, _i32], _i32),
+    "jzt_to_bf16": ([_p, _p, _p, _i64], _i32),
     "jzt_node_to_child_label": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
     "jzt_fof_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
     "jzt_fof_leaf2leaf": ([_p, _p, _p, _i64, _p, _p, _p, _p, _i64, _i64, _f32, _f32, _i32, _i32], _i32),

@@ -18,6 +18,7 @@
 // Order: ascending by (canonical FP32 r2, reported id).  The reference inserts with strict '<' in
 // visit order (knn.cuh:84-102), so its tie order depends on the traversal; here it does not.
 #include "util.h"
+#include "knn_common.h"
 #include "../../include/jztree_amd.h"
 
 namespace jzt {
@@ -318,77 +319,6 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // ------------------------------------------------------------------------------------------------
 // leaf-vs-leaf top-k
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ float key_r2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
-
-// Insert `key` into the ascending list best[0..K) (caller guarantees key < best[K-1]; keys are
-// unique).  Top-down: slot i takes best[i-1] where key < best[i-1], takes key where
-// best[i-1] <= key < best[i], and keeps its value otherwise; each comparison serves two
-// neighbouring slots.  Every 8 slots the wave leaves the (fully unrolled, statically indexed)
-// network once no active lane still shifts.
-template <int K>
-__device__ __forceinline__ void topk_insert(uint64_t (&best)[K], uint64_t key) {
-  if constexpr (K == 1) {
-    best[0] = key;
-  } else {
-    bool hi = true;                     // key < best[i]
-    bool lo_next = key < best[K - 2];   // compare issued one step ahead of its selects
-#pragma unroll
-    for (int i = K - 1; i >= 1; i--) {
-      const bool lo = lo_next;          // key < best[i-1]
-      if (i >= 2) lo_next = key < best[i - 2];
-      const uint64_t t = hi ? key : best[i];
-      best[i] = lo ? best[i - 1] : t;
-      hi = lo;
-      // materialise the slot before the next exit test: otherwise the selects sink below the
-      // exits and the compare masks of several blocks stay live in (spilled) SGPRs
-      asm volatile("" : "+v"(best[i]));
-      if ((i & 7) == 0 && !wave_any(lo)) return;
-    }
-    best[0] = hi ? key : best[0];
-  }
-}
-
-constexpr uint64_t kEmptyKey = (0x7f800000ull << 32) | 0xffffffffull;  // (r2 = +inf, id = -1)
-
-__device__ __forceinline__ int rdlane(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
-__device__ __forceinline__ float rdlane(float v, int i) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
-}
-__device__ __forceinline__ float wave_minf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// One source particle as staged in LDS: position and reported id in one 16-byte record.
-struct alignas(16) SrcRec {
-  float p[3];
-  uint32_t id;
-};
-
-// Stream n staged source particles into the per-lane top-k.  `wrap` (wave-uniform) applies the
-// periodic wrap; the caller clears it only when no difference of the chunk needs it.
-template <int K, bool PER, bool FLOOR, int D>
-__device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
-                                              int n, bool wrap, float boxsize, float half, uint64_t floor_key) {
-  for (int j = 0; j < n; j++) {
-    const SrcRec c = s_rec[j];
-    Vec<D, float> p;
-#pragma unroll
-    for (int d = 0; d < D; d++) p[d] = c.p[d];
-    float r2;
-    if (PER && wrap) r2 = dist2P<true, D, float>(q, p, boxsize, half);
-    else r2 = dist2P<false, D, float>(q, p, boxsize, half);
-    const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | c.id;
-    if (key < best[K - 1] && (!FLOOR || key > floor_key)) topk_insert<K>(best, key);
-  }
-}
-
 // One wave per query leaf.  Interaction-list metadata (source leaf, r2, particle range) is loaded
 // 64 entries at a time, one per lane, and broadcast with readlane.  Source particles move in
 // chunks of <= 64: the next chunk is loaded into registers (one particle per lane) while the
@@ -528,16 +458,25 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
   }
 }
 
+__global__ void k_scale_rmax2(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] * kRmaxSlack;
+}
+
 template <int D, typename T>
 int node2node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, const float* pil_r2,
-                   const int32_t* parent_spl, const void* nodes, const int32_t* npart, float* rmax2, int32_t* il_spl,
-                   int32_t* il_src, float* il_r2, int64_t size_parents, int64_t size_nodes, int64_t pil_size,
-                   int64_t il_size, int k, float boxsize) {
+                   const int32_t* parent_spl, const void* nodes, const int32_t* npart, const float* rmax2_in,
+                   float* rmax2, int32_t* il_spl, int32_t* il_src, float* il_r2, int64_t size_parents,
+                   int64_t size_nodes, int64_t pil_size, int64_t il_size, int k, float boxsize) {
   const T* nd = (const T*)nodes;
   if (fill_u32(s, il_spl, 0u, size_nodes + 1)) return 1;
   if (size_parents <= 0) return 0;
   dim3 g((unsigned)size_parents), b(64);
-  if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
+  if (rmax2_in) {
+    if (size_nodes > 0)
+      hipLaunchKernelGGL(k_scale_rmax2, dim3((unsigned)div_up(size_nodes, 256)), dim3(256), 0, s, rmax2_in, rmax2,
+                         size_nodes);
+  } else if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
   else if (k <= 32) hipLaunchKernelGGL((k_find_rmax<16, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
   else hipLaunchKernelGGL((k_find_rmax<32, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
   if (check_launch("KnnNode2Node FindRmax")) return 1;
@@ -604,16 +543,30 @@ extern "C" int jzt_knn_node2node(jzt_stream_t stream, const int32_t* parent_ilis
                                  int32_t* node_ilist_isrc, float* node_ilist_r2, int64_t size_parents,
                                  int64_t size_nodes, int64_t parent_ilist_size, int64_t node_ilist_size, int k,
                                  float boxsize, int dim, int dtype) {
+  return jzt_knn_node2node_ex(stream, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,
+                              nodes_npart, nullptr, node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2,
+                              size_parents, size_nodes, parent_ilist_size, node_ilist_size, k, boxsize, dim, dtype);
+}
+
+extern "C" int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_ilist_spl,
+                                    const int32_t* parent_ilist_isrc, const float* parent_ilist_r2,
+                                    const int32_t* parent_spl, const void* nodes, const int32_t* nodes_npart,
+                                    const float* node_r2_in, float* node_rmax2, int32_t* node_ilist_spl,
+                                    int32_t* node_ilist_isrc, float* node_ilist_r2, int64_t size_parents,
+                                    int64_t size_nodes, int64_t parent_ilist_size, int64_t node_ilist_size, int k,
+                                    float boxsize, int dim, int dtype) {
   hipStream_t s = (hipStream_t)stream;
   if (k < 1) return fail("KnnNode2Node: k must be >= 1");
   if (dtype == JZT_F32 && dim == 3)
-    return node2node_impl<3, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes, nodes_npart,
-                                    node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2, size_parents, size_nodes,
-                                    parent_ilist_size, node_ilist_size, k, boxsize);
+    return node2node_impl<3, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,
+                                    nodes_npart, node_r2_in, node_rmax2, node_ilist_spl, node_ilist_isrc,
+                                    node_ilist_r2, size_parents, size_nodes, parent_ilist_size, node_ilist_size, k,
+                                    boxsize);
   if (dtype == JZT_F32 && dim == 2)
-    return node2node_impl<2, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes, nodes_npart,
-                                    node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2, size_parents, size_nodes,
-                                    parent_ilist_size, node_ilist_size, k, boxsize);
+    return node2node_impl<2, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,
+                                    nodes_npart, node_r2_in, node_rmax2, node_ilist_spl, node_ilist_isrc,
+                                    node_ilist_r2, size_parents, size_nodes, parent_ilist_size, node_ilist_size, k,
+                                    boxsize);
   return fail("KnnNode2Node: unsupported dim/dtype");
 }
 

@@ -1,0 +1,83 @@
+// Per-lane exact top-k shared by the kNN leaf kernels (knn.hip, knn_fused.hip).
+//
+// Each lane owns one query and keeps its k best candidates as u64 keys (r2 bits << 32 | id),
+// ascending, in registers.  Keys order candidates by (canonical FP32 r2, reported id): the
+// reference's SortedNearestK (knn.cuh:38-106) keeps (r2, index) pairs sorted by r2 with its
+// insertion order deciding ties; here ties are broken by id so the result is unique.
+#pragma once
+#include "util.h"
+
+namespace jzt {
+
+__device__ __forceinline__ float key_r2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+// Insert `key` into the ascending list best[0..K) (caller guarantees key < best[K-1]; keys are
+// unique).  Top-down: slot i takes best[i-1] where key < best[i-1], takes key where
+// best[i-1] <= key < best[i], and keeps its value otherwise; each comparison serves two
+// neighbouring slots.  Every 8 slots the wave leaves the (fully unrolled, statically indexed)
+// network once no active lane still shifts.
+template <int K>
+__device__ __forceinline__ void topk_insert(uint64_t (&best)[K], uint64_t key) {
+  if constexpr (K == 1) {
+    best[0] = key;
+  } else {
+    bool hi = true;                     // key < best[i]
+    bool lo_next = key < best[K - 2];   // compare issued one step ahead of its selects
+#pragma unroll
+    for (int i = K - 1; i >= 1; i--) {
+      const bool lo = lo_next;          // key < best[i-1]
+      if (i >= 2) lo_next = key < best[i - 2];
+      const uint64_t t = hi ? key : best[i];
+      best[i] = lo ? best[i - 1] : t;
+      hi = lo;
+      // materialise the slot before the next exit test: otherwise the selects sink below the
+      // exits and the compare masks of several blocks stay live in (spilled) SGPRs
+      asm volatile("" : "+v"(best[i]));
+      if ((i & 7) == 0 && !wave_any(lo)) return;
+    }
+    best[0] = hi ? key : best[0];
+  }
+}
+
+constexpr uint64_t kEmptyKey = (0x7f800000ull << 32) | 0xffffffffull;  // (r2 = +inf, id = -1)
+
+__device__ __forceinline__ int rdlane(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
+__device__ __forceinline__ float rdlane(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One source particle as staged in LDS: position and reported id in one 16-byte record.
+struct alignas(16) SrcRec {
+  float p[3];
+  uint32_t id;
+};
+
+// Stream n staged source particles into the per-lane top-k.  `wrap` (wave-uniform) applies the
+// periodic wrap; the caller clears it only when no difference of the chunk needs it.
+template <int K, bool PER, bool FLOOR, int D>
+__device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
+                                              int n, bool wrap, float boxsize, float half, uint64_t floor_key) {
+  for (int j = 0; j < n; j++) {
+    const SrcRec c = s_rec[j];
+    Vec<D, float> p;
+#pragma unroll
+    for (int d = 0; d < D; d++) p[d] = c.p[d];
+    float r2;
+    if (PER && wrap) r2 = dist2P<true, D, float>(q, p, boxsize, half);
+    else r2 = dist2P<false, D, float>(q, p, boxsize, half);
+    const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | c.id;
+    if (key < best[K - 1] && (!FLOOR || key > floor_key)) topk_insert<K>(best, key);
+  }
+}
+
+}  // namespace jzt

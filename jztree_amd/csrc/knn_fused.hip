@@ -18,23 +18,13 @@
 // whose list position is not beyond the stop point, and in a child that is not skipped (the bounds
 // are conservative by the slack), so the top-k equals the brute-force (r2, id) top-k.
 #include "util.h"
+#include "knn_common.h"
 #include "../../include/jztree_amd.h"
 
 namespace jzt {
 namespace {
 
 constexpr float kSlack = 1.0f + kBoundSlack;
-
-__device__ __forceinline__ int rdl(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
-__device__ __forceinline__ float rdl(float v, int i) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ float key_r2f(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
 
 template <int D, typename T> struct Cell {
   Vec<D, T> c, h;
@@ -50,30 +40,6 @@ __device__ __forceinline__ Cell<D, T> load_cell(const T* __restrict__ nodes, int
   else level = (int)__double_as_longlong(lv);
   r.h = lvl_half_ext<D, T>(level);
   return r;
-}
-
-template <int K>
-__device__ __forceinline__ void topk_ins(uint64_t (&best)[K], uint64_t key) {
-  bool hi = true;
-#pragma unroll
-  for (int i = K - 1; i >= 1; i--) {
-    const bool lo = key < best[i - 1];
-    best[i] = hi ? (lo ? best[i - 1] : key) : best[i];
-    hi = lo;
-    if ((i & 7) == 0 && !wave_any(lo)) return;
-  }
-  best[0] = hi ? key : best[0];
-}
-
-template <int K, bool WRAP, int D, typename T>
-__device__ __forceinline__ void consume(uint64_t (&best)[K], const Vec<D, T>& q, const Vec<D, T>* s_p,
-                                        const uint32_t* s_id, int n, T boxsize, T half) {
-  for (int j = 0; j < n; j++) {
-    const Vec<D, T> p = s_p[j];
-    const float r2 = (float)dist2P<WRAP, D, T>(q, p, boxsize, half);
-    const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | s_id[j];
-    if (key < best[K - 1]) topk_ins<K>(best, key);
-  }
 }
 
 constexpr int kListCap = 512;  // LDS list entries per wave (4 KB)
@@ -113,27 +79,30 @@ __device__ __forceinline__ void wave_sort(uint64_t* a, int n) {
 }
 
 // Consume the sorted list s_list[0..n) (key = mindist2 bits << 32 | wrap << 31 | leaf) nearest
-// first; stop at the first leaf no lane can reach.  Particles are staged through LDS, the next
-// leaf's (first) chunk being loaded into registers while the current one is consumed.
-template <int K, bool PER, int D, typename T>
-__device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, T>& q, const uint64_t* s_list, int n,
-                                             Vec<D, T>* s_p, uint32_t* s_id, const int32_t* __restrict__ l2p,
-                                             const T* __restrict__ x, const int32_t* __restrict__ src_ids,
-                                             T boxsize, T half) {
+// first; stop at the first leaf no lane can reach.  Particles are staged through LDS as 16-byte
+// records, the next leaf's (first) chunk being loaded into registers while the current one is
+// consumed.
+template <int K, bool PER, int D>
+__device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, float>& q, const uint64_t* s_list,
+                                             int n, SrcRec* s_rec, const int32_t* __restrict__ l2p,
+                                             const float* __restrict__ x, const int32_t* __restrict__ src_ids,
+                                             float boxsize, float half) {
   const int lane = threadIdx.x;
   int e = 0, off = 0;
-  Vec<D, T> cp;
-  uint32_t cid = 0;
-  int cn = 0, cwrap = 0;
+  SrcRec cr;
+  int cn = 0;
+  bool cwrap = false;
   auto load = [&](int ee, int oo) {
     const uint64_t v = s_list[ee];
     const int leaf = (int)(uint32_t)(v & 0x7fffffffull);
     const int a = l2p[leaf] + oo, b = l2p[leaf + 1];
     cn = min(64, b - a);
-    cwrap = (int)((v >> 31) & 1ull);
+    cwrap = ((v >> 31) & 1ull) != 0;
     if (lane < cn) {
-      cp = load_vec<D, T>(x, a + lane);
-      cid = src_ids ? (uint32_t)src_ids[a + lane] : (uint32_t)(a + lane);
+      const Vec<D, float> p = load_vec<D, float>(x, a + lane);
+#pragma unroll
+      for (int d = 0; d < D; d++) cr.p[d] = p[d];
+      cr.id = src_ids ? (uint32_t)src_ids[a + lane] : (uint32_t)(a + lane);
     }
   };
   if (n <= 0) return;
@@ -141,9 +110,10 @@ __device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, T
   while (e < n) {
     if (off == 0) {
       const float md = __uint_as_float((uint32_t)(s_list[e] >> 32));
-      if (!wave_any(md <= key_r2f(best[K - 1]) * kSlack)) return;
+      if (!wave_any(md <= key_r2(best[K - 1]) * kSlack)) return;
     }
-    const int n_cur = cn, w_cur = cwrap;
+    const int n_cur = cn;
+    const bool w_cur = cwrap;
     int ne = e, noff = off + 64;
     {
       const uint64_t v = s_list[e];
@@ -153,14 +123,10 @@ __device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, T
         noff = 0;
       }
     }
-    if (lane < n_cur) {
-      s_p[lane] = cp;
-      s_id[lane] = cid;
-    }
+    if (lane < n_cur) s_rec[lane] = cr;
     __syncthreads();
     if (ne < n) load(ne, noff);
-    if (PER && w_cur) consume<K, true, D, T>(best, q, s_p, s_id, n_cur, boxsize, half);
-    else consume<K, false, D, T>(best, q, s_p, s_id, n_cur, boxsize, half);
+    consume_chunk<K, PER, false, D>(best, q, s_rec, n_cur, w_cur, boxsize, half, 0ull);
     __syncthreads();
     e = ne;
     off = noff;
@@ -168,15 +134,14 @@ __device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, T
 }
 
 template <int K, bool PER, int D, typename T>
-__global__ __launch_bounds__(64) void k_knn_fused(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 : 1))) void k_knn_fused(
     const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src, const float* __restrict__ pil_r2,
     int64_t pil_size, const int32_t* __restrict__ n2c, const int32_t* __restrict__ leaf_parent,
     const T* __restrict__ leaf_nodes, const int32_t* __restrict__ l2p, const T* __restrict__ x,
     const int32_t* __restrict__ src_ids, const int32_t* __restrict__ dst_rows, const float* __restrict__ init_r2,
     int32_t* __restrict__ retry, int only_retry, T* __restrict__ rad_out, int32_t* __restrict__ id_out, int ktot,
     T boxsize) {
-  __shared__ Vec<D, T> s_p[64];
-  __shared__ uint32_t s_id[64];
+  __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_list[kListCap];
   const int lane = threadIdx.x;
   const int lq = blockIdx.x;
@@ -185,7 +150,6 @@ __global__ __launch_bounds__(64) void k_knn_fused(
   const int par = leaf_parent[lq];
   const int e0 = pil_spl[par], e1 = (int)min((int64_t)pil_spl[par + 1], pil_size);
   const T half = (T)(float)(T(0.5) * boxsize);
-  const Cell<D, T> cq = load_cell<D, T>(leaf_nodes, lq);
   const float r2_init = (init_r2 && !only_retry) ? init_r2[lq] : INFINITY;
   const uint64_t empty = ((uint64_t)__float_as_uint(r2_init) << 32) | 0xffffffffull;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -198,66 +162,80 @@ __global__ __launch_bounds__(64) void k_knn_fused(
     uint64_t best[K];
 #pragma unroll
     for (int i = 0; i < K; i++) best[i] = empty;
-    int cnt = 0;  // entries collected in s_list (wave-uniform)
-    bool done = false;
-    for (int eb = e0; eb < e1 && !done; eb += 64) {
-      const int ne = min(64, e1 - eb);
-      int m_c0 = 0, m_c1 = 0;
-      float m_r2 = INFINITY;
-      if (lane < ne) {
-        const int src = pil_src[eb + lane];
-        m_r2 = pil_r2[eb + lane];
-        m_c0 = n2c[src];
-        m_c1 = n2c[src + 1];
-      }
-      for (int i = 0; i < ne; i++) {
-        // the collection bound is the current k-th radius of the wave (the starting bound at first)
-        const float kmax = wave_max(key_r2f(best[K - 1])) * kSlack;
-        if (!(rdl(m_r2, i) <= kmax)) {
-          done = true;
-          break;
+    // Collect child leaves into s_list until the parent list is exhausted or the list is full,
+    // then sort and consume them; a full list resumes collection at the child where it stopped.
+    // (One consume site keeps one copy of the insertion network in the kernel.)
+    int eb = e0, ii = 0, cbo = 0;  // resume point: list batch, entry within it, child offset
+    bool more = e0 < e1;
+    while (true) {
+      int cnt = 0;  // entries collected in s_list (wave-uniform)
+      bool full = false;
+      while (more && !full) {
+        const Cell<D, T> cq = load_cell<D, T>(leaf_nodes, lq);  // (re)loaded here: not live across consume
+        const int ne = min(64, e1 - eb);
+        int m_c0 = 0, m_c1 = 0;
+        float m_r2 = INFINITY;
+        if (lane < ne) {
+          const int src = pil_src[eb + lane];
+          m_r2 = pil_r2[eb + lane];
+          m_c0 = n2c[src];
+          m_c1 = n2c[src + 1];
         }
-        const int c0 = rdl(m_c0, i), c1 = rdl(m_c1, i);
-        for (int cb = c0; cb < c1; cb += 64) {
-          const int nc = min(64, c1 - cb);
-          bool keep = false;
-          uint64_t item = 0;
-          if (lane < nc) {
-            const int lt = cb + lane;
-            const Cell<D, T> ct = load_cell<D, T>(leaf_nodes, lt);
-            Vec<D, T> w;
-#pragma unroll
-            for (int d = 0; d < D; d++) w[d] = cq.h[d] + ct.h[d];
-            const float md = (float)mindist2<D, T>(ct.c, cq.c, w, boxsize);
-            keep = (md <= kmax) && (l2p[lt + 1] > l2p[lt]);
-            int wrap = 0;
-            if (PER) {
-#pragma unroll
-              for (int d = 0; d < D; d++) {
-                const T lo = (cq.c[d] - cq.h[d]) - (ct.c[d] + ct.h[d]);
-                const T hi = (cq.c[d] + cq.h[d]) - (ct.c[d] - ct.h[d]);
-                wrap |= !(lo >= -half && hi < half) ? 1 : 0;
-              }
+        for (; ii < ne; ii++) {
+          // the collection bound is the current k-th radius of the wave (the starting bound at first)
+          const float kmax = wave_maxf(key_r2(best[K - 1])) * kSlack;
+          if (!(rdlane(m_r2, ii) <= kmax)) {
+            more = false;
+            break;
+          }
+          const int c0 = rdlane(m_c0, ii), c1 = rdlane(m_c1, ii);
+          for (int cb = c0 + cbo; cb < c1; cb += 64) {
+            if (cnt + 64 > kListCap) {
+              full = true;
+              cbo = cb - c0;
+              break;
             }
-            item = ((uint64_t)__float_as_uint(md) << 32) | ((uint64_t)wrap << 31) | (uint32_t)lt;
+            const int nc = min(64, c1 - cb);
+            bool keep = false;
+            uint64_t item = 0;
+            if (lane < nc) {
+              const int lt = cb + lane;
+              const Cell<D, T> ct = load_cell<D, T>(leaf_nodes, lt);
+              Vec<D, T> w;
+#pragma unroll
+              for (int d = 0; d < D; d++) w[d] = cq.h[d] + ct.h[d];
+              const float md = (float)mindist2<D, T>(ct.c, cq.c, w, boxsize);
+              keep = (md <= kmax) && (l2p[lt + 1] > l2p[lt]);
+              int wrap = 0;
+              if (PER) {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                  const T lo = (cq.c[d] - cq.h[d]) - (ct.c[d] + ct.h[d]);
+                  const T hi = (cq.c[d] + cq.h[d]) - (ct.c[d] - ct.h[d]);
+                  wrap |= !(lo >= -half && hi < half) ? 1 : 0;
+                }
+              }
+              item = ((uint64_t)__float_as_uint(md) << 32) | ((uint64_t)wrap << 31) | (uint32_t)lt;
+            }
+            const uint64_t mask = __ballot(keep);
+            if (keep) s_list[cnt + __popcll(mask & lt_mask)] = item;
+            cnt += __popcll(mask);
           }
-          const uint64_t mask = __ballot(keep);
-          if (cnt + 64 > kListCap) {
-            // list full: consume what we have (sorted) before collecting more
-            wave_sort(s_list, cnt);
-            consume_list<K, PER, D, T>(best, q, s_list, cnt, s_p, s_id, l2p, x, src_ids, boxsize, half);
-            cnt = 0;
-            __syncthreads();
-          }
-          if (keep) s_list[cnt + __popcll(mask & lt_mask)] = item;
-          cnt += __popcll(mask);
+          if (full) break;
+          cbo = 0;
+        }
+        if (!full && more) {
+          eb += 64;
+          ii = 0;
+          more = eb < e1;
         }
       }
+      __syncthreads();
+      wave_sort(s_list, cnt);
+      consume_list<K, PER, D>(best, q, s_list, cnt, s_rec, l2p, x, src_ids, boxsize, half);
+      __syncthreads();
+      if (!full) break;
     }
-    __syncthreads();
-    wave_sort(s_list, cnt);
-    consume_list<K, PER, D, T>(best, q, s_list, cnt, s_p, s_id, l2p, x, src_ids, boxsize, half);
-    __syncthreads();
 
     if (valid && (!only_retry || retry[iq] != 0)) {
       uint64_t kth_key = best[0];
@@ -270,7 +248,7 @@ __global__ __launch_bounds__(64) void k_knn_fused(
 #pragma unroll
         for (int i = 0; i < K; i++) {
           if (i < ktot) {
-            rad_out[row + i] = (T)sqrtf(key_r2f(best[i]));
+            rad_out[row + i] = (T)sqrtf(key_r2(best[i]));
             id_out[row + i] = (int32_t)(uint32_t)best[i];
           }
         }
@@ -279,20 +257,18 @@ __global__ __launch_bounds__(64) void k_knn_fused(
   }
 }
 
-__global__ void k_any_flag(const int32_t* __restrict__ f, int64_t n, int32_t* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && f[i]) out[0] = 1;
-}
-
+// passes: bit 0 = bounded pass (flags short queries when init_r2 is given), bit 1 = retry pass
+// (recomputes the queries flagged in `retry` without a bound)
 template <int D, typename T>
 int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, const float* pil_r2, int64_t pil_size,
                const int32_t* n2c, const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* l2p,
                const void* x, const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2, int32_t* retry,
-               void* rad, int32_t* ids, int64_t nleaves, int64_t npart, int k, float boxsize) {
+               void* rad, int32_t* ids, int64_t nleaves, int64_t npart, int k, float boxsize, int passes) {
   if (nleaves <= 0) return 0;
   if (init_r2 && !retry) return fail("KnnFused: a bound needs the retry buffer");
   dim3 g((unsigned)nleaves), b(64);
-  for (int pass = 0; pass < (init_r2 ? 2 : 1); pass++) {
+  for (int pass = 0; pass < 2; pass++) {
+    if (!(passes & (1 << pass))) continue;
     const int only = pass;
 #define JZT_FUSED(KK, PP)                                                                                        \
   hipLaunchKernelGGL((k_knn_fused<KK, PP, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, pil_size, n2c, leaf_parent, \
@@ -314,6 +290,13 @@ int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, co
   return 0;
 }
 
+template <typename... A>
+int fused_dispatch(int dim, int dtype, A... a) {
+  if (dtype == JZT_F32 && dim == 3) return fused_impl<3, float>(a...);
+  if (dtype == JZT_F32 && dim == 2) return fused_impl<2, float>(a...);
+  return fail("KnnFused: unsupported dim/dtype");
+}
+
 }  // namespace
 }  // namespace jzt
 
@@ -325,15 +308,23 @@ extern "C" int jzt_knn_fused(jzt_stream_t stream, const int32_t* parent_ilist_sp
                              const void* xz, const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2,
                              int32_t* retry, void* knn_rad, int32_t* knn_id, int64_t size_leaves, int64_t size_part,
                              int k, float boxsize, int dim, int dtype) {
-  hipStream_t s = (hipStream_t)stream;
   if (k < 1 || k > 64) return fail("KnnFused: k must be in [1, 64]");
-  if (dtype == JZT_F32 && dim == 3)
-    return fused_impl<3, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_ilist_size,
-                                parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows, init_r2,
-                                retry, knn_rad, knn_id, size_leaves, size_part, k, boxsize);
-  if (dtype == JZT_F32 && dim == 2)
-    return fused_impl<2, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_ilist_size,
-                                parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows, init_r2,
-                                retry, knn_rad, knn_id, size_leaves, size_part, k, boxsize);
-  return fail("KnnFused: unsupported dim/dtype");
+  return fused_dispatch(dim, dtype, (hipStream_t)stream, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2,
+                        parent_ilist_size, parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows,
+                        init_r2, retry, knn_rad, knn_id, size_leaves, size_part, k, boxsize, init_r2 ? 3 : 1);
+}
+
+extern "C" int jzt_knn_fused_retry(jzt_stream_t stream, const int32_t* parent_ilist_spl,
+                                   const int32_t* parent_ilist_isrc, const float* parent_ilist_r2,
+                                   int64_t parent_ilist_size, const int32_t* parent_child_spl,
+                                   const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* leaf_spl,
+                                   const void* xz, const int32_t* src_ids, const int32_t* dst_rows,
+                                   const int32_t* retry, void* knn_rad, int32_t* knn_id, int64_t size_leaves,
+                                   int64_t size_part, int k, float boxsize, int dim, int dtype) {
+  if (k < 1 || k > 64) return fail("KnnFused: k must be in [1, 64]");
+  if (!retry) return fail("KnnFused: retry flags required");
+  return fused_dispatch(dim, dtype, (hipStream_t)stream, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2,
+                        parent_ilist_size, parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows,
+                        (const float*)nullptr, const_cast<int32_t*>(retry), knn_rad, knn_id, size_leaves, size_part,
+                        k, boxsize, 2);
 }

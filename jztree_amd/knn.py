@@ -80,8 +80,11 @@ _knn_leaf2leaf.jit = _knn_leaf2leaf
 
 
 def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNum, k: int = 32,
-                         boxsize: float = 0., with_radii: bool = False):
-    """knn.py:62-110 -> KnnNode2Node."""
+                         boxsize: float = 0., with_radii: bool = False, rmax2_in=None):
+    """knn.py:62-110 -> KnnNode2Node.
+
+    rmax2_in (extension): search radius^2 per node instead of the guaranteed k-neighbour radius;
+    each list then holds every node within rmax2_in * (1 + 2^-16)."""
     boxsize = 0. if boxsize is None else boxsize
     assert ilist.ispl.shape[0] == spl_parent.shape[0]
     assert ilist.isrc.shape == ilist.rad2.shape
@@ -93,10 +96,12 @@ def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNu
     il = torch.empty(ilist.size(), dtype=torch.int32, device=dev)
     ilr2 = torch.empty(ilist.size(), dtype=torch.float32, device=dev)
     npart = node_data.npart.to(torch.int32).contiguous()
-    L.call("jzt_knn_node2node", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2),
-           L.ptr(spl_parent.contiguous()), L.ptr(poslvl), L.ptr(npart), L.ptr(radii), L.ptr(ispl), L.ptr(il),
-           L.ptr(ilr2), spl_parent.shape[0] - 1, size, ilist.size(), ilist.size(), int(k), float(boxsize),
-           node_data.pos.shape[-1], L.dtype_code(node_data.pos.dtype))
+    r2in = None if rmax2_in is None else rmax2_in.to(torch.float32).contiguous()
+    assert r2in is None or r2in.shape[0] >= size
+    L.call("jzt_knn_node2node_ex", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2),
+           L.ptr(spl_parent.contiguous()), L.ptr(poslvl), L.ptr(npart), L.ptr(r2in), L.ptr(radii), L.ptr(ispl),
+           L.ptr(il), L.ptr(ilr2), spl_parent.shape[0] - 1, size, ilist.size(), ilist.size(), int(k),
+           float(boxsize), node_data.pos.shape[-1], L.dtype_code(node_data.pos.dtype))
     raise_if(ispl[-1] > il.numel(),
              "The interaction list allocation is too small. (need: {n1}, have: {n2})\n"
              "Hint: increase alloc_fac_ilist", n1=ispl[-1], n2=il.numel())
@@ -139,6 +144,15 @@ def _leaf_parent(n2c: torch.Tensor, size: int) -> torch.Tensor:
     return torch.clamp(inverse_of_splits(n2c, size), 0, size - 1)
 
 
+def _leaf_plane_args(th: TreeHierarchy, n2c: torch.Tensor):
+    """(leaf -> plane-1 parent, leaf centre+level rows, leaf -> particle splits) for the fused
+    leaf-plane kernels."""
+    size = th.size()
+    leaf_parent = _leaf_parent(n2c, size)
+    leaf_nodes = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=leaf_parent).pos_lvl()
+    return leaf_parent, leaf_nodes, th.splits_leaf_to_part().contiguous()
+
+
 def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, alloc_fac_ilist: float = 256.,
                src_ids=None, dst_rows=None, out_rows: int | None = None, init_r2=None):
     """Leaf plane as one fused kernel (extension; see csrc/knn_fused.hip): walks the plane-1 lists
@@ -150,12 +164,9 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     dev = posz.device
     n = len(posz)
     size = th.size()
-    nlevels = th.num_planes()
     ilist_p, rmax_p, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
                                           return_parents=True)
-    leaf_parent = _leaf_parent(n2c, size)
-    leaf_nodes = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=leaf_parent).pos_lvl()
-    l2p = th.splits_leaf_to_part()
+    leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
     if init_r2 is None and rmax_p is not None:
         init_r2 = rmax_p[leaf_parent.long()]
     rows = n if out_rows is None else out_rows
@@ -166,9 +177,42 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
     ir2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
     L.call("jzt_knn_fused", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
-           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p.contiguous()),
+           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
            L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(ir2), L.ptr(retry), L.ptr(rnn), L.ptr(inn),
            size, n, int(k), float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    return rnn, inn, retry
+
+
+def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
+                 alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None):
+    """Exact kNN from a proposed search radius per leaf (extension; k <= 64).
+
+    The dual walk runs as usual down to plane 1.  Leaf lists are then built for the proposed
+    radius leaf_r2 (KnnNode2Node with given radii) instead of the guaranteed one, and the leaf
+    kernel starts its top-k at that bound.  Queries that find fewer than k neighbours inside it are
+    flagged and recomputed by the unbounded fused retry pass, so the result is independent of the
+    proposal: identical to knn() for any leaf_r2 >= 0.  Returns (rnn, inn, retry flags)."""
+    assert k <= 64, "a start bound needs k <= 64"
+    dev = posz.device
+    n = len(posz)
+    size = th.size()
+    boxsize = 0. if boxsize is None else boxsize
+    ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                                     return_parents=True)
+    r2 = leaf_r2.to(torch.float32).contiguous()
+    node_data = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=th.npart(0, size=size))
+    ilist_l = _knn_node2node_ilist(ilist_p, n2c, node_data, k=k, boxsize=boxsize, rmax2_in=r2)
+    spl = th.splits_leaf_to_part()
+    rnn, inn, retry = _knn_leaf2leaf(ilist_l, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids,
+                                     dst_rows=dst_rows, out_rows=out_rows if out_rows is not None else n,
+                                     init_r2=r2)
+    leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
+    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+    L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
+           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
+           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
+           int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
     return rnn, inn, retry
 
 

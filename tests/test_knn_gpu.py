@@ -140,3 +140,50 @@ def test_fused_bound_and_retry(dev, oracle):
     torch.cuda.synchronize()
     assert int(retry.sum()) > 0
     assert torch.equal(i1, i2) and torch.equal(r1, r2_)
+
+
+def _leaf_max_r2k(jz, th, rz, k):
+    """max over each leaf's queries of the k-th neighbour r2 (from z-ordered rows)."""
+    from jztree_amd.tools import inverse_of_splits
+    n = rz.shape[0]
+    spl = th.splits_leaf_to_part()
+    nl = int(th.num(0))
+    leafof = inverse_of_splits(spl[: nl + 1].contiguous(), n).long()
+    r2k = rz[:, k - 1].double() ** 2
+    out = torch.zeros(th.size(), dtype=torch.float64, device=rz.device)
+    return out.scatter_reduce(0, leafof, r2k, "amax", include_self=True).float()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,boxsize", [(16, 1.0), (32, 0.0), (32, 1.0)])
+def test_bounded_matches_plain(dev, oracle, k, boxsize):
+    """Bounded search from a proposed radius per leaf (lists + leaf kernel + retry) equals the plain
+    search for bounds too small (retries), about right, and too large."""
+    import jztree_amd as jz
+    x = clustered(1 << 16, nhalos=64, seed=21)
+    t = torch.from_numpy(x).to(dev)
+    cfg = _clustered_cfg()
+    partz, originz, th = jz.tree.zsort_and_tree(t, cfg.tree, data=jz.data.RankIdx(
+        rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev)))
+    il = jz.knn._knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
+    rz, iz = jz.knn._knn_leaf2leaf(il, th.splits_leaf_to_part(), partz, k=k, boxsize=boxsize, src_ids=originz.idx)
+    exact = _leaf_max_r2k(jz, th, rz, k)
+    mixed = exact.clone()
+    mixed[::3] = 0.0
+    for fac, bound in [(0.25, exact * 0.25), (1.0, exact * (1 + 1e-6)), (3.0, exact * 3.0), ("mixed", mixed)]:
+        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize,
+                                            alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx)
+        jz.checks.flush()
+        torch.cuda.synchronize()
+        assert torch.equal(i1, iz) and torch.equal(r1, rz), f"bound factor {fac}"
+        nr = int(retry.sum())
+        if fac in (0.25, "mixed"):
+            assert nr > 0
+        if fac in (1.0, 3.0):
+            assert nr == 0
+    # and the plain result is the oracle's
+    inn = torch.empty_like(iz)
+    inn[originz.idx.long()] = iz
+    rnn = torch.empty_like(rz)
+    rnn[originz.idx.long()] = rz
+    _check(x, k, rnn.cpu().numpy(), inn.cpu().numpy(), oracle, boxsize=boxsize)
