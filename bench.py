@@ -2,23 +2,32 @@
 
 Workload: 2^26 clustered float32 points in the periodic unit box (70% in 65,536 Plummer halos with
 scale radius 0.002 and power-law member weights dN/dM ~ M^-1.9 on [32, 2^16], 30% uniform; seeded,
-synthetic).  One step = exact kNN (k=32, periodic, neighbours in input order) followed by FoF
-labels (linking length 0.2 x mean spacing, periodic), each from positions resident in HBM: z-order
-sort, tree build, dual tree walk, leaf kernels.
+synthetic).  One step = exact kNN (k=32, periodic, neighbours in input order) with the learned
+radius proposer, followed by FoF labels (linking length 0.2 x mean spacing, periodic), each from
+positions resident in HBM: z-order sort, tree build, (features + MLP bound,) dual tree walk, leaf
+kernels.  The plain exact kNN (guaranteed radii, no proposer) is timed on the same points; both
+return bit-identical neighbours, which the run checks.
 
-Prints one JSON line.  value = kNN queries/s of the step's kNN call; FoF throughput is reported
-next to it.  roofline = the kNN leaf kernel (the dominant kernel of the step), timed with HIP
-events on the launch stream; cpu_baseline = the oracle's grid kNN on the host cores for a bounded
-sample of queries of the same point set.
+The proposer is trained first, on the GPU with the library's own kernels, on the plain kNN radii
+of 4 other seeds of the same configuration (generated on the host in parallel); its margin is
+calibrated on those seeds.
+
+Prints one JSON line.  value = learned kNN queries/s; plain queries/s, the ratio, the fallback
+rate and the log2-radius RMSE on the benchmark points, and FoF throughput are reported next to it.
+roofline = the leaf kernel of the learned search (the step's dominant kernel), timed with HIP
+events on the launch stream; cpu_baseline = the oracle's k-d tree kNN on the host cores for a
+bounded sample of queries of the same point set.
 """
 from __future__ import annotations
 
 import argparse
 import dataclasses
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
+from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
 
@@ -27,6 +36,10 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of the leaf kernel per launch on this workload, from two rocprofv3 --pmc passes
+# (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain copy kernel of known size reads);
+# keyed by (kernel, n, k, halos); None where not measured for the current kernel.
+LEAF_TRAFFIC_BYTES = {}
 
 
 def parse():
@@ -38,9 +51,20 @@ def parse():
     p.add_argument("--halos", type=int, default=65536)
     p.add_argument("--k", type=int, default=32)
     p.add_argument("--b", type=float, default=0.2)
-    p.add_argument("--cpu-queries", type=int, default=1 << 20)
+    p.add_argument("--train-seeds", type=int, default=4)
+    p.add_argument("--train-steps", type=int, default=2000)
+    p.add_argument("--quantile", type=float, default=0.98)
+    p.add_argument("--proposer", default=None, help="load a trained proposer (.npz) instead of training")
+    p.add_argument("--save-proposer", default=None)
+    p.add_argument("--cpu-queries", type=int, default=1 << 21)
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
+
+
+def _gen(args):
+    from gen import clustered
+    n, halos, seed = args
+    return clustered(n, nhalos=halos, seed=seed)
 
 
 def main():
@@ -50,15 +74,19 @@ def main():
               "multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
+    n, k = args.n, args.k
+    # point sets first, in worker processes started before anything touches the GPU
+    t0 = time.time()
+    seeds = [0] + ([] if args.proposer else [1000 + s for s in range(args.train_seeds)])
+    with ProcessPoolExecutor(max_workers=min(len(seeds), 5), mp_context=mp.get_context("fork")) as ex:
+        sets = list(ex.map(_gen, [(n, args.halos, s) for s in seeds]))
+    xh = sets[0]
+    print(f"# generated {len(sets)} sets of {n} points in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
     import torch
     import jztree_amd as jz
-    from gen import clustered
 
     dev = torch.device("cuda:0")
-    n, k = args.n, args.k
-    t0 = time.time()
-    xh = clustered(n, nhalos=args.halos, seed=0)
-    print(f"# generated {n} points in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     x = torch.from_numpy(xh).to(dev)
 
     # tree settings for this workload: leaves of <= 64 points (one per lane of a wave64) and a
@@ -68,52 +96,105 @@ def main():
     fcfg = jz.config.FofConfig()
     fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0, max_leaf_size=64))
     rlink = args.b * n ** (-1.0 / 3.0)
+    box = 1.0
+
+    # ---- proposer: train on other seeds (or load) ----
+    t0 = time.time()
+    if args.proposer:
+        prop = jz.proposer.RadiusProposer.load(args.proposer, device=dev)
+        train_info = {"loaded": os.path.basename(args.proposer)}
+    else:
+        feats, targets = [], []
+        for xs in sets[1:]:
+            f, t = jz.proposer.training_data([torch.from_numpy(xs).to(dev)], k, kcfg, boxsize=box)
+            feats.append(f)
+            targets.append(t)
+        feats, targets = torch.cat(feats), torch.cat(targets)
+        prop = jz.proposer.RadiusProposer(k, device=dev, seed=0)
+        rep = prop.train(feats, targets, steps=args.train_steps, batch=8192, lr=3e-3, lr_final=1e-4)
+        prop.calibrate(feats, targets, quantile=args.quantile)
+        torch.cuda.synchronize()
+        train_info = {"seeds": seeds[1:], "leaves": int(len(targets)), "steps": args.train_steps,
+                      "loss_first": rep.loss_first, "loss_last": rep.loss_last, "margin_log2": prop.margin,
+                      "quantile": args.quantile, "seconds": round(time.time() - t0, 1)}
+        del feats, targets
+        if args.save_proposer:
+            prop.save(args.save_proposer)
+    del sets
+    print(f"# proposer ready: {train_info}", file=sys.stderr, flush=True)
 
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
-    def step():
-        e0, e1, e2 = ev(), ev(), ev()
-        e0.record()
-        rnn, inn = jz.knn.knn(x, k=k, boxsize=1.0, cfg=kcfg)
-        e1.record()
-        _, ig = jz.fof.fof_labels(x, rlink, boxsize=1.0, cfg=fcfg)
-        e2.record()
-        return e0, e1, e2
+    def step_plain():
+        return jz.knn.knn(x, k=k, boxsize=box, cfg=kcfg)
 
-    for i in range(args.warmup):
-        step()
+    def step_learned():
+        return jz.proposer.knn_learned(x, k, prop, boxsize=box, cfg=kcfg)
+
+    def step_fof():
+        return jz.fof.fof_labels(x, rlink, boxsize=box, cfg=fcfg)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
         torch.cuda.synchronize()
-        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+        evs = []
+        for _ in range(steps):
+            a, b = ev(), ev()
+            a.record()
+            fn()
+            b.record()
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in evs]))
 
+    # exactness of the learned search on the benchmark points, and prediction quality
+    r0, i0 = step_plain()
+    r1, i1, retry = jz.proposer.knn_learned(x, k, prop, boxsize=box, cfg=kcfg, return_retry=True)
     torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    evs = [step() for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    t_knn = float(np.mean([a.elapsed_time(b) for a, b, _ in evs]))
-    t_fof = float(np.mean([b.elapsed_time(c) for _, b, c in evs]))
-    print(f"# timed {args.steps} steps: knn {t_knn:.2f} ms, fof {t_fof:.2f} ms, wall {wall:.2f} s",
-          file=sys.stderr, flush=True)
-
-    # dominant kernel: the kNN leaf kernel, timed alone on the same stream (one launch per call)
+    identical = bool(torch.equal(i0, i1) and torch.equal(r0, r1))
+    fallback = float(retry.float().mean())
     partz, originz, th = jz.tree.zsort_and_tree(x, kcfg.tree, data=jz.data.RankIdx(
         rank=None, idx=torch.arange(n, dtype=torch.int32, device=dev)))
-    ilist = jz.knn._knn_dual_walk(th, k, boxsize=1.0, alloc_fac_ilist=kcfg.alloc_fac_ilist)
+    nl = int(th.num(0))
+    rz = r0[originz.idx.long()]  # z-order rows
+    target = jz.proposer.leaf_log2_radius(th, rz, k)[:nl]
+    pred = prop.predict_log2r(jz.proposer.leaf_features(th)[:nl])
+    rmse = float(torch.sqrt(torch.mean((pred.double() - target.double()) ** 2)))
+    del r0, i0, r1, i1, rz
+    print(f"# identical {identical}, fallback rate {fallback:.4f}, log2-radius RMSE {rmse:.3f}",
+          file=sys.stderr, flush=True)
+
+    wall0 = time.perf_counter()
+    t_plain = timed(step_plain, args.steps, args.warmup)
+    t_learn = timed(step_learned, args.steps, args.warmup)
+    t_fof = timed(step_fof, args.steps, args.warmup)
+    wall = time.perf_counter() - wall0
+    print(f"# timed {args.steps} steps each: plain knn {t_plain:.2f} ms, learned knn {t_learn:.2f} ms, "
+          f"fof {t_fof:.2f} ms (wall {wall:.1f} s)", file=sys.stderr, flush=True)
+
+    # dominant kernel: the leaf kernel of the learned search, timed alone on the launch stream
+    bound = prop.bound_r2(th)
+    ilb = jz.knn._knn_radius_walk(th, k, jz.knn._plane_radii(th, bound), boxsize=box,
+                                  alloc_fac_ilist=kcfg.alloc_fac_ilist)
     spl = th.splits_leaf_to_part()
     jz.checks.flush()
-    jz.knn._knn_leaf2leaf(ilist, spl, partz, k=k, boxsize=1.0, src_ids=originz.idx)
+    leaf = lambda: jz.knn._knn_leaf2leaf(ilb, spl, partz, k=k, boxsize=box, src_ids=originz.idx,
+                                        dst_rows=originz.idx, init_r2=bound)
+    leaf()
     torch.cuda.synchronize()
     reps = 5
     ka, kb = ev(), ev()
     ka.record()
     for _ in range(reps):
-        jz.knn._knn_leaf2leaf(ilist, spl, partz, k=k, boxsize=1.0, src_ids=originz.idx)
+        leaf()
     kb.record()
     torch.cuda.synchronize()
     t_leaf = ka.elapsed_time(kb) / reps
     alg_bytes = n * (16 + 8 * k)  # read query (12) + origin index (4), write k (radius, id) pairs
     achieved = alg_bytes / (t_leaf * 1e-3) / 1e9
+    traffic = LEAF_TRAFFIC_BYTES.get(("k_knn_leaf", n, k, args.halos))
 
     cpu = None
     if not args.no_cpu:
@@ -121,33 +202,41 @@ def main():
         nq = min(args.cpu_queries, n)
         qsel = np.random.default_rng(1).choice(n, nq, replace=False)
         c0 = time.perf_counter()
-        O.knn(xh, k, xq=xh[qsel], boxsize=1.0)
+        O.knn(xh, k, xq=xh[qsel], boxsize=box)
         ct = time.perf_counter() - c0
         cpu = {"value": nq / ct, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"oracle grid kNN (k={k}, periodic) of {nq} random queries against all {n} points, "
-                         f"incl. grid build ({ct:.1f} s)"}
+               "sample": f"oracle k-d tree kNN (k={k}, periodic) of {nq} random queries against all {n} points, "
+                         f"incl. tree build ({ct:.1f} s)"}
 
     out = {
         "metric": "kNN queries/s (k=32, 2^26 pts) learned vs plain; FoF Mparticles/s; %HBM peak",
-        "value": n / (t_knn * 1e-3),
+        "value": n / (t_learn * 1e-3),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_knn + t_fof,
+        "ms_per_step": t_learn + t_fof,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded clustered point set, generated in-process)",
+        "data": "synthetic (seeded clustered point sets, generated in-process; proposer trained in-process)",
         "config": {"workload": f"clustered 2^{int(np.log2(n))} points ({args.halos} Plummer halos, 70% in halos), "
-                               f"periodic unit box; kNN k={k} (plain) + FoF b={args.b}",
-                   "n_points": n, "k": k, "fof_b": args.b, "knn_method": "plain"},
-        "knn_ms": t_knn,
+                               f"periodic unit box; exact kNN k={k} with the learned radius proposer, "
+                               f"then FoF b={args.b}",
+                   "n_points": n, "k": k, "fof_b": args.b, "leaf_size": 64},
+        "plain_queries_per_s": n / (t_plain * 1e-3),
+        "learned_speedup": t_plain / t_learn,
+        "learned_identical": identical,
+        "fallback_rate": fallback,
+        "log2_radius_rmse": rmse,
+        "proposer": train_info,
+        "knn_plain_ms": t_plain,
+        "knn_learned_ms": t_learn,
         "fof_ms": t_fof,
         "fof_mparticles_per_s": n / (t_fof * 1e-3) / 1e6,
-        "roofline": {"bound": "hbm", "kernel": "k_knn_leaf (KnnLeaf2Leaf)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": "k_knn_leaf (KnnLeaf2Leaf, bounded)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": t_leaf, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -152,7 +152,8 @@ int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32
  *     written), so results can land in input order directly;
  *   init_r2 (size_leaves_query float, k <= 64): starting bound per query leaf.  Only candidates with
  *     r2 <= init_r2 enter the top-k.  The caller guarantees that each query leaf's list holds every
- *     leaf within init_r2 of it; then every query that finds k neighbours is exact;
+ *     leaf within init_r2 of it; then every query that finds k neighbours is exact.  A leaf with
+ *     init_r2 < 0 (or NaN) is skipped: its rows and retry flags are left as they are;
  *   retry (size_part_query int32, with init_r2): 1 for queries left with fewer than k neighbours
  *     inside the bound (their rows hold sentinels and must be recomputed), else 0. */
 int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,

@@ -156,20 +156,53 @@ __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil
   }
 }
 
-template <int D, typename T>
+template <int D, typename T> struct alignas(sizeof(T) == 4 ? 16 : 8) FofRec {
+  T p[D];
+  int32_t g;
+};
+
+template <typename T> __device__ __forceinline__ T wave_min_t(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max_t(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One wave per query leaf; each leaf pair once (source index >= query index).  Source particles
+// are staged 64 at a time through LDS as (position, label) records.  Before staging, each lane
+// tests its source particle p against the bounding box [qlo, qhi] of the query chunk:
+//   - wrap: some difference fl(q - p), q in the box, may leave [-half, half) in some dim (by
+//     monotone rounding it suffices to test fl(qlo - p) and fl(qhi - p)); only then does the chunk
+//     use the periodic wrap (the distances are bit-identical either way);
+//   - reach: p is within r2link * (1 + 2^-16) of the box (or needs the wrap).  A p that fails
+//     cannot link with any query of the chunk, because the box gap bounds every |q - p| per dim
+//     from below (the slack covers the rounding of the gap and of the canonical distance).
+// Only reachable sources are visited; a chunk with none is skipped.
+template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
                                                  float r2link, T boxsize, int64_t il_size) {
-  __shared__ Vec<D, T> s_x[64];
-  __shared__ int s_g[64];
+  __shared__ FofRec<D, T> s_r[64];
   const int lane = threadIdx.x;
   const int lq = blockIdx.x;
   const int q0 = spl[lq], q1 = spl[lq + 1];
   const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
+  const T half = (T)(float)(T(0.5) * boxsize);
+  const T r2skip = (T)(r2link * (1.f + kBoundSlack));
   for (int qoff = q0; qoff < q1; qoff += 64) {
     const int iq = min(qoff + lane, q1 - 1);
     const bool valid = qoff + lane < q1;
-    Vec<D, T> xq = load_vec<D, T>(pos, iq);
+    const Vec<D, T> xq = load_vec<D, T>(pos, iq);
+    Vec<D, T> qlo, qhi;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      qlo[d] = wave_min_t<T>(xq[d]);
+      qhi[d] = wave_max_t<T>(xq[d]);
+    }
     const int gq = ig[iq];
     int glast = -1;
     for (int e = e0; e < e1; e++) {
@@ -178,19 +211,39 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
       const int t0 = spl[lt], t1 = spl[lt + 1];
       for (int toff = t0; toff < t1; toff += 64) {
         const int it = toff + lane;
+        FofRec<D, T> r;
+        bool need_wrap = false, reach = false;
         if (it < t1) {
-          s_x[lane] = load_vec<D, T>(pos, it);
-          s_g[lane] = ig[it];
+          const Vec<D, T> p = load_vec<D, T>(pos, it);
+          T g2 = T(0);
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            r.p[d] = p[d];
+            const T dlo = qlo[d] - p[d], dhi = qhi[d] - p[d];
+            if (PER) need_wrap |= !(dlo >= -half) || !(dhi < half);
+            const T gap = max(T(0), max(dlo, -dhi));
+            g2 = fmaT(gap, gap, g2);
+          }
+          r.g = ig[it];
+          reach = need_wrap || g2 <= r2skip;
         }
+        uint64_t m = __ballot(reach);
+        if (m == 0ull) continue;
+        const bool wrap = PER && wave_any(need_wrap);
+        s_r[lane] = r;
         __syncthreads();
-        const int nl = min(t1 - toff, 64);
         if (valid) {
-          for (int j = 0; j < nl; j++) {
-            const int gj = s_g[j];
+          while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1ull;
+            const int gj = s_r[j].g;
             // labels are snapshots of set members; a label equal to q's own, or to one q was
             // already joined with, belongs to q's set by now
             if (gj == gq || gj == glast) continue;
-            T r2 = dist2<D, T>(xq, s_x[j], boxsize);
+            Vec<D, T> xp;
+#pragma unroll
+            for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
+            const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
             if (r2 <= (T)r2link) {
               uf_link(ig, iq, toff + j);
               glast = gj;
@@ -247,9 +300,14 @@ int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64
                   float r2link, float boxsize) {
   if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_part * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return fail("FofLeaf2Leaf: copy failed");
-  if (size_leaves > 0)
-    hipLaunchKernelGGL((k_fof_leaf<D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl, (const T*)pos, ig,
-                       r2link, (T)boxsize, il_size);
+  if (size_leaves > 0) {
+    if (boxsize > 0.f)
+      hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, (T)boxsize, il_size);
+    else
+      hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, (T)boxsize, il_size);
+  }
   if (size_part > 0)
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);
   return check_launch("FofLeaf2Leaf");

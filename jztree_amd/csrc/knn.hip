@@ -330,7 +330,8 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // and the canonical wrap leaves it unchanged: the distances are bit-identical.
 //
 // Optional start bound (init_r2, per query leaf, K <= 64 single pass): the slots start at
-// (r2_0, id = 2^32-1), so only candidates with r2 <= r2_0 enter.  A query that ends with fewer
+// (r2_0, id = 2^32-1), so only candidates with r2 <= r2_0 enter.  A leaf with r2_0 < 0 is skipped
+// entirely (its rows and retry flags are not written), so a later pass can redo selected leaves.  A query that ends with fewer
 // than K real neighbours is flagged in `retry` (its row holds the sentinel entries); every other
 // query's row is exact, because all its K neighbours lie within r2_0 and the list holds every
 // leaf within r2_0 (the caller's contract).
@@ -350,8 +351,10 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
   const int q0 = splQ[lq], q1 = splQ[lq + 1];
   const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
   const float half = 0.5f * boxsize;
-  const uint64_t start_key =
-      init_r2 ? (((uint64_t)__float_as_uint(init_r2[lq]) << 32) | 0xffffffffull) : kEmptyKey;
+  // a negative (or NaN) start bound marks a leaf that this launch leaves untouched
+  const float r2_0 = init_r2 ? init_r2[lq] : 0.f;
+  if (init_r2 && !(r2_0 >= 0.f)) return;
+  const uint64_t start_key = init_r2 ? (((uint64_t)__float_as_uint(r2_0) << 32) | 0xffffffffull) : kEmptyKey;
   for (int qoff = q0; qoff < q1; qoff += 64) {
     const int iq = min(qoff + lane, q1 - 1);
     const Vec<D, float> q = load_vec<D, float>(xQ, iq);

@@ -18,7 +18,7 @@ from . import checks
 from .checks import raise_if
 from .config import KNNConfig
 from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
-from .tools import masked_inverse
+from .tools import inverse_of_splits, masked_inverse
 from .tree import _tmp, grouped_dense_interaction_list, zsort_and_tree
 
 
@@ -39,14 +39,15 @@ class PosLvlNum:
 
 
 def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, boxsize=0., src_ids=None,
-                   dst_rows=None, out_rows: int | None = None, init_r2=None):
+                   dst_rows=None, out_rows: int | None = None, init_r2=None, out=None):
     """knn.py:28-60 -> KnnLeaf2Leaf.  Returns (rnn (rows,k), inn (rows,k) int32), plus the retry
     flags (nq int32) when init_r2 is given.
 
     Extensions: src_ids = id reported for each source particle (default its z-index); dst_rows =
     output row of each query (default its z-index; rows = out_rows, default nq); init_r2 = starting
     bound per query leaf (k <= 64; the lists must hold every leaf within it), queries left with
-    fewer than k neighbours inside it are flagged and their rows must be recomputed."""
+    fewer than k neighbours inside it are flagged and their rows must be recomputed; leaves with a
+    negative bound are skipped.  out = (rnn, inn[, retry]) writes into existing buffers."""
     if xQ is None:
         xQ = xT
     if splQ is None:
@@ -59,14 +60,22 @@ def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, b
     nq = xQ.shape[0]
     rows = nq if out_rows is None else int(out_rows)
     dt = L.dtype_code(xT.dtype)
-    rnn = torch.empty((rows, k), dtype=xT.dtype, device=dev)
-    inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
+    if out is not None:
+        rnn, inn = out[0], out[1]
+        assert rnn.shape == (rows, k) and inn.shape == (rows, k) and rnn.is_contiguous() and inn.is_contiguous()
+        retry = out[2] if len(out) > 2 else None
+    else:
+        rnn = torch.empty((rows, k), dtype=xT.dtype, device=dev)
+        inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
+        retry = None
     nb = L.LIB.jzt_knn_leaf2leaf_tmp_bytes(nq, k, dt)
     tmp = _tmp(nb, dev)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
     ir2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
-    retry = None if init_r2 is None else torch.empty(nq, dtype=torch.int32, device=dev)
+    if init_r2 is not None and retry is None:
+        retry = torch.empty(nq, dtype=torch.int32, device=dev)
+    assert retry is None or (retry.shape == (nq,) and retry.dtype == torch.int32)
     L.call("jzt_knn_leaf2leaf_ex", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(ilist.rad2),
            ilist.size(), L.ptr(splT), L.ptr(xT.contiguous()), L.ptr(splQ), L.ptr(xQ.contiguous()), L.ptr(sid),
            L.ptr(drow), L.ptr(ir2), L.ptr(retry), L.ptr(rnn), L.ptr(inn), L.ptr(tmp), nb, len(splQ) - 1, nq, rows,
@@ -183,37 +192,89 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     return rnn, inn, retry
 
 
-def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
-                 alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None):
-    """Exact kNN from a proposed search radius per leaf (extension; k <= 64).
+def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor) -> list:
+    """Search radius^2 for every plane: leaf_r2 for the leaves (negative = no search), and for a
+    node the largest radius among its children.  A node's list then holds every node that a leaf
+    inside it may reach: mindist(child, T') >= mindist(node, T) for children of T."""
+    size = th.size()
+    dev = leaf_r2.device
+    ar = torch.arange(size, device=dev)
+    radii = [torch.where(ar < th.num(0), leaf_r2.float(), torch.full_like(leaf_r2, -1.0, dtype=torch.float32))]
+    for p in range(1, th.num_planes()):
+        parent = _leaf_parent(th.ispl_n2n.get(p, size + 1), size).long()
+        child = torch.where(ar < th.num(p - 1), radii[-1], torch.full_like(radii[-1], -1.0))
+        r = torch.full((size,), -1.0, dtype=torch.float32, device=dev).scatter_reduce(0, parent, child, "amax")
+        radii.append(torch.where(ar < th.num(p), r, torch.full_like(r, -1.0)))
+    return radii
 
-    The dual walk runs as usual down to plane 1.  Leaf lists are then built for the proposed
-    radius leaf_r2 (KnnNode2Node with given radii) instead of the guaranteed one, and the leaf
-    kernel starts its top-k at that bound.  Queries that find fewer than k neighbours inside it are
-    flagged and recomputed by the unbounded fused retry pass, so the result is independent of the
-    proposal: identical to knn() for any leaf_r2 >= 0.  Returns (rnn, inn, retry flags)."""
+
+def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0., alloc_fac_ilist: float = 256.):
+    """Dual walk with given search radii on every plane (no guaranteed-radius search): from the dense
+    top-plane list down to leaf lists holding every leaf within each leaf's radius."""
+    nlevels = th.num_planes()
+    size = th.size()
+    spl, ilist, nsup = grouped_dense_interaction_list(
+        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size)
+    ilist.rad2 = torch.zeros(ilist.isrc.shape, dtype=torch.float32, device=ilist.isrc.device)
+    spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
+    raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
+    for i in range(nlevels):
+        level = nlevels - i - 1
+        parent_spl = spl_n2n.get(level + 1, size + 1)
+        node_data = PosLvlNum(pos=th.geom_cent.get(level, size), lvl=th.lvl.get(level, size),
+                              npart=th.npart(level, size=size, ptype=0))
+        ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize, rmax2_in=radii[level])
+    return ilist
+
+
+def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
+                 alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None,
+                 max_rounds: int = 3, grow: float = 16.0):
+    """Exact kNN from a proposed search radius^2 per leaf (extension; k <= 64).
+
+    Round 1 walks the tree with the proposed radii on every plane (a node searches as far as its
+    farthest-reaching child) and runs the leaf kernel with each leaf's radius as starting bound.
+    Queries that find fewer than k neighbours inside it are flagged; their leaves are redone with
+    the radius^2 grown by `grow` while every other leaf sits out, up to max_rounds rounds.  Anything
+    still flagged then is recomputed by the unbounded fused retry pass on the guaranteed-radius
+    lists.  The result is therefore independent of the proposal: identical to knn() for any
+    leaf_r2.  Returns (rnn, inn, retry flags of round 1, rounds run)."""
     assert k <= 64, "a start bound needs k <= 64"
     dev = posz.device
     n = len(posz)
     size = th.size()
     boxsize = 0. if boxsize is None else boxsize
-    ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
-                                     return_parents=True)
-    r2 = leaf_r2.to(torch.float32).contiguous()
-    node_data = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=th.npart(0, size=size))
-    ilist_l = _knn_node2node_ilist(ilist_p, n2c, node_data, k=k, boxsize=boxsize, rmax2_in=r2)
+    rows = n if out_rows is None else out_rows
     spl = th.splits_leaf_to_part()
-    rnn, inn, retry = _knn_leaf2leaf(ilist_l, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids,
-                                     dst_rows=dst_rows, out_rows=out_rows if out_rows is not None else n,
-                                     init_r2=r2)
-    leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
-    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
-    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
-    L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
-           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
-           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
-           int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
-    return rnn, inn, retry
+    leafof = torch.clamp(inverse_of_splits(spl, n), 0, size - 1).long()
+    rnn = torch.empty((rows, k), dtype=posz.dtype, device=dev)
+    inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
+    retry = torch.zeros(n, dtype=torch.int32, device=dev)
+    r2 = leaf_r2.to(torch.float32).contiguous()
+    first = None
+    rounds = 0
+    for rnd in range(max_rounds):
+        rounds += 1
+        il = _knn_radius_walk(th, k, _plane_radii(th, r2), boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist)
+        _knn_leaf2leaf(il, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids, dst_rows=dst_rows, out_rows=rows,
+                       init_r2=r2, out=(rnn, inn, retry))
+        if first is None:
+            first = retry.clone()
+        leaf_short = torch.zeros(size, dtype=torch.int32, device=dev).scatter_reduce(0, leafof, retry, "amax")
+        if rnd == max_rounds - 1 or not bool(leaf_short.any()):
+            break
+        r2 = torch.where(leaf_short > 0, r2 * grow, torch.full_like(r2, -1.0)).contiguous()
+    if bool(retry.any()):
+        ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                                         return_parents=True)
+        leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
+        sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+        drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+        L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
+               ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
+               L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
+               int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
+    return rnn, inn, first, rounds
 
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,

@@ -230,8 +230,8 @@ def training_data(points: list, k: int, cfg: KNNConfig = KNNConfig(), boxsize: f
 def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = None,
                 cfg: KNNConfig = KNNConfig(), output_order: str = "input", return_retry: bool = False):
     """Exact kNN (radii, global indices) with the proposer's starting bound; identical to
-    knn(part, k, boxsize, output_order=...).  With return_retry, also the per-query fallback flags
-    (z order)."""
+    knn(part, k, boxsize, output_order=...).  With return_retry, also the per-query flags of the
+    queries that the first (proposed-radius) round could not finish (z order)."""
     from .knn import _knn_bounded
     from .tree import zsort_and_tree
     assert output_order in ("z", "input")
@@ -247,8 +247,8 @@ def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = 
             sid, dst = originz.idx, originz.idx
         else:  # z order: rows and ids are z-indices
             sid, dst = None, None
-        rnn, inn, retry = _knn_bounded(th, get_pos(partz), k, bound, boxsize=boxsize,
-                                       alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=sid, dst_rows=dst)
+        rnn, inn, retry, _ = _knn_bounded(th, get_pos(partz), k, bound, boxsize=boxsize,
+                                          alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=sid, dst_rows=dst)
         checks.flush()
     except BaseException:
         checks.clear()

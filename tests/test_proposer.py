@@ -103,19 +103,28 @@ def _tree(dev, n=1 << 16, seed=3, leaf=32):
 
 
 @pytest.mark.gpu
-def test_leaf_features_vs_oracle(dev):
+@pytest.mark.parametrize("n,leaf", [(1 << 19, 16), (1 << 17, 32), (3000, 64)])
+def test_leaf_features_vs_oracle(dev, n, leaf):
     import jztree_amd as jz
     from jztree_amd.proposer import _plane_parent
-    _, _, _, _, th = _tree(dev, n=1 << 17)
-    assert th.num_planes() >= 3
+    _, _, _, _, th = _tree(dev, n=n, leaf=leaf)
+    nplanes = th.num_planes()
+    print(f"[features] n={n} leaf={leaf} planes={nplanes}", flush=True)
     f = jz.proposer.leaf_features(th).cpu().numpy()
     size = th.size()
     g = lambda t: t.cpu().numpy().astype(np.int64)
-    n2c1 = th.ispl_n2n.get(1, size + 1).to(torch.int32)
-    ref = P.leaf_features(g(th.npart(0, size=size)), g(th.lvl.get(0, size)), g(_plane_parent(n2c1, size)),
-                          g(th.npart(1, size=size)), g(th.lvl.get(1, size)), g(n2c1),
-                          g(_plane_parent(th.ispl_n2n.get(2, size + 1).to(torch.int32), size)),
-                          g(th.npart(2, size=size)), g(th.lvl.get(2, size)), nleaves=int(th.num(0)), size=size)
+    npart0, lvl0 = g(th.npart(0, size=size)), g(th.lvl.get(0, size))
+    if nplanes >= 2:
+        n2c1 = th.ispl_n2n.get(1, size + 1).to(torch.int32)
+        par1, npart1, lvl1, n2c1 = g(_plane_parent(n2c1, size)), g(th.npart(1, size=size)), g(th.lvl.get(1, size)), g(n2c1)
+    else:
+        par1, npart1, lvl1, n2c1 = np.arange(size), npart0, lvl0, np.arange(size + 1)
+    par2 = npart2 = lvl2 = None
+    if nplanes >= 3:
+        par2 = g(_plane_parent(th.ispl_n2n.get(2, size + 1).to(torch.int32), size))
+        npart2, lvl2 = g(th.npart(2, size=size)), g(th.lvl.get(2, size))
+    ref = P.leaf_features(npart0, lvl0, par1, npart1, lvl1, n2c1, par2, npart2, lvl2, nleaves=int(th.num(0)),
+                          size=size)
     assert np.allclose(f, ref, rtol=1e-6, atol=1e-6)
 
 
@@ -202,11 +211,8 @@ def test_trained_proposer_exact_and_saves_work(dev, k, boxsize):
     partz, originz, th = jz.tree.zsort_and_tree(x, cfg.tree, data=jz.data.RankIdx(
         rank=None, idx=torch.arange(n, dtype=torch.int32, device=dev)))
     plain = jz.knn._knn_dual_walk(th, k, boxsize=boxsize or None, alloc_fac_ilist=cfg.alloc_fac_ilist)
-    ilp, _, n2c = jz.knn._knn_dual_walk(th, k, boxsize=boxsize or None, alloc_fac_ilist=cfg.alloc_fac_ilist,
-                                        stop_level=1, return_parents=True)
-    size = th.size()
-    nd = jz.knn.PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=th.npart(0, size=size))
-    learned = jz.knn._knn_node2node_ilist(ilp, n2c, nd, k=k, boxsize=boxsize, rmax2_in=prop.bound_r2(th))
+    learned = jz.knn._knn_radius_walk(th, k, jz.knn._plane_radii(th, prop.bound_r2(th)), boxsize=boxsize,
+                                      alloc_fac_ilist=cfg.alloc_fac_ilist)
     ne_plain, ne_learned = int(plain.ispl[-1]), int(learned.ispl[-1])
     print(f"[proposer] leaf-list entries plain {ne_plain} learned {ne_learned}", flush=True)
     assert ne_learned < 0.6 * ne_plain
