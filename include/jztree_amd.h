@@ -135,6 +135,12 @@ int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_ilist_spl, c
                          int64_t size_parents, int64_t size_nodes, int64_t parent_ilist_size,
                          int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
 
+/* Segment maximum (extension, used to pass search radii up the tree planes): out[i] = max of
+ * vals over [spl[i], spl[i+1]) clamped to [0, nvals), or `empty` for an empty segment.  spl has
+ * nseg+1 entries. */
+int jzt_segment_max_f32(jzt_stream_t stream, const int32_t* spl, const float* vals, int64_t nvals, float* out,
+                        int64_t nseg, float empty);
+
 /* KnnLeaf2Leaf -- replaces ffi target "KnnLeaf2Leaf" (generated/ffi_knn.cu:31-188, host
  * knn.cuh:309-356, caller knn.py:28-60).  knn_rad (nq, k) of dtype, knn_id (nq, k) int32.
  * Neighbours ascending by (canonical r2, id); id = z-index in xT, or src_ids[z-index] when src_ids

@@ -25,6 +25,9 @@ namespace jzt {
 namespace {
 
 constexpr float kRmaxSlack = 1.0f + kBoundSlack;
+#ifndef JZT_LEAF_WAVES
+#define JZT_LEAF_WAVES 4
+#endif
 
 template <int D, typename T> struct NodeT {
   Vec<D, T> cent;
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // query's row is exact, because all its K neighbours lie within r2_0 and the list holds every
 // leaf within r2_0 (the caller's contract).
 template <int K, bool PER, bool FLOOR, int D>
-__global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZT_LEAF_WAVES : 1))) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
                                                  const float* __restrict__ il_r2, const int32_t* __restrict__ splT,
                                                  const float* __restrict__ xT, const int32_t* __restrict__ splQ,
                                                  const float* __restrict__ xQ, const int32_t* __restrict__ src_ids,
@@ -346,8 +349,11 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
                                                  uint64_t* __restrict__ resume, int koff, int ktot, float boxsize,
                                                  int64_t il_size, int64_t out_rows) {
   __shared__ SrcRec s_rec[64];
+  __shared__ uint64_t s_buf[kBufWords];
   const int lane = threadIdx.x;
   const int lq = blockIdx.x;
+  uint64_t* const row = s_buf + lane * kBufStride;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   const int q0 = splQ[lq], q1 = splQ[lq + 1];
   const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
   const float half = 0.5f * boxsize;
@@ -359,17 +365,16 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
     const int iq = min(qoff + lane, q1 - 1);
     const Vec<D, float> q = load_vec<D, float>(xQ, iq);
     const uint64_t floor_key = FLOOR ? resume[iq] : 0ull;
-    Vec<D, float> qlo, qhi;
-    if constexpr (PER) {
+    Vec<D, float> qlo, qhi;  // bounding box of the query chunk
 #pragma unroll
-      for (int d = 0; d < D; d++) {
-        qlo[d] = wave_minf(q[d]);
-        qhi[d] = wave_maxf(q[d]);
-      }
+    for (int d = 0; d < D; d++) {
+      qlo[d] = wave_minf(q[d]);
+      qhi[d] = wave_maxf(q[d]);
     }
     uint64_t best[K];
 #pragma unroll
     for (int i = 0; i < K; i++) best[i] = start_key;
+    int cnt = 0;  // entries in this lane's buffer row
     bool done = false;
     for (int eb = e0; eb < e1 && !done; eb += 64) {
       const int ne = min(64, e1 - eb);
@@ -410,16 +415,27 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
           noff = 0;
         }
         const int n_cur = cn;
-        bool wrap = false;
-        if constexpr (PER) {
-          bool need = false;
-          if (lane < n_cur) {
+        // Source filter: p can enter some lane's top-k only if its gap to the query box is within
+        // the largest current k-th distance of the wave (or a periodic wrap is possible); the
+        // kept records are compacted into LDS.  Exact: by monotone rounding every canonical r2
+        // from the box is >= the gap^2 computed here with the same FMA chain.
+        const float kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
+        bool need = false, reach = false;
+        if (lane < n_cur) {
+          float g2 = 0.f;
 #pragma unroll
-            for (int d = 0; d < D; d++) need |= !((qlo[d] - cr.p[d]) >= -half) || !((qhi[d] - cr.p[d]) < half);
+          for (int d = 0; d < D; d++) {
+            const float dlo = qlo[d] - cr.p[d], dhi = qhi[d] - cr.p[d];
+            if (PER) need |= !(dlo >= -half) || !(dhi < half);
+            const float gap = fmaxf(0.f, fmaxf(dlo, -dhi));
+            g2 = fmaf(gap, gap, g2);
           }
-          wrap = wave_any(need);
+          reach = need || g2 <= kmax;
         }
-        if (lane < n_cur) s_rec[lane] = cr;
+        const uint64_t keep = __ballot(reach);
+        const bool wrap = PER && wave_any(need);
+        const int n_keep = __popcll(keep);
+        if (reach) s_rec[__popcll(keep & lt_mask)] = cr;
         __syncthreads();
         if (ni < ne) {
           const int t0 = rdlane(m_t0, ni) + noff, t1 = rdlane(m_t1, ni);
@@ -431,12 +447,13 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
             cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
           }
         }
-        consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_cur, wrap, boxsize, half, floor_key);
+        consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_keep, wrap, boxsize, half, floor_key, row, cnt);
         __syncthreads();
         i = ni;
         off = noff;
       }
     }
+    if (wave_any(cnt > 0)) topk_flush<K>(best, row, cnt);
     if (qoff + lane < q1) {
       const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
       if (orow >= 0 && orow < out_rows) {
@@ -464,6 +481,17 @@ __global__ __launch_bounds__(64) void k_knn_leaf(const int32_t* __restrict__ il_
 __global__ void k_scale_rmax2(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i] * kRmaxSlack;
+}
+
+// out[i] = max of v over the segment [spl[i], spl[i+1]) (clamped to [0, nv)), `empty` if it is empty
+__global__ void k_segment_max(const int32_t* __restrict__ spl, const float* __restrict__ v, int64_t nv,
+                              float* __restrict__ out, int64_t nseg, float empty) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  const int64_t a = min(max((int64_t)spl[i], (int64_t)0), nv), b = min(max((int64_t)spl[i + 1], (int64_t)0), nv);
+  float m = empty;
+  for (int64_t j = a; j < b; j++) m = fmaxf(m, v[j]);
+  out[i] = m;
 }
 
 template <int D, typename T>
@@ -571,6 +599,14 @@ extern "C" int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_i
                                     node_ilist_r2, size_parents, size_nodes, parent_ilist_size, node_ilist_size, k,
                                     boxsize);
   return fail("KnnNode2Node: unsupported dim/dtype");
+}
+
+extern "C" int jzt_segment_max_f32(jzt_stream_t stream, const int32_t* spl, const float* vals, int64_t nvals,
+                                   float* out, int64_t nseg, float empty) {
+  if (nseg <= 0) return 0;
+  hipLaunchKernelGGL(k_segment_max, dim3((unsigned)div_up(nseg, 256)), dim3(256), 0, (hipStream_t)stream, spl, vals,
+                     nvals, out, nseg, empty);
+  return check_launch("SegmentMax");
 }
 
 extern "C" size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype) {

@@ -62,11 +62,82 @@ struct alignas(16) SrcRec {
   uint32_t id;
 };
 
-// Stream n staged source particles into the per-lane top-k.  `wrap` (wave-uniform) applies the
-// periodic wrap; the caller clears it only when no difference of the chunk needs it.
+// ---- buffered top-k ------------------------------------------------------------------------
+// A candidate that beats a lane's current k-th key is not inserted at once: it is appended to the
+// lane's row of an LDS buffer (kBufCap(K) entries).  When some lane's row is full (or the query is
+// done) the wave flushes: each lane sorts its row with a bitonic network and merges it into its
+// sorted top-k (half-cleaner against the reversed row, then a bitonic merge).  A flush costs
+// about as much as three single insertions but absorbs up to kBufCap of them per lane, and the
+// network no longer runs whenever any one lane accepts a candidate.  Rows are kBufStride u64 apart
+// (odd stride: lanes reading the same slot hit different banks).
+
+#ifndef JZT_TOPK_BUF
+#define JZT_TOPK_BUF 16
+#endif
+template <int K> constexpr int kBufCap = K < JZT_TOPK_BUF ? K : JZT_TOPK_BUF;
+constexpr int kBufStride = 17;
+constexpr int kBufWords = 64 * kBufStride;  // u64 per wave
+
+__device__ __forceinline__ void ce_asc(uint64_t& a, uint64_t& b) {
+  const bool sw = b < a;
+  const uint64_t lo = sw ? b : a, hi = sw ? a : b;
+  a = lo;
+  b = hi;
+}
+
+// ascending bitonic sort of a register array (N a power of two)
+template <int N>
+__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[N]) {
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const int j = i ^ stride;
+        if (j > i) {
+          if ((i & size) == 0) ce_asc(v[i], v[j]);
+          else ce_asc(v[j], v[i]);
+        }
+      }
+    }
+  }
+}
+
+// Merge the lane's buffered keys row[0..cnt) into the ascending best[0..K): best becomes the K
+// smallest of the union, ascending.  cnt is reset.
+template <int K>
+__device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* row, int& cnt) {
+  constexpr int M = kBufCap<K>;
+  uint64_t b[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) b[i] = i < cnt ? row[i] : kEmptyKey;
+  cnt = 0;
+  bitonic_sort_regs<M>(b);
+  // half-cleaner of best ++ reverse(b padded to K): the low half holds the K smallest, bitonic
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    const uint64_t x = best[K - M + i], y = b[M - 1 - i];
+    best[K - M + i] = y < x ? y : x;
+  }
+  // bitonic merge to ascending order
+#pragma unroll
+  for (int stride = K / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      const int j = i ^ stride;
+      if (j > i) ce_asc(best[i], best[j]);
+    }
+  }
+}
+
+// Stream n staged source particles into the lane's buffer (and the top-k on flushes).  `wrap`
+// (wave-uniform) applies the periodic wrap; the caller clears it only when no difference of the
+// chunk needs it.  row = this lane's buffer row, cnt = its fill.
 template <int K, bool PER, bool FLOOR, int D>
 __device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
-                                              int n, bool wrap, float boxsize, float half, uint64_t floor_key) {
+                                              int n, bool wrap, float boxsize, float half, uint64_t floor_key,
+                                              uint64_t* row, int& cnt) {
   for (int j = 0; j < n; j++) {
     const SrcRec c = s_rec[j];
     Vec<D, float> p;
@@ -76,7 +147,11 @@ __device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, 
     if (PER && wrap) r2 = dist2P<true, D, float>(q, p, boxsize, half);
     else r2 = dist2P<false, D, float>(q, p, boxsize, half);
     const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | c.id;
-    if (key < best[K - 1] && (!FLOOR || key > floor_key)) topk_insert<K>(best, key);
+    if (key < best[K - 1] && (!FLOOR || key > floor_key)) {
+      row[cnt] = key;
+      cnt++;
+    }
+    if (wave_any(cnt >= kBufCap<K>)) topk_flush<K>(best, row, cnt);
   }
 }
 

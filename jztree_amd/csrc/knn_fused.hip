@@ -86,7 +86,7 @@ template <int K, bool PER, int D>
 __device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, float>& q, const uint64_t* s_list,
                                              int n, SrcRec* s_rec, const int32_t* __restrict__ l2p,
                                              const float* __restrict__ x, const int32_t* __restrict__ src_ids,
-                                             float boxsize, float half) {
+                                             float boxsize, float half, uint64_t* row, int& cnt) {
   const int lane = threadIdx.x;
   int e = 0, off = 0;
   SrcRec cr;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void consume_list(uint64_t (&best)[K], const Vec<D, f
     if (lane < n_cur) s_rec[lane] = cr;
     __syncthreads();
     if (ne < n) load(ne, noff);
-    consume_chunk<K, PER, false, D>(best, q, s_rec, n_cur, w_cur, boxsize, half, 0ull);
+    consume_chunk<K, PER, false, D>(best, q, s_rec, n_cur, w_cur, boxsize, half, 0ull, row, cnt);
     __syncthreads();
     e = ne;
     off = noff;
@@ -143,6 +143,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     T boxsize) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_list[kListCap];
+  __shared__ uint64_t s_buf[kBufWords];
+  uint64_t* const row = s_buf + threadIdx.x * kBufStride;
   const int lane = threadIdx.x;
   const int lq = blockIdx.x;
   const int q0 = l2p[lq], q1 = l2p[lq + 1];
@@ -162,9 +164,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     uint64_t best[K];
 #pragma unroll
     for (int i = 0; i < K; i++) best[i] = empty;
+    int bcnt = 0;  // entries in this lane's top-k buffer row
     // Collect child leaves into s_list until the parent list is exhausted or the list is full,
     // then sort and consume them; a full list resumes collection at the child where it stopped.
-    // (One consume site keeps one copy of the insertion network in the kernel.)
+    // (One consume site keeps one copy of the flush network in the kernel.)
     int eb = e0, ii = 0, cbo = 0;  // resume point: list batch, entry within it, child offset
     bool more = e0 < e1;
     while (true) {
@@ -232,11 +235,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       }
       __syncthreads();
       wave_sort(s_list, cnt);
-      consume_list<K, PER, D>(best, q, s_list, cnt, s_rec, l2p, x, src_ids, boxsize, half);
+      consume_list<K, PER, D>(best, q, s_list, cnt, s_rec, l2p, x, src_ids, boxsize, half, row, bcnt);
       __syncthreads();
       if (!full) break;
     }
 
+    if (wave_any(bcnt > 0)) topk_flush<K>(best, row, bcnt);
     if (valid && (!only_retry || retry[iq] != 0)) {
       uint64_t kth_key = best[0];
 #pragma unroll

@@ -7,6 +7,7 @@ top-k (KnnLeaf2Leaf).  Neighbours come out ascending by (canonical FP32 r2, repo
 from __future__ import annotations
 
 import dataclasses
+import math
 from dataclasses import dataclass
 from typing import Any, Callable, Tuple
 
@@ -89,11 +90,12 @@ _knn_leaf2leaf.jit = _knn_leaf2leaf
 
 
 def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNum, k: int = 32,
-                         boxsize: float = 0., with_radii: bool = False, rmax2_in=None):
+                         boxsize: float = 0., with_radii: bool = False, rmax2_in=None, overflow: list | None = None):
     """knn.py:62-110 -> KnnNode2Node.
 
     rmax2_in (extension): search radius^2 per node instead of the guaranteed k-neighbour radius;
-    each list then holds every node within rmax2_in * (1 + 2^-16)."""
+    each list then holds every node within rmax2_in * (1 + 2^-16).  overflow (extension): instead of
+    raising on a too small list, append the device flag "list overflowed" to this list."""
     boxsize = 0. if boxsize is None else boxsize
     assert ilist.ispl.shape[0] == spl_parent.shape[0]
     assert ilist.isrc.shape == ilist.rad2.shape
@@ -111,9 +113,12 @@ def _knn_node2node_ilist(ilist: InteractionList, spl_parent, node_data: PosLvlNu
            L.ptr(spl_parent.contiguous()), L.ptr(poslvl), L.ptr(npart), L.ptr(r2in), L.ptr(radii), L.ptr(ispl),
            L.ptr(il), L.ptr(ilr2), spl_parent.shape[0] - 1, size, ilist.size(), ilist.size(), int(k),
            float(boxsize), node_data.pos.shape[-1], L.dtype_code(node_data.pos.dtype))
-    raise_if(ispl[-1] > il.numel(),
-             "The interaction list allocation is too small. (need: {n1}, have: {n2})\n"
-             "Hint: increase alloc_fac_ilist", n1=ispl[-1], n2=il.numel())
+    if overflow is not None:  # the caller checks (and handles) overflow itself
+        overflow.append(ispl[-1] > il.numel())
+    else:
+        raise_if(ispl[-1] > il.numel(),
+                 "The interaction list allocation is too small. (need: {n1}, have: {n2})\n"
+                 "Hint: increase alloc_fac_ilist", n1=ispl[-1], n2=il.numel())
     if with_radii:
         return InteractionList(ispl, il, rad2=ilr2), radii
     return InteractionList(ispl, il, rad2=ilr2)
@@ -192,6 +197,39 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     return rnn, inn, retry
 
 
+def _cell_diag2(lvl: torch.Tensor, dim: int) -> torch.Tensor:
+    """Squared diagonal of the Morton cell of each level (float64): per-dim sides 2^l_i with the
+    level split as in lvl_vec (math.cuh:524-541)."""
+    lv = lvl.to(torch.int64)
+    o = torch.div(lv + 2000 * dim, dim, rounding_mode="floor") - 2000
+    r = lv - o * dim
+    d2 = torch.zeros(lv.shape, dtype=torch.float64, device=lv.device)
+    for i in range(dim):
+        li = o + (r >= dim - i).to(torch.int64)
+        d2 = d2 + torch.pow(4.0, li.to(torch.float64))
+    return d2
+
+
+def _guaranteed_leaf_r2(th: TreeHierarchy, k: int) -> torch.Tensor:
+    """Per leaf slot, a radius^2 within which every query of the leaf has its k nearest (itself
+    included): the squared cell diagonal of the smallest enclosing node (the leaf or an ancestor)
+    that holds >= k particles (all of them lie in that cell, and the periodic wrap only shortens
+    differences); +inf when no node holds k particles."""
+    size = th.size()
+    dim = th.geom_cent.data.shape[-1]
+    dev = th.lvl.data.device
+    cap = torch.full((size,), math.inf, dtype=torch.float64, device=dev)
+    idx = torch.arange(size, device=dev)
+    for p in range(th.num_planes()):
+        npart = th.npart(p, size=size)
+        d2 = _cell_diag2(th.lvl.get(p, size), dim)
+        take = (npart[idx] >= k) & torch.isinf(cap)
+        cap = torch.where(take, d2[idx], cap)
+        if p + 1 < th.num_planes():
+            idx = _leaf_parent(th.ispl_n2n.get(p + 1, size + 1), size).long()[idx]
+    return cap
+
+
 def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor) -> list:
     """Search radius^2 for every plane: leaf_r2 for the leaves (negative = no search), and for a
     node the largest radius among its children.  A node's list then holds every node that a leaf
@@ -199,16 +237,18 @@ def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor) -> list:
     size = th.size()
     dev = leaf_r2.device
     ar = torch.arange(size, device=dev)
-    radii = [torch.where(ar < th.num(0), leaf_r2.float(), torch.full_like(leaf_r2, -1.0, dtype=torch.float32))]
+    neg = torch.full((size,), -1.0, dtype=torch.float32, device=dev)
+    radii = [torch.where(ar < th.num(0), leaf_r2.float(), neg).contiguous()]
     for p in range(1, th.num_planes()):
-        parent = _leaf_parent(th.ispl_n2n.get(p, size + 1), size).long()
-        child = torch.where(ar < th.num(p - 1), radii[-1], torch.full_like(radii[-1], -1.0))
-        r = torch.full((size,), -1.0, dtype=torch.float32, device=dev).scatter_reduce(0, parent, child, "amax")
-        radii.append(torch.where(ar < th.num(p), r, torch.full_like(r, -1.0)))
+        n2c = th.ispl_n2n.get(p, size + 1).to(torch.int32).contiguous()
+        r = torch.empty(size, dtype=torch.float32, device=dev)
+        L.call("jzt_segment_max_f32", L.stream(), L.ptr(n2c), L.ptr(radii[-1]), size, L.ptr(r), size, -1.0)
+        radii.append(torch.where(ar < th.num(p), r, neg).contiguous())
     return radii
 
 
-def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0., alloc_fac_ilist: float = 256.):
+def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0., alloc_fac_ilist: float = 256.,
+                     overflow: list | None = None):
     """Dual walk with given search radii on every plane (no guaranteed-radius search): from the dense
     top-plane list down to leaf lists holding every leaf within each leaf's radius."""
     nlevels = th.num_planes()
@@ -223,22 +263,26 @@ def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0.
         parent_spl = spl_n2n.get(level + 1, size + 1)
         node_data = PosLvlNum(pos=th.geom_cent.get(level, size), lvl=th.lvl.get(level, size),
                               npart=th.npart(level, size=size, ptype=0))
-        ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize, rmax2_in=radii[level])
+        ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize, rmax2_in=radii[level],
+                                     overflow=overflow)
     return ilist
 
 
 def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
                  alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None,
-                 max_rounds: int = 3, grow: float = 16.0):
+                 grow: float = 16.0, stats: dict | None = None):
     """Exact kNN from a proposed search radius^2 per leaf (extension; k <= 64).
 
     Round 1 walks the tree with the proposed radii on every plane (a node searches as far as its
-    farthest-reaching child) and runs the leaf kernel with each leaf's radius as starting bound.
-    Queries that find fewer than k neighbours inside it are flagged; their leaves are redone with
-    the radius^2 grown by `grow` while every other leaf sits out, up to max_rounds rounds.  Anything
-    still flagged then is recomputed by the unbounded fused retry pass on the guaranteed-radius
-    lists.  The result is therefore independent of the proposal: identical to knn() for any
-    leaf_r2.  Returns (rnn, inn, retry flags of round 1, rounds run)."""
+    farthest-reaching child; each radius clipped to the guaranteed one, _guaranteed_leaf_r2) and
+    runs the leaf kernel with each leaf's radius as starting bound.  Queries that find fewer than k
+    neighbours inside it are flagged; round 2 redoes their leaves with the radius^2 grown by `grow`
+    (still clipped) while every other leaf sits out, round 3 the leaves still short with the
+    guaranteed radius.  Whatever is left -- or everything, should a round's lists overflow their
+    allocation -- is recomputed without any proposal: by the fused retry pass on guaranteed-radius
+    lists, or by the plain walk when round 1 itself overflowed.
+    The result is therefore independent of the proposal: identical to knn() for any leaf_r2.
+    Returns (rnn, inn, retry flags of round 1)."""
     assert k <= 64, "a start bound needs k <= 64"
     dev = posz.device
     n = len(posz)
@@ -246,35 +290,64 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     boxsize = 0. if boxsize is None else boxsize
     rows = n if out_rows is None else out_rows
     spl = th.splits_leaf_to_part()
-    leafof = torch.clamp(inverse_of_splits(spl, n), 0, size - 1).long()
     rnn = torch.empty((rows, k), dtype=posz.dtype, device=dev)
     inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
     retry = torch.zeros(n, dtype=torch.int32, device=dev)
-    r2 = leaf_r2.to(torch.float32).contiguous()
-    first = None
-    rounds = 0
-    for rnd in range(max_rounds):
-        rounds += 1
-        il = _knn_radius_walk(th, k, _plane_radii(th, r2), boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist)
+    st = stats if stats is not None else {}
+    # never search beyond the guaranteed radius (a wild proposal would only inflate the lists); the
+    # 2^-10 widening keeps rounding of the canonical r2 from causing avoidable retries
+    cap = (_guaranteed_leaf_r2(th, k) * (1.0 + 2.0 ** -10)).to(torch.float32)
+    r2 = torch.minimum(leaf_r2.to(torch.float32), cap).contiguous()
+
+    def run(r2):
+        ovf = []
+        il = _knn_radius_walk(th, k, _plane_radii(th, r2), boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist,
+                              overflow=ovf)
         _knn_leaf2leaf(il, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids, dst_rows=dst_rows, out_rows=rows,
                        init_r2=r2, out=(rnn, inn, retry))
-        if first is None:
-            first = retry.clone()
-        leaf_short = torch.zeros(size, dtype=torch.int32, device=dev).scatter_reduce(0, leafof, retry, "amax")
-        if rnd == max_rounds - 1 or not bool(leaf_short.any()):
+        return torch.stack(ovf).any()
+
+    ovf1 = run(r2)
+    first = retry.clone()
+    # one read-back per round: (lists overflowed, queries left)
+    h = torch.stack([ovf1.to(torch.int64), first.sum().to(torch.int64)]).cpu()
+    st["short_round1"] = int(h[1])
+    if bool(h[0]):
+        st["fallback"] = "plain (round 1 lists overflowed)"
+        il = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist)
+        _knn_leaf2leaf(il, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids, dst_rows=dst_rows, out_rows=rows,
+                       out=(rnn, inn))
+        return rnn, inn, first
+    if int(h[1]) == 0:
+        return rnn, inn, first
+    # Later rounds: the leaves holding a flagged query (segment sums of the flags over the leaf
+    # splits) with the radius grown by `grow`, then with the guaranteed radius; each round's lists
+    # and flags are read back once.
+    sl = spl.long().clamp(0, n)
+    for rnd, r2_next in enumerate((lambda: torch.minimum(r2 * grow, cap), lambda: cap), start=2):
+        cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(retry, 0)])
+        short = (cs[sl[1:]] - cs[sl[:-1]]) > 0
+        ovf = run(torch.where(short, r2_next(), torch.full_like(r2, -1.0)).contiguous())
+        h = torch.stack([ovf.to(torch.int64), retry.sum().to(torch.int64), short.sum().to(torch.int64)]).cpu()
+        st[f"short_leaves_round{rnd}"] = int(h[2])
+        st[f"short_round{rnd}"] = int(h[1])
+        if bool(h[0]):  # this round's lists overflowed: every row it rewrote (all short leaves) is suspect
+            leafof = inverse_of_splits(spl, n).long().clamp_(0, size - 1)
+            retry.copy_(short[leafof].to(torch.int32))
             break
-        r2 = torch.where(leaf_short > 0, r2 * grow, torch.full_like(r2, -1.0)).contiguous()
-    if bool(retry.any()):
-        ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
-                                         return_parents=True)
-        leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
-        sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
-        drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
-        L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
-               ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
-               L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
-               int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
-    return rnn, inn, first, rounds
+        if int(h[1]) == 0:
+            return rnn, inn, first
+    st["fallback"] = "fused retry"
+    ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                                     return_parents=True)
+    leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
+    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+    L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
+           ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
+           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
+           int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
+    return rnn, inn, first
 
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,

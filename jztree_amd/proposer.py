@@ -247,8 +247,8 @@ def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = 
             sid, dst = originz.idx, originz.idx
         else:  # z order: rows and ids are z-indices
             sid, dst = None, None
-        rnn, inn, retry, _ = _knn_bounded(th, get_pos(partz), k, bound, boxsize=boxsize,
-                                          alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=sid, dst_rows=dst)
+        rnn, inn, retry = _knn_bounded(th, get_pos(partz), k, bound, boxsize=boxsize,
+                                       alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=sid, dst_rows=dst)
         checks.flush()
     except BaseException:
         checks.clear()

@@ -15,10 +15,13 @@ def cumsum_starting_with_zero(x: torch.Tensor) -> torch.Tensor:
 
 
 def inverse_of_splits(ispl: torch.Tensor, size: int) -> torch.Tensor:
-    """tools.py:122-125: given [0, 4, 7] returns [0,0,0,0,1,1,1] for size=7."""
-    mask = torch.zeros(size + 1, dtype=torch.int32, device=ispl.device)
-    mask.index_add_(0, torch.clamp(ispl.long(), 0, size), torch.ones_like(ispl, dtype=torch.int32))
-    return (torch.cumsum(mask[:size], 0) - 1).to(torch.int32)
+    """tools.py:122-125: given [0, 4, 7] returns [0,0,0,0,1,1,1] for size=7.
+
+    Element i belongs to the last segment j with ispl[j] <= i (a binary search per element; the
+    reference's scatter-add of ones onto the split positions is equivalent, but padded splits pile
+    up on one position, which serialises device atomics)."""
+    ar = torch.arange(size, dtype=ispl.dtype, device=ispl.device)
+    return (torch.searchsorted(ispl.contiguous(), ar, right=True) - 1).to(torch.int32)
 
 
 def inverse_indices(iargsort: torch.Tensor) -> torch.Tensor:

@@ -171,10 +171,10 @@ def test_bounded_matches_plain(dev, oracle, k, boxsize):
     mixed = exact.clone()
     mixed[::3] = 0.0
     for fac, bound in [(0.25, exact * 0.25), (1.0, exact * (1 + 1e-6)), (3.0, exact * 3.0), ("mixed", mixed)]:
-        r1, i1, retry, rounds = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize,
-                                                    alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx)
-        print(f"[bounded] k={k} box={boxsize} factor {fac}: {int(retry.sum())} short queries, {rounds} rounds",
-              flush=True)
+        st = {}
+        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize,
+                                            alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx, stats=st)
+        print(f"[bounded] k={k} box={boxsize} factor {fac}: {st}", flush=True)
         jz.checks.flush()
         torch.cuda.synchronize()
         assert torch.equal(i1, iz) and torch.equal(r1, rz), f"bound factor {fac}"
