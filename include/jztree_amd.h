@@ -106,6 +106,14 @@ int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl, const int3
                              int32_t* out_level, int min_lvl, int max_lvl, float max_volume_fac,
                              float min_percentile, void* tmp, size_t tmp_bytes);
 
+/* jzt_dense_ilist: tree.py:868-930 `grouped_dense_interaction_list` as one device op.  With
+ * nn = *nnodes top-plane nodes grouped into nsuper = ceil(nn/ngroup) super-nodes:
+ * spl_super[i] = min(i*ngroup, nn) and ispl[i] = min(i*nsuper, nsuper^2) for i <= size_super,
+ * ilist[j] = j % nsuper and r2[j] = 0 (r2 may be NULL) for j < min(nsuper^2, size_ilist);
+ * *nsuper (device) receives nsuper.  The caller checks nsuper^2 <= size_ilist. */
+int jzt_dense_ilist(jzt_stream_t stream, const int32_t* nnodes, int ngroup, int32_t* spl_super, int32_t* ispl,
+                    int64_t size_super, int32_t* ilist, float* r2, int64_t size_ilist, int32_t* nsuper);
+
 /* ------------------------------------------------------------------------------------------ */
 /* kNN                                                                                          */
 /* ------------------------------------------------------------------------------------------ */

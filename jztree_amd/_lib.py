@@ -33,6 +33,7 @@ _SIGNATURES = {
     "jzt_get_node_geometry": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32, _i32, _u32, _i32], _i32),
     "jzt_where_tmp_bytes": ([_i64], _sz),
     "jzt_where_flags": ([_p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _sz], _i32),
+    "jzt_dense_ilist": ([_p, _p, _i32, _p, _p, _i64, _p, _p, _i64, _p], _i32),
     "jzt_regularization_tmp_bytes": ([_i32, _i32], _sz),
     "jzt_regularization_level": ([_p, _p, _p, _i64, _p, _i32, _i32, _f32, _f32, _p, _sz], _i32),
     "jzt_knn_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i32, _f32, _i32,

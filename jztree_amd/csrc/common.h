@@ -243,5 +243,9 @@ __device__ __forceinline__ void st_rel(uint32_t* p, uint32_t v) {
 }
 
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+// Grid of a kernel whose blocks stride over `n` work items: at most kMaxGrid blocks, so launching
+// over a padded allocation costs no block dispatch per empty slot.
+constexpr int64_t kMaxGrid = 65536;
+inline unsigned grid_for(int64_t n) { return (unsigned)(n <= 0 ? 1 : (n < kMaxGrid ? n : kMaxGrid)); }
 
 }  // namespace jzt

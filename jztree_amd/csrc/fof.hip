@@ -85,16 +85,19 @@ template <int D, typename T> __device__ __forceinline__ T norm2(const Vec<D, T>&
 template <int D, typename T>
 __global__ __launch_bounds__(64) void k_node_to_child(const int32_t* __restrict__ pig, const uint8_t* __restrict__ local,
                                                       const int32_t* __restrict__ plvl, const int32_t* __restrict__ pspl,
-                                                      int32_t* __restrict__ cig, int64_t size_parent, float r2link) {
-  const int64_t node = blockIdx.x;
-  if (node >= size_parent || (local && !local[node])) return;
-  const int root = pig[node];
-  if (root >= size_parent || (local && !local[root])) return;
-  const T L2 = norm2<D, T>(lvl_ext<D, T>(plvl[node]));
-  const bool self = (root == node) && (L2 * (T)kSlackHi > (T)r2link);
-  const int c0 = pspl[node], c1 = pspl[node + 1];
-  const int lab = pspl[root];
-  for (int i = c0 + threadIdx.x; i < c1; i += 64) cig[i] = self ? i : lab;
+                                                      int32_t* __restrict__ cig, int64_t size_parent, float r2link, int64_t nitems) {
+  const int64_t nact_ = active_items(pspl, nitems);
+  for (int64_t node = blockIdx.x; node < nact_; node += gridDim.x) {
+    if (node >= size_parent || (local && !local[node])) continue;
+    const int root = pig[node];
+    if (root >= size_parent || (local && !local[root])) continue;
+    const T L2 = norm2<D, T>(lvl_ext<D, T>(plvl[node]));
+    const bool self = (root == node) && (L2 * (T)kSlackHi > (T)r2link);
+    const int c0 = pspl[node], c1 = pspl[node + 1];
+    const int lab = pspl[root];
+    for (int i = c0 + threadIdx.x; i < c1; i += 64) cig[i] = self ? i : lab;
+  
+  }
 }
 
 // PASS 0: link guaranteed pairs; PASS 1: count uncertain pairs; PASS 2: insert them
@@ -103,56 +106,59 @@ __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil
                                                  const int32_t* __restrict__ spl, const T* __restrict__ nodes,
                                                  const int32_t* __restrict__ il_spl, int32_t* ig, int32_t* __restrict__ count,
                                                  int32_t* __restrict__ il, float r2link, T boxsize, int64_t il_size,
-                                                 int64_t pil_size) {
+                                                 int64_t pil_size, int64_t nitems) {
   __shared__ NodeX<D, T> s_n[64];
   __shared__ int s_g[64];
   const int lane = threadIdx.x;
-  const int pq = blockIdx.x;
-  const int q0 = spl[pq], q1 = spl[pq + 1];
-  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
-  const T rl_hi = (T)r2link * (T)kSlackHi, rl_lo = (T)r2link * (T)kSlackLo;
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    const bool valid = qoff + lane < q1;
-    NodeX<D, T> nq = load_nodex<D, T>(nodes, iq);
-    const int gq = ig[iq];
-    int64_t off = (PASS == 2) ? (int64_t)il_spl[iq] : 0;
-    int cnt = 0;
-    for (int e = e0; e < e1; e++) {
-      const int pt = pil[e];
-      if (pt < pq) continue;  // each parent pair once
-      const int t0 = spl[pt], t1 = spl[pt + 1];
-      for (int toff = t0; toff < t1; toff += 64) {
-        const int it = toff + lane;
-        if (it < t1) {
-          s_n[lane] = load_nodex<D, T>(nodes, it);
-          s_g[lane] = ig[it];
-        }
-        __syncthreads();
-        const int nl = min(t1 - toff, 64);
-        if (valid) {
-          for (int j = 0; j < nl; j++) {
-            const int jt = toff + j;
-            if (gq == s_g[j] && iq != jt) continue;  // already linked
-            NodeX<D, T> nt = s_n[j];
-            Vec<D, T> w;
-#pragma unroll
-            for (int d = 0; d < D; d++) w[d] = nq.h[d] + nt.h[d];
-            T r2max = maxdist2<D, T>(nt.c, nq.c, w, boxsize);
-            T r2min = mindist2<D, T>(nt.c, nq.c, w, boxsize);
-            T L2 = norm2<D, T>(nt.h) * T(4);
-            if (max(r2max, L2) <= rl_lo) {
-              if (PASS == 0) uf_link(ig, iq, jt);
-            } else if (PASS >= 1 && r2min <= rl_hi) {
-              if (PASS == 2 && off + cnt < il_size) il[off + cnt] = jt;
-              cnt++;
+  const int64_t nact_ = active_items(spl, nitems);
+  for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
+    const int q0 = spl[pq], q1 = spl[pq + 1];
+    const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+    const T rl_hi = (T)r2link * (T)kSlackHi, rl_lo = (T)r2link * (T)kSlackLo;
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      const bool valid = qoff + lane < q1;
+      NodeX<D, T> nq = load_nodex<D, T>(nodes, iq);
+      const int gq = ig[iq];
+      int64_t off = (PASS == 2) ? (int64_t)il_spl[iq] : 0;
+      int cnt = 0;
+      for (int e = e0; e < e1; e++) {
+        const int pt = pil[e];
+        if (pt < pq) continue;  // each parent pair once
+        const int t0 = spl[pt], t1 = spl[pt + 1];
+        for (int toff = t0; toff < t1; toff += 64) {
+          const int it = toff + lane;
+          if (it < t1) {
+            s_n[lane] = load_nodex<D, T>(nodes, it);
+            s_g[lane] = ig[it];
+          }
+          __syncthreads();
+          const int nl = min(t1 - toff, 64);
+          if (valid) {
+            for (int j = 0; j < nl; j++) {
+              const int jt = toff + j;
+              if (gq == s_g[j] && iq != jt) continue;  // already linked
+              NodeX<D, T> nt = s_n[j];
+              Vec<D, T> w;
+  #pragma unroll
+              for (int d = 0; d < D; d++) w[d] = nq.h[d] + nt.h[d];
+              T r2max = maxdist2<D, T>(nt.c, nq.c, w, boxsize);
+              T r2min = mindist2<D, T>(nt.c, nq.c, w, boxsize);
+              T L2 = norm2<D, T>(nt.h) * T(4);
+              if (max(r2max, L2) <= rl_lo) {
+                if (PASS == 0) uf_link(ig, iq, jt);
+              } else if (PASS >= 1 && r2min <= rl_hi) {
+                if (PASS == 2 && off + cnt < il_size) il[off + cnt] = jt;
+                cnt++;
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
       }
+      if (PASS == 1 && valid) count[iq] = cnt;
     }
-    if (PASS == 1 && valid) count[iq] = cnt;
+  
   }
 }
 
@@ -185,74 +191,77 @@ template <typename T> __device__ __forceinline__ T wave_max_t(T v) {
 template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
-                                                 float r2link, T boxsize, int64_t il_size) {
+                                                 float r2link, T boxsize, int64_t il_size, int64_t nitems) {
   __shared__ FofRec<D, T> s_r[64];
   const int lane = threadIdx.x;
-  const int lq = blockIdx.x;
-  const int q0 = spl[lq], q1 = spl[lq + 1];
-  const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
-  const T half = (T)(float)(T(0.5) * boxsize);
-  const T r2skip = (T)(r2link * (1.f + kBoundSlack));
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    const bool valid = qoff + lane < q1;
-    const Vec<D, T> xq = load_vec<D, T>(pos, iq);
-    Vec<D, T> qlo, qhi;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      qlo[d] = wave_min_t<T>(xq[d]);
-      qhi[d] = wave_max_t<T>(xq[d]);
-    }
-    const int gq = ig[iq];
-    int glast = -1;
-    for (int e = e0; e < e1; e++) {
-      const int lt = il[e];
-      if (lt < lq) continue;  // each leaf pair once
-      const int t0 = spl[lt], t1 = spl[lt + 1];
-      for (int toff = t0; toff < t1; toff += 64) {
-        const int it = toff + lane;
-        FofRec<D, T> r;
-        bool need_wrap = false, reach = false;
-        if (it < t1) {
-          const Vec<D, T> p = load_vec<D, T>(pos, it);
-          T g2 = T(0);
-#pragma unroll
-          for (int d = 0; d < D; d++) {
-            r.p[d] = p[d];
-            const T dlo = qlo[d] - p[d], dhi = qhi[d] - p[d];
-            if (PER) need_wrap |= !(dlo >= -half) || !(dhi < half);
-            const T gap = max(T(0), max(dlo, -dhi));
-            g2 = fmaT(gap, gap, g2);
+  const int64_t nact_ = active_items(spl, nitems);
+  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+    const int q0 = spl[lq], q1 = spl[lq + 1];
+    const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
+    const T half = (T)(float)(T(0.5) * boxsize);
+    const T r2skip = (T)(r2link * (1.f + kBoundSlack));
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      const bool valid = qoff + lane < q1;
+      const Vec<D, T> xq = load_vec<D, T>(pos, iq);
+      Vec<D, T> qlo, qhi;
+  #pragma unroll
+      for (int d = 0; d < D; d++) {
+        qlo[d] = wave_min_t<T>(xq[d]);
+        qhi[d] = wave_max_t<T>(xq[d]);
+      }
+      const int gq = ig[iq];
+      int glast = -1;
+      for (int e = e0; e < e1; e++) {
+        const int lt = il[e];
+        if (lt < lq) continue;  // each leaf pair once
+        const int t0 = spl[lt], t1 = spl[lt + 1];
+        for (int toff = t0; toff < t1; toff += 64) {
+          const int it = toff + lane;
+          FofRec<D, T> r;
+          bool need_wrap = false, reach = false;
+          if (it < t1) {
+            const Vec<D, T> p = load_vec<D, T>(pos, it);
+            T g2 = T(0);
+  #pragma unroll
+            for (int d = 0; d < D; d++) {
+              r.p[d] = p[d];
+              const T dlo = qlo[d] - p[d], dhi = qhi[d] - p[d];
+              if (PER) need_wrap |= !(dlo >= -half) || !(dhi < half);
+              const T gap = max(T(0), max(dlo, -dhi));
+              g2 = fmaT(gap, gap, g2);
+            }
+            r.g = ig[it];
+            reach = need_wrap || g2 <= r2skip;
           }
-          r.g = ig[it];
-          reach = need_wrap || g2 <= r2skip;
-        }
-        uint64_t m = __ballot(reach);
-        if (m == 0ull) continue;
-        const bool wrap = PER && wave_any(need_wrap);
-        s_r[lane] = r;
-        __syncthreads();
-        if (valid) {
-          while (m) {
-            const int j = __builtin_ctzll(m);
-            m &= m - 1ull;
-            const int gj = s_r[j].g;
-            // labels are snapshots of set members; a label equal to q's own, or to one q was
-            // already joined with, belongs to q's set by now
-            if (gj == gq || gj == glast) continue;
-            Vec<D, T> xp;
-#pragma unroll
-            for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
-            const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
-            if (r2 <= (T)r2link) {
-              uf_link(ig, iq, toff + j);
-              glast = gj;
+          uint64_t m = __ballot(reach);
+          if (m == 0ull) continue;
+          const bool wrap = PER && wave_any(need_wrap);
+          s_r[lane] = r;
+          __syncthreads();
+          if (valid) {
+            while (m) {
+              const int j = __builtin_ctzll(m);
+              m &= m - 1ull;
+              const int gj = s_r[j].g;
+              // labels are snapshots of set members; a label equal to q's own, or to one q was
+              // already joined with, belongs to q's set by now
+              if (gj == gq || gj == glast) continue;
+              Vec<D, T> xp;
+  #pragma unroll
+              for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
+              const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
+              if (r2 <= (T)r2link) {
+                uf_link(ig, iq, toff + j);
+                glast = gj;
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
+  
   }
 }
 
@@ -261,8 +270,8 @@ int node2child_impl(hipStream_t s, const int32_t* pig, const uint8_t* local, con
                     int32_t* cig, int64_t size_parent, int64_t size_child, float r2link) {
   if (fill_u32(s, cig, 0u, size_child)) return 1;
   if (size_parent <= 0) return 0;
-  hipLaunchKernelGGL((k_node_to_child<D, T>), dim3((unsigned)size_parent), dim3(64), 0, s, pig, local, plvl, pspl, cig,
-                     size_parent, r2link);
+  hipLaunchKernelGGL((k_node_to_child<D, T>), dim3(grid_for(size_parent)), dim3(64), 0, s, pig, local, plvl, pspl, cig,
+                     size_parent, r2link, size_parent);
   return check_launch("NodeToChildLabel");
 }
 
@@ -276,20 +285,20 @@ int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, con
   if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_node * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return fail("FofNode2Node: copy failed");
   if (size_parent > 0) {
-    dim3 g((unsigned)size_parent), b(64);
+    dim3 g(grid_for(size_parent)), b(64);
     hipLaunchKernelGGL((k_fof_node<0, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
-                       (int32_t*)nullptr, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size);
+                       (int32_t*)nullptr, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size, size_parent);
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_node, 256)), dim3(256), 0, s, ig, size_node);
     hipLaunchKernelGGL((k_fof_node<1, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
-                       il_spl + 1, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size);
+                       il_spl + 1, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size, size_parent);
     if (check_launch("FofNode2Node link/count")) return 1;
   }
   if (scan_tmp_words(size_node) > il_size) return fail("FofNode2Node: interaction list too small for scan temp");
   if (scan_i32(s, il_spl + 1, il_spl + 1, size_node, true, il)) return 1;
   if (fill_u32(s, il, 0u, scan_tmp_words(size_node))) return 1;
   if (size_parent > 0) {
-    hipLaunchKernelGGL((k_fof_node<2, D, T>), dim3((unsigned)size_parent), dim3(64), 0, s, pil_spl, pil, pspl, nd,
-                       il_spl, ig, (int32_t*)nullptr, il, r2link, (T)boxsize, il_size, pil_size);
+    hipLaunchKernelGGL((k_fof_node<2, D, T>), dim3(grid_for(size_parent)), dim3(64), 0, s, pil_spl, pil, pspl, nd,
+                       il_spl, ig, (int32_t*)nullptr, il, r2link, (T)boxsize, il_size, pil_size, size_parent);
   }
   return check_launch("FofNode2Node insert");
 }
@@ -302,11 +311,11 @@ int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64
     return fail("FofLeaf2Leaf: copy failed");
   if (size_leaves > 0) {
     if (boxsize > 0.f)
-      hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, (T)boxsize, il_size);
+      hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, (T)boxsize, il_size, size_leaves);
     else
-      hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3((unsigned)size_leaves), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, (T)boxsize, il_size);
+      hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, (T)boxsize, il_size, size_leaves);
   }
   if (size_part > 0)
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);

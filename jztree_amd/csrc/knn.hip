@@ -117,44 +117,47 @@ template <int KB, int D, typename T>
 __global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src,
                                                   const float* __restrict__ pil_r2, const int32_t* __restrict__ parent_spl,
                                                   const T* __restrict__ nodes, const int32_t* __restrict__ npart,
-                                                  float* __restrict__ rmax_out, int k, T boxsize, int64_t pil_size) {
+                                                  float* __restrict__ rmax_out, int k, T boxsize, int64_t pil_size, int64_t nitems) {
   __shared__ Vec<D, T> s_x[64];
   __shared__ Vec<D, T> s_e[64];
   __shared__ int s_n[64];
   const int lane = threadIdx.x;
-  const int pq = blockIdx.x;
-  const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
-  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    NodeT<D, T> nq = load_node<D, T>(nodes, iq);
-    Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
-    RmaxBins<KB> bins(k);
-    float rmax2 = INFINITY;
-    for (int e = e0; e < e1; e++) {
-      const int pt = pil_src[e];
-      const float r2t = pil_r2[e];
-      if (!wave_any(r2t <= rmax2)) break;
-      const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
-      for (int toff = t0; toff < t1; toff += 64) {
-        const int it = toff + lane;
-        if (it < t1) {
-          NodeT<D, T> nt = load_node<D, T>(nodes, it);
-          s_x[lane] = nt.cent;
-          s_e[lane] = lvl_half_ext<D, T>(nt.level);
-          s_n[lane] = npart[it];
+  const int64_t nact_ = active_items(parent_spl, nitems);
+  for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
+    const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
+    const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      NodeT<D, T> nq = load_node<D, T>(nodes, iq);
+      Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
+      RmaxBins<KB> bins(k);
+      float rmax2 = INFINITY;
+      for (int e = e0; e < e1; e++) {
+        const int pt = pil_src[e];
+        const float r2t = pil_r2[e];
+        if (!wave_any(r2t <= rmax2)) break;
+        const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
+        for (int toff = t0; toff < t1; toff += 64) {
+          const int it = toff + lane;
+          if (it < t1) {
+            NodeT<D, T> nt = load_node<D, T>(nodes, it);
+            s_x[lane] = nt.cent;
+            s_e[lane] = lvl_half_ext<D, T>(nt.level);
+            s_n[lane] = npart[it];
+          }
+          __syncthreads();
+          const int nl = min(t1 - toff, 64);
+          for (int j = 0; j < nl; j++) {
+            float r2 = (float)maxdist2<D, T>(s_x[j], nq.cent, add<D, T>(eq, s_e[j]), boxsize);
+            bins.consider(r2, s_n[j]);
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        const int nl = min(t1 - toff, 64);
-        for (int j = 0; j < nl; j++) {
-          float r2 = (float)maxdist2<D, T>(s_x[j], nq.cent, add<D, T>(eq, s_e[j]), boxsize);
-          bins.consider(r2, s_n[j]);
-        }
-        __syncthreads();
+        rmax2 = bins.max_r2(k) * kRmaxSlack;
       }
-      rmax2 = bins.max_r2(k) * kRmaxSlack;
+      if (qoff + lane < q1) rmax_out[iq] = rmax2;
     }
-    if (qoff + lane < q1) rmax_out[iq] = rmax2;
+  
   }
 }
 
@@ -165,55 +168,58 @@ __global__ __launch_bounds__(64) void k_count_insert(const int32_t* __restrict__
                                                      const float* __restrict__ rmax2, int32_t* __restrict__ count_out,
                                                      const int32_t* __restrict__ out_spl, int32_t* __restrict__ out_src,
                                                      float* __restrict__ out_r2, int64_t nmax, T boxsize,
-                                                     int64_t pil_size) {
+                                                     int64_t pil_size, int64_t nitems) {
   __shared__ Vec<D, T> s_x[64];
   __shared__ Vec<D, T> s_e[64];
   const int lane = threadIdx.x;
-  const int pq = blockIdx.x;
-  const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
-  const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    const bool valid = qoff + lane < q1;
-    NodeT<D, T> nq = load_node<D, T>(nodes, iq);
-    Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
-    const float rq = rmax2[iq];
-    int64_t off = (PASS == 1) ? (int64_t)out_spl[iq] : 0;
-    int cnt = 0;
-    for (int e = e0; e < e1; e++) {
-      const int pt = pil_src[e];
-      const float r2t = pil_r2[e];
-      if (!wave_any(r2t <= rq)) break;
-      const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
-      for (int toff = t0; toff < t1; toff += 64) {
-        const int it = toff + lane;
-        if (it < t1) {
-          NodeT<D, T> nt = load_node<D, T>(nodes, it);
-          s_x[lane] = nt.cent;
-          s_e[lane] = lvl_half_ext<D, T>(nt.level);
-        }
-        __syncthreads();
-        const int nl = min(t1 - toff, 64);
-        for (int j = 0; j < nl; j++) {
-          Vec<D, T> ew = add<D, T>(eq, s_e[j]);
-          float r2 = (float)mindist2<D, T>(s_x[j], nq.cent, ew, boxsize);
-          if (valid && r2 <= rq) {
-            if (PASS == 1) {
-              int64_t o = off + cnt;
-              if (o < nmax) {
-                // touching nodes get a tiny positive radius so the node itself sorts first
-                if (r2 == 0.f) r2 = 1e-10f * (float)maxdist2<D, T>(s_x[j], nq.cent, ew, boxsize);
-                out_r2[o] = r2;
-                out_src[o] = toff + j;
-              }
-            }
-            cnt++;
+  const int64_t nact_ = active_items(parent_spl, nitems);
+  for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
+    const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
+    const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      const bool valid = qoff + lane < q1;
+      NodeT<D, T> nq = load_node<D, T>(nodes, iq);
+      Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
+      const float rq = rmax2[iq];
+      int64_t off = (PASS == 1) ? (int64_t)out_spl[iq] : 0;
+      int cnt = 0;
+      for (int e = e0; e < e1; e++) {
+        const int pt = pil_src[e];
+        const float r2t = pil_r2[e];
+        if (!wave_any(r2t <= rq)) break;
+        const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
+        for (int toff = t0; toff < t1; toff += 64) {
+          const int it = toff + lane;
+          if (it < t1) {
+            NodeT<D, T> nt = load_node<D, T>(nodes, it);
+            s_x[lane] = nt.cent;
+            s_e[lane] = lvl_half_ext<D, T>(nt.level);
           }
+          __syncthreads();
+          const int nl = min(t1 - toff, 64);
+          for (int j = 0; j < nl; j++) {
+            Vec<D, T> ew = add<D, T>(eq, s_e[j]);
+            float r2 = (float)mindist2<D, T>(s_x[j], nq.cent, ew, boxsize);
+            if (valid && r2 <= rq) {
+              if (PASS == 1) {
+                int64_t o = off + cnt;
+                if (o < nmax) {
+                  // touching nodes get a tiny positive radius so the node itself sorts first
+                  if (r2 == 0.f) r2 = 1e-10f * (float)maxdist2<D, T>(s_x[j], nq.cent, ew, boxsize);
+                  out_r2[o] = r2;
+                  out_src[o] = toff + j;
+                }
+              }
+              cnt++;
+            }
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
+      if (PASS == 0 && valid) count_out[iq] = cnt;
     }
-    if (PASS == 0 && valid) count_out[iq] = cnt;
+  
   }
 }
 
@@ -288,34 +294,126 @@ struct GlobalKV {
   }
 };
 
+// ---- wave-level segment sort ---------------------------------------------------------------
+// One wave per segment of up to 64*E entries, the keys in registers: entry i*64 + lane lives in
+// lane `lane`, register i.  Bitonic stages with stride >= 64 compare registers inside a lane,
+// smaller strides exchange with a lane shuffle; no barriers, no LDS.
+constexpr int kWaveSortMax = 1024;
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
+  const int lo = __shfl_xor((int)(uint32_t)x, m, 64), hi = __shfl_xor((int)(uint32_t)(x >> 32), m, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <int E>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&v)[E], int lane) {
+  constexpr int N = 64 * E;
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size / 2; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int rs = stride / 64;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+          const int j = i ^ rs;
+          if (j > i) {
+            const bool asc = ((i * 64 + lane) & size) == 0;
+            const uint64_t a = v[i], b = v[j];
+            const bool sw = asc ? (b < a) : (a < b);
+            v[i] = sw ? b : a;
+            v[j] = sw ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+          const uint64_t o = shfl_xor_u64(v[i], stride);
+          const bool asc = ((i * 64 + lane) & size) == 0;
+          const bool lower = (lane & stride) == 0;
+          const bool take_min = lower == asc;
+          const bool o_less = o < v[i];
+          v[i] = (take_min == o_less) ? o : v[i];
+        }
+      }
+    }
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void wave_sort_segment(float* key, int32_t* val, const float* key_in, const int32_t* val_in,
+                                                  int64_t b, int len, int lane) {
+  uint64_t v[E];
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int idx = i * 64 + lane;
+    v[i] = idx < len ? kv_pack(key_in[b + idx], val_in[b + idx]) : ~0ull;
+  }
+  wave_bitonic_sort<E>(v, lane);
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int idx = i * 64 + lane;
+    if (idx < len) {
+      key[b + idx] = kv_key(v[i]);
+      val[b + idx] = (int32_t)(uint32_t)v[i];
+    }
+  }
+}
+
+// Segments of up to kWaveSortMax entries, one wave each (in place allowed: a wave reads its whole
+// segment into registers before writing); longer ones are left to k_segment_sort.
+__global__ __launch_bounds__(64) void k_segment_sort_wave(const int32_t* __restrict__ spl, float* key, int32_t* val,
+                                                          const float* key_in, const int32_t* val_in, int64_t nmax, int64_t nitems) {
+  const int64_t nact_ = active_items(spl, nitems);
+  for (int seg = blockIdx.x; seg < nact_; seg += gridDim.x) {
+    const int lane = threadIdx.x;
+    int64_t b = spl[seg], e = spl[seg + 1];
+    if (e > nmax) e = nmax;
+    if (b >= e) continue;
+    const int64_t len = e - b;
+    if (len > kWaveSortMax) continue;
+    if (len <= 64) wave_sort_segment<1>(key, val, key_in, val_in, b, (int)len, lane);
+    else if (len <= 128) wave_sort_segment<2>(key, val, key_in, val_in, b, (int)len, lane);
+    else if (len <= 256) wave_sort_segment<4>(key, val, key_in, val_in, b, (int)len, lane);
+    else if (len <= 512) wave_sort_segment<8>(key, val, key_in, val_in, b, (int)len, lane);
+    else wave_sort_segment<16>(key, val, key_in, val_in, b, (int)len, lane);
+  
+  }
+}
+
 // One workgroup per segment.  Segments up to kSegLds entries are sorted in LDS; longer ones in
 // place in the output arrays (global memory, visible within the workgroup across barriers).
 __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __restrict__ spl, float* key,
                                                               int32_t* val, const float* key_in, const int32_t* val_in,
-                                                              int64_t nmax) {
+                                                              int64_t nmax, int skip_short, int64_t nitems) {
   __shared__ uint64_t s_a[kSegLds];
-  const int seg = blockIdx.x;
-  int64_t b = spl[seg], e = spl[seg + 1];
-  if (e > nmax) e = nmax;
-  if (b >= e) return;
-  const int len = (int)(e - b);
-  if (len <= kSegLds) {
-    for (int i = threadIdx.x; i < len; i += blockDim.x) s_a[i] = kv_pack(key_in[b + i], val_in[b + i]);
-    __syncthreads();
-    bitonic_sort_asc(LdsArr{s_a}, len);
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
-      key[b + i] = kv_key(s_a[i]);
-      val[b + i] = (int32_t)(uint32_t)s_a[i];
-    }
-  } else {
-    if (key != key_in) {
-      for (int i = threadIdx.x; i < len; i += blockDim.x) {
-        key[b + i] = key_in[b + i];
-        val[b + i] = val_in[b + i];
-      }
+  const int64_t nact_ = active_items(spl, nitems);
+  for (int seg = blockIdx.x; seg < nact_; seg += gridDim.x) {
+    int64_t b = spl[seg], e = spl[seg + 1];
+    if (e > nmax) e = nmax;
+    if (b >= e) continue;
+    const int len = (int)(e - b);
+    if (len <= kWaveSortMax && skip_short) continue;  // done by k_segment_sort_wave
+    if (len <= kSegLds) {
+      for (int i = threadIdx.x; i < len; i += blockDim.x) s_a[i] = kv_pack(key_in[b + i], val_in[b + i]);
       __syncthreads();
+      bitonic_sort_asc(LdsArr{s_a}, len);
+      for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        key[b + i] = kv_key(s_a[i]);
+        val[b + i] = (int32_t)(uint32_t)s_a[i];
+      }
+    } else {
+      if (key != key_in) {
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+          key[b + i] = key_in[b + i];
+          val[b + i] = val_in[b + i];
+        }
+        __syncthreads();
+      }
+      bitonic_sort_asc(GlobalKV{key + b, val + b}, len);
     }
-    bitonic_sort_asc(GlobalKV{key + b, val + b}, len);
+    __syncthreads();  // the next segment reuses s_a
+  
   }
 }
 
@@ -347,135 +445,151 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZ
                                                  const float* __restrict__ init_r2, int32_t* __restrict__ retry,
                                                  float* __restrict__ rad_out, int32_t* __restrict__ id_out,
                                                  uint64_t* __restrict__ resume, int koff, int ktot, float boxsize,
-                                                 int64_t il_size, int64_t out_rows) {
+                                                 int64_t il_size, int64_t out_rows, int64_t nitems) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_buf[kBufWords];
   const int lane = threadIdx.x;
-  const int lq = blockIdx.x;
-  uint64_t* const row = s_buf + lane * kBufStride;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
-  const int q0 = splQ[lq], q1 = splQ[lq + 1];
-  const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
-  const float half = 0.5f * boxsize;
-  // a negative (or NaN) start bound marks a leaf that this launch leaves untouched
-  const float r2_0 = init_r2 ? init_r2[lq] : 0.f;
-  if (init_r2 && !(r2_0 >= 0.f)) return;
-  const uint64_t start_key = init_r2 ? (((uint64_t)__float_as_uint(r2_0) << 32) | 0xffffffffull) : kEmptyKey;
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    const Vec<D, float> q = load_vec<D, float>(xQ, iq);
-    const uint64_t floor_key = FLOOR ? resume[iq] : 0ull;
-    Vec<D, float> qlo, qhi;  // bounding box of the query chunk
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      qlo[d] = wave_minf(q[d]);
-      qhi[d] = wave_maxf(q[d]);
-    }
-    uint64_t best[K];
-#pragma unroll
-    for (int i = 0; i < K; i++) best[i] = start_key;
-    int cnt = 0;  // entries in this lane's buffer row
-    bool done = false;
-    for (int eb = e0; eb < e1 && !done; eb += 64) {
-      const int ne = min(64, e1 - eb);
-      int m_t0 = 0, m_t1 = 0;
-      float m_r2 = INFINITY;
-      if (lane < ne) {
-        const int src = il_src[eb + lane];
-        m_r2 = il_r2[eb + lane];
-        m_t0 = splT[src];
-        m_t1 = splT[src + 1];
+  const int64_t nact_ = active_items(splQ, nitems);
+  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+    uint64_t* const row = s_buf + lane * kBufStride;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int q0 = splQ[lq], q1 = splQ[lq + 1];
+    const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
+    const float half = 0.5f * boxsize;
+    // a negative (or NaN) start bound marks a leaf that this launch leaves untouched
+    const float r2_0 = init_r2 ? init_r2[lq] : 0.f;
+    if (init_r2 && !(r2_0 >= 0.f)) continue;
+    const uint64_t start_key = init_r2 ? (((uint64_t)__float_as_uint(r2_0) << 32) | 0xffffffffull) : kEmptyKey;
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      const Vec<D, float> q = load_vec<D, float>(xQ, iq);
+      const uint64_t floor_key = FLOOR ? resume[iq] : 0ull;
+      Vec<D, float> qlo, qhi;  // bounding box of the query chunk
+  #pragma unroll
+      for (int d = 0; d < D; d++) {
+        qlo[d] = wave_minf(q[d]);
+        qhi[d] = wave_maxf(q[d]);
       }
-      // first chunk of this batch
-      SrcRec cr;
-      int cn;
-      {
-        const int t0 = rdlane(m_t0, 0), t1 = rdlane(m_t1, 0);
-        cn = min(64, t1 - t0);
-        if (lane < cn) {
-          const Vec<D, float> p = load_vec<D, float>(xT, t0 + lane);
-#pragma unroll
-          for (int d = 0; d < D; d++) cr.p[d] = p[d];
-          cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+      uint64_t best[K];
+  #pragma unroll
+      for (int i = 0; i < K; i++) best[i] = start_key;
+      int cnt = 0;  // entries in this lane's buffer row
+      bool done = false;
+      for (int eb = e0; eb < e1 && !done; eb += 64) {
+        const int ne = min(64, e1 - eb);
+        int m_t0 = 0, m_t1 = 0;
+        float m_r2 = INFINITY;
+        if (lane < ne) {
+          const int src = il_src[eb + lane];
+          m_r2 = il_r2[eb + lane];
+          m_t0 = splT[src];
+          m_t1 = splT[src + 1];
         }
-      }
-      int i = 0, off = 0;
-      while (i < ne) {
-        if (off == 0) {
-          const float r2t = rdlane(m_r2, i);
-          if (!wave_any(r2t <= key_r2(best[K - 1]) * kRmaxSlack)) {
-            done = true;
-            break;
-          }
-        }
-        const int len = rdlane(m_t1, i) - rdlane(m_t0, i);
-        int ni = i, noff = off + 64;
-        if (noff >= len) {
-          ni = i + 1;
-          noff = 0;
-        }
-        const int n_cur = cn;
-        // Source filter: p can enter some lane's top-k only if its gap to the query box is within
-        // the largest current k-th distance of the wave (or a periodic wrap is possible); the
-        // kept records are compacted into LDS.  Exact: by monotone rounding every canonical r2
-        // from the box is >= the gap^2 computed here with the same FMA chain.
-        const float kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
-        bool need = false, reach = false;
-        if (lane < n_cur) {
-          float g2 = 0.f;
-#pragma unroll
-          for (int d = 0; d < D; d++) {
-            const float dlo = qlo[d] - cr.p[d], dhi = qhi[d] - cr.p[d];
-            if (PER) need |= !(dlo >= -half) || !(dhi < half);
-            const float gap = fmaxf(0.f, fmaxf(dlo, -dhi));
-            g2 = fmaf(gap, gap, g2);
-          }
-          reach = need || g2 <= kmax;
-        }
-        const uint64_t keep = __ballot(reach);
-        const bool wrap = PER && wave_any(need);
-        const int n_keep = __popcll(keep);
-        if (reach) s_rec[__popcll(keep & lt_mask)] = cr;
-        __syncthreads();
-        if (ni < ne) {
-          const int t0 = rdlane(m_t0, ni) + noff, t1 = rdlane(m_t1, ni);
+        // first chunk of this batch
+        SrcRec cr;
+        int cn;
+        {
+          const int t0 = rdlane(m_t0, 0), t1 = rdlane(m_t1, 0);
           cn = min(64, t1 - t0);
           if (lane < cn) {
             const Vec<D, float> p = load_vec<D, float>(xT, t0 + lane);
-#pragma unroll
+  #pragma unroll
             for (int d = 0; d < D; d++) cr.p[d] = p[d];
             cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
           }
         }
-        consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_keep, wrap, boxsize, half, floor_key, row, cnt);
-        __syncthreads();
-        i = ni;
-        off = noff;
-      }
-    }
-    if (wave_any(cnt > 0)) topk_flush<K>(best, row, cnt);
-    if (qoff + lane < q1) {
-      const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
-      if (orow >= 0 && orow < out_rows) {
-        const int64_t row = orow * ktot + koff;
-#pragma unroll
-        for (int i = 0; i < K; i++) {
-          if (i < ktot - koff) {
-            rad_out[row + i] = sqrtf(key_r2(best[i]));
-            id_out[row + i] = (int32_t)(uint32_t)best[i];
+        int i = 0, off = 0;
+        while (i < ne) {
+          if (off == 0) {
+            const float r2t = rdlane(m_r2, i);
+            if (!wave_any(r2t <= key_r2(best[K - 1]) * kRmaxSlack)) {
+              done = true;
+              break;
+            }
           }
+          const int len = rdlane(m_t1, i) - rdlane(m_t0, i);
+          int ni = i, noff = off + 64;
+          if (noff >= len) {
+            ni = i + 1;
+            noff = 0;
+          }
+          const int n_cur = cn;
+          // Source filter: p can enter some lane's top-k only if its gap to the query box is within
+          // the largest current k-th distance of the wave (or a periodic wrap is possible); the
+          // kept records are compacted into LDS.  Exact: by monotone rounding every canonical r2
+          // from the box is >= the gap^2 computed here with the same FMA chain.
+          const float kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
+          bool need = false, reach = false;
+          if (lane < n_cur) {
+            float g2 = 0.f;
+  #pragma unroll
+            for (int d = 0; d < D; d++) {
+              const float dlo = qlo[d] - cr.p[d], dhi = qhi[d] - cr.p[d];
+              if (PER) need |= !(dlo >= -half) || !(dhi < half);
+              const float gap = fmaxf(0.f, fmaxf(dlo, -dhi));
+              g2 = fmaf(gap, gap, g2);
+            }
+            reach = need || g2 <= kmax;
+          }
+          const uint64_t keep = __ballot(reach);
+          const bool wrap = PER && wave_any(need);
+          const int n_keep = __popcll(keep);
+          if (reach) s_rec[__popcll(keep & lt_mask)] = cr;
+          __syncthreads();
+          if (ni < ne) {
+            const int t0 = rdlane(m_t0, ni) + noff, t1 = rdlane(m_t1, ni);
+            cn = min(64, t1 - t0);
+            if (lane < cn) {
+              const Vec<D, float> p = load_vec<D, float>(xT, t0 + lane);
+  #pragma unroll
+              for (int d = 0; d < D; d++) cr.p[d] = p[d];
+              cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
+            }
+          }
+          consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_keep, wrap, boxsize, half, floor_key, row, cnt);
+          __syncthreads();
+          i = ni;
+          off = noff;
         }
       }
-      if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
-      if (resume) {
-        const int last = min(K, ktot - koff) - 1;
-        uint64_t lk = best[0];
-#pragma unroll
-        for (int i = 0; i < K; i++) lk = (i == last) ? best[i] : lk;
-        resume[iq] = lk;
+      if (wave_any(cnt > 0)) topk_flush<K>(best, row, cnt);
+      if (qoff + lane < q1) {
+        const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
+        if (orow >= 0 && orow < out_rows) {
+          const int64_t row = orow * ktot + koff;
+  #pragma unroll
+          for (int i = 0; i < K; i++) {
+            if (i < ktot - koff) {
+              rad_out[row + i] = sqrtf(key_r2(best[i]));
+              id_out[row + i] = (int32_t)(uint32_t)best[i];
+            }
+          }
+        }
+        if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
+        if (resume) {
+          const int last = min(K, ktot - koff) - 1;
+          uint64_t lk = best[0];
+  #pragma unroll
+          for (int i = 0; i < K; i++) lk = (i == last) ? best[i] : lk;
+          resume[iq] = lk;
+        }
       }
     }
+  
   }
+}
+
+// Sort each segment by (key, val): segments up to kWaveSortMax entries one wave each in registers,
+// longer ones by a workgroup (LDS up to kSegLds entries, else in place in global memory).
+int segment_sort(hipStream_t s, const int32_t* spl, float* key, int32_t* val, const float* key_in,
+                 const int32_t* val_in, int64_t nseg, int64_t nmax, const char* what) {
+  if (nseg <= 0) return 0;
+  hipLaunchKernelGGL(k_segment_sort_wave, dim3(grid_for(nseg)), dim3(64), 0, s, spl, key, val, key_in, val_in, nmax, nseg);
+  if (check_launch(what)) return 1;
+  // the two kernels touch disjoint segments
+  hipLaunchKernelGGL(k_segment_sort, dim3(grid_for(nseg)), dim3(kSegThreads), 0, s, spl, key, val, key_in, val_in, nmax,
+                     1, nseg);
+  return check_launch(what);
 }
 
 __global__ void k_scale_rmax2(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
@@ -502,27 +616,25 @@ int node2node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src
   const T* nd = (const T*)nodes;
   if (fill_u32(s, il_spl, 0u, size_nodes + 1)) return 1;
   if (size_parents <= 0) return 0;
-  dim3 g((unsigned)size_parents), b(64);
+  dim3 g(grid_for(size_parents)), b(64);
   if (rmax2_in) {
     if (size_nodes > 0)
       hipLaunchKernelGGL(k_scale_rmax2, dim3((unsigned)div_up(size_nodes, 256)), dim3(256), 0, s, rmax2_in, rmax2,
                          size_nodes);
-  } else if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
-  else if (k <= 32) hipLaunchKernelGGL((k_find_rmax<16, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
-  else hipLaunchKernelGGL((k_find_rmax<32, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size);
+  } else if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
+  else if (k <= 32) hipLaunchKernelGGL((k_find_rmax<16, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
+  else hipLaunchKernelGGL((k_find_rmax<32, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
   if (check_launch("KnnNode2Node FindRmax")) return 1;
   hipLaunchKernelGGL((k_count_insert<0, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, rmax2, il_spl + 1,
-                     (const int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr, il_size, (T)boxsize, pil_size);
+                     (const int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr, il_size, (T)boxsize, pil_size, size_parents);
   // inclusive scan of the counts in place; the scan temp lives in the (not yet written) list
   if (scan_tmp_words(size_nodes) > il_size) return fail("KnnNode2Node: interaction list too small for scan temp");
   if (scan_i32(s, il_spl + 1, il_spl + 1, size_nodes, true, il_src)) return 1;
   hipLaunchKernelGGL((k_count_insert<1, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, rmax2,
-                     (int32_t*)nullptr, il_spl, il_src, il_r2, il_size, (T)boxsize, pil_size);
+                     (int32_t*)nullptr, il_spl, il_src, il_r2, il_size, (T)boxsize, pil_size, size_parents);
   if (check_launch("KnnNode2Node insert")) return 1;
   // sort each node's list by (r2, src) in place
-  hipLaunchKernelGGL(k_segment_sort, dim3((unsigned)size_nodes), dim3(kSegThreads), 0, s, il_spl, il_r2, il_src,
-                     il_r2, il_src, il_size);
-  return check_launch("KnnNode2Node sort");
+  return segment_sort(s, il_spl, il_r2, il_src, il_r2, il_src, size_nodes, il_size, "KnnNode2Node sort");
 }
 
 template <int D>
@@ -533,7 +645,7 @@ int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, 
   if (nleaves_q <= 0) return 0;
   if (init_r2 && k > 64) return fail("KnnLeaf2Leaf: a start bound needs k <= 64");
   uint64_t* resume = k > 64 ? (uint64_t*)tmp : nullptr;
-  dim3 g((unsigned)nleaves_q), b(64);
+  dim3 g(grid_for(nleaves_q)), b(64);
   const float* xt = (const float*)xT;
   const float* xq = (const float*)xQ;
   for (int koff = 0; koff < k; koff += 64) {
@@ -541,7 +653,7 @@ int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, 
     const bool floor = resume && koff > 0;
 #define JZT_KNN_LAUNCH(KK, PER, FLOOR)                                                                              \
   hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
-                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows)
+                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q)
 #define JZT_KNN_LAUNCH_K(KK)                                      \
   do {                                                            \
     if (boxsize > 0.f) {                                          \
@@ -650,7 +762,5 @@ extern "C" int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const f
                                 float* key_out, int32_t* val_out, int64_t size_segs, int64_t size_keys) {
   hipStream_t s = (hipStream_t)stream;
   if (size_segs <= 0) return 0;
-  hipLaunchKernelGGL(k_segment_sort, dim3((unsigned)size_segs), dim3(kSegThreads), 0, s, spl, key_out, val_out, key, val,
-                     size_keys);
-  return check_launch("SegmentSort");
+  return segment_sort(s, spl, key_out, val_out, key, val, size_segs, size_keys, "SegmentSort");
 }

@@ -140,124 +140,127 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     const T* __restrict__ leaf_nodes, const int32_t* __restrict__ l2p, const T* __restrict__ x,
     const int32_t* __restrict__ src_ids, const int32_t* __restrict__ dst_rows, const float* __restrict__ init_r2,
     int32_t* __restrict__ retry, int only_retry, T* __restrict__ rad_out, int32_t* __restrict__ id_out, int ktot,
-    T boxsize) {
+    T boxsize, int64_t nitems) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_list[kListCap];
   __shared__ uint64_t s_buf[kBufWords];
   uint64_t* const row = s_buf + threadIdx.x * kBufStride;
   const int lane = threadIdx.x;
-  const int lq = blockIdx.x;
-  const int q0 = l2p[lq], q1 = l2p[lq + 1];
-  if (q1 <= q0) return;
-  const int par = leaf_parent[lq];
-  const int e0 = pil_spl[par], e1 = (int)min((int64_t)pil_spl[par + 1], pil_size);
-  const T half = (T)(float)(T(0.5) * boxsize);
-  const float r2_init = (init_r2 && !only_retry) ? init_r2[lq] : INFINITY;
-  const uint64_t empty = ((uint64_t)__float_as_uint(r2_init) << 32) | 0xffffffffull;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const int64_t nact_ = active_items(l2p, nitems);
+  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+    const int q0 = l2p[lq], q1 = l2p[lq + 1];
+    if (q1 <= q0) continue;
+    const int par = leaf_parent[lq];
+    const int e0 = pil_spl[par], e1 = (int)min((int64_t)pil_spl[par + 1], pil_size);
+    const T half = (T)(float)(T(0.5) * boxsize);
+    const float r2_init = (init_r2 && !only_retry) ? init_r2[lq] : INFINITY;
+    const uint64_t empty = ((uint64_t)__float_as_uint(r2_init) << 32) | 0xffffffffull;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
 
-  for (int qoff = q0; qoff < q1; qoff += 64) {
-    const int iq = min(qoff + lane, q1 - 1);
-    const bool valid = qoff + lane < q1;
-    if (only_retry && !wave_any(valid && retry[iq] != 0)) continue;
-    const Vec<D, T> q = load_vec<D, T>(x, iq);
-    uint64_t best[K];
-#pragma unroll
-    for (int i = 0; i < K; i++) best[i] = empty;
-    int bcnt = 0;  // entries in this lane's top-k buffer row
-    // Collect child leaves into s_list until the parent list is exhausted or the list is full,
-    // then sort and consume them; a full list resumes collection at the child where it stopped.
-    // (One consume site keeps one copy of the flush network in the kernel.)
-    int eb = e0, ii = 0, cbo = 0;  // resume point: list batch, entry within it, child offset
-    bool more = e0 < e1;
-    while (true) {
-      int cnt = 0;  // entries collected in s_list (wave-uniform)
-      bool full = false;
-      while (more && !full) {
-        const Cell<D, T> cq = load_cell<D, T>(leaf_nodes, lq);  // (re)loaded here: not live across consume
-        const int ne = min(64, e1 - eb);
-        int m_c0 = 0, m_c1 = 0;
-        float m_r2 = INFINITY;
-        if (lane < ne) {
-          const int src = pil_src[eb + lane];
-          m_r2 = pil_r2[eb + lane];
-          m_c0 = n2c[src];
-          m_c1 = n2c[src + 1];
-        }
-        for (; ii < ne; ii++) {
-          // the collection bound is the current k-th radius of the wave (the starting bound at first)
-          const float kmax = wave_maxf(key_r2(best[K - 1])) * kSlack;
-          if (!(rdlane(m_r2, ii) <= kmax)) {
-            more = false;
-            break;
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int iq = min(qoff + lane, q1 - 1);
+      const bool valid = qoff + lane < q1;
+      if (only_retry && !wave_any(valid && retry[iq] != 0)) continue;
+      const Vec<D, T> q = load_vec<D, T>(x, iq);
+      uint64_t best[K];
+  #pragma unroll
+      for (int i = 0; i < K; i++) best[i] = empty;
+      int bcnt = 0;  // entries in this lane's top-k buffer row
+      // Collect child leaves into s_list until the parent list is exhausted or the list is full,
+      // then sort and consume them; a full list resumes collection at the child where it stopped.
+      // (One consume site keeps one copy of the flush network in the kernel.)
+      int eb = e0, ii = 0, cbo = 0;  // resume point: list batch, entry within it, child offset
+      bool more = e0 < e1;
+      while (true) {
+        int cnt = 0;  // entries collected in s_list (wave-uniform)
+        bool full = false;
+        while (more && !full) {
+          const Cell<D, T> cq = load_cell<D, T>(leaf_nodes, lq);  // (re)loaded here: not live across consume
+          const int ne = min(64, e1 - eb);
+          int m_c0 = 0, m_c1 = 0;
+          float m_r2 = INFINITY;
+          if (lane < ne) {
+            const int src = pil_src[eb + lane];
+            m_r2 = pil_r2[eb + lane];
+            m_c0 = n2c[src];
+            m_c1 = n2c[src + 1];
           }
-          const int c0 = rdlane(m_c0, ii), c1 = rdlane(m_c1, ii);
-          for (int cb = c0 + cbo; cb < c1; cb += 64) {
-            if (cnt + 64 > kListCap) {
-              full = true;
-              cbo = cb - c0;
+          for (; ii < ne; ii++) {
+            // the collection bound is the current k-th radius of the wave (the starting bound at first)
+            const float kmax = wave_maxf(key_r2(best[K - 1])) * kSlack;
+            if (!(rdlane(m_r2, ii) <= kmax)) {
+              more = false;
               break;
             }
-            const int nc = min(64, c1 - cb);
-            bool keep = false;
-            uint64_t item = 0;
-            if (lane < nc) {
-              const int lt = cb + lane;
-              const Cell<D, T> ct = load_cell<D, T>(leaf_nodes, lt);
-              Vec<D, T> w;
-#pragma unroll
-              for (int d = 0; d < D; d++) w[d] = cq.h[d] + ct.h[d];
-              const float md = (float)mindist2<D, T>(ct.c, cq.c, w, boxsize);
-              keep = (md <= kmax) && (l2p[lt + 1] > l2p[lt]);
-              int wrap = 0;
-              if (PER) {
-#pragma unroll
-                for (int d = 0; d < D; d++) {
-                  const T lo = (cq.c[d] - cq.h[d]) - (ct.c[d] + ct.h[d]);
-                  const T hi = (cq.c[d] + cq.h[d]) - (ct.c[d] - ct.h[d]);
-                  wrap |= !(lo >= -half && hi < half) ? 1 : 0;
-                }
+            const int c0 = rdlane(m_c0, ii), c1 = rdlane(m_c1, ii);
+            for (int cb = c0 + cbo; cb < c1; cb += 64) {
+              if (cnt + 64 > kListCap) {
+                full = true;
+                cbo = cb - c0;
+                break;
               }
-              item = ((uint64_t)__float_as_uint(md) << 32) | ((uint64_t)wrap << 31) | (uint32_t)lt;
+              const int nc = min(64, c1 - cb);
+              bool keep = false;
+              uint64_t item = 0;
+              if (lane < nc) {
+                const int lt = cb + lane;
+                const Cell<D, T> ct = load_cell<D, T>(leaf_nodes, lt);
+                Vec<D, T> w;
+  #pragma unroll
+                for (int d = 0; d < D; d++) w[d] = cq.h[d] + ct.h[d];
+                const float md = (float)mindist2<D, T>(ct.c, cq.c, w, boxsize);
+                keep = (md <= kmax) && (l2p[lt + 1] > l2p[lt]);
+                int wrap = 0;
+                if (PER) {
+  #pragma unroll
+                  for (int d = 0; d < D; d++) {
+                    const T lo = (cq.c[d] - cq.h[d]) - (ct.c[d] + ct.h[d]);
+                    const T hi = (cq.c[d] + cq.h[d]) - (ct.c[d] - ct.h[d]);
+                    wrap |= !(lo >= -half && hi < half) ? 1 : 0;
+                  }
+                }
+                item = ((uint64_t)__float_as_uint(md) << 32) | ((uint64_t)wrap << 31) | (uint32_t)lt;
+              }
+              const uint64_t mask = __ballot(keep);
+              if (keep) s_list[cnt + __popcll(mask & lt_mask)] = item;
+              cnt += __popcll(mask);
             }
-            const uint64_t mask = __ballot(keep);
-            if (keep) s_list[cnt + __popcll(mask & lt_mask)] = item;
-            cnt += __popcll(mask);
+            if (full) break;
+            cbo = 0;
           }
-          if (full) break;
-          cbo = 0;
+          if (!full && more) {
+            eb += 64;
+            ii = 0;
+            more = eb < e1;
+          }
         }
-        if (!full && more) {
-          eb += 64;
-          ii = 0;
-          more = eb < e1;
-        }
+        __syncthreads();
+        wave_sort(s_list, cnt);
+        consume_list<K, PER, D>(best, q, s_list, cnt, s_rec, l2p, x, src_ids, boxsize, half, row, bcnt);
+        __syncthreads();
+        if (!full) break;
       }
-      __syncthreads();
-      wave_sort(s_list, cnt);
-      consume_list<K, PER, D>(best, q, s_list, cnt, s_rec, l2p, x, src_ids, boxsize, half, row, bcnt);
-      __syncthreads();
-      if (!full) break;
-    }
 
-    if (wave_any(bcnt > 0)) topk_flush<K>(best, row, bcnt);
-    if (valid && (!only_retry || retry[iq] != 0)) {
-      uint64_t kth_key = best[0];
-#pragma unroll
-      for (int i = 0; i < K; i++) kth_key = (i == ktot - 1) ? best[i] : kth_key;
-      const bool short_list = kth_key == empty && r2_init != INFINITY;
-      if (retry && !only_retry) retry[iq] = short_list ? 1 : 0;
-      if (!short_list) {
-        const int64_t row = (int64_t)(dst_rows ? dst_rows[iq] : iq) * ktot;
-#pragma unroll
-        for (int i = 0; i < K; i++) {
-          if (i < ktot) {
-            rad_out[row + i] = (T)sqrtf(key_r2(best[i]));
-            id_out[row + i] = (int32_t)(uint32_t)best[i];
+      if (wave_any(bcnt > 0)) topk_flush<K>(best, row, bcnt);
+      if (valid && (!only_retry || retry[iq] != 0)) {
+        uint64_t kth_key = best[0];
+  #pragma unroll
+        for (int i = 0; i < K; i++) kth_key = (i == ktot - 1) ? best[i] : kth_key;
+        const bool short_list = kth_key == empty && r2_init != INFINITY;
+        if (retry && !only_retry) retry[iq] = short_list ? 1 : 0;
+        if (!short_list) {
+          const int64_t row = (int64_t)(dst_rows ? dst_rows[iq] : iq) * ktot;
+  #pragma unroll
+          for (int i = 0; i < K; i++) {
+            if (i < ktot) {
+              rad_out[row + i] = (T)sqrtf(key_r2(best[i]));
+              id_out[row + i] = (int32_t)(uint32_t)best[i];
+            }
           }
         }
       }
     }
+  
   }
 }
 
@@ -270,14 +273,14 @@ int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, co
                void* rad, int32_t* ids, int64_t nleaves, int64_t npart, int k, float boxsize, int passes) {
   if (nleaves <= 0) return 0;
   if (init_r2 && !retry) return fail("KnnFused: a bound needs the retry buffer");
-  dim3 g((unsigned)nleaves), b(64);
+  dim3 g(grid_for(nleaves)), b(64);
   for (int pass = 0; pass < 2; pass++) {
     if (!(passes & (1 << pass))) continue;
     const int only = pass;
 #define JZT_FUSED(KK, PP)                                                                                        \
   hipLaunchKernelGGL((k_knn_fused<KK, PP, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, pil_size, n2c, leaf_parent, \
                      (const T*)leaf_nodes, l2p, (const T*)x, src_ids, dst_rows, init_r2, retry, only, (T*)rad, ids, k,    \
-                     (T)boxsize)
+                     (T)boxsize, nleaves)
 #define JZT_FUSED_K(PP)              \
   do {                               \
     if (k <= 8) JZT_FUSED(8, PP);    \

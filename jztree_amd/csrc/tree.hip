@@ -207,14 +207,23 @@ int geometry_impl(hipStream_t s, const void* pos, const int32_t* lbound, const i
 // ------------------------------------------------------------------------------------------------
 // regularization level (tree.py:55-68 _level_histogram, :160-187 find_regularization_level)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_reg_hist(const int32_t* __restrict__ lvl, const int32_t* __restrict__ w, int64_t n, int min_lvl,
-                           int nlvl, unsigned long long* __restrict__ hist) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int b = lvl[i] - min_lvl;
-  b = b < 0 ? 0 : b;  // jnp.bincount clips below and drops above
-  int64_t wi = w ? (int64_t)w[i] : 1;
-  if (b < nlvl && wi != 0) atomicAdd(&hist[b], (unsigned long long)wi);
+// Block-private histogram in LDS (levels cluster on a few bins: global atomics alone would
+// serialise on them), merged with one global atomic per nonzero bin and block.
+__global__ __launch_bounds__(256) void k_reg_hist(const int32_t* __restrict__ lvl, const int32_t* __restrict__ w,
+                                                  int64_t n, int min_lvl, int nlvl,
+                                                  unsigned long long* __restrict__ hist) {
+  extern __shared__ unsigned long long s_hist[];
+  for (int j = threadIdx.x; j < nlvl; j += blockDim.x) s_hist[j] = 0ull;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int b = lvl[i] - min_lvl;
+    b = b < 0 ? 0 : b;  // jnp.bincount clips below and drops above
+    const int64_t wi = w ? (int64_t)w[i] : 1;
+    if (b < nlvl && wi != 0) atomicAdd(&s_hist[b], (unsigned long long)wi);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nlvl; j += blockDim.x)
+    if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
 }
 
 // FP32 evaluation in ascending level order by one thread (the reference reduces under XLA's
@@ -271,9 +280,11 @@ extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl,
   hipStream_t s = (hipStream_t)stream;
   const int nlvl = max_lvl - min_lvl + 1;
   if (tmp_bytes < jzt_regularization_tmp_bytes(min_lvl, max_lvl)) return fail("regularization: tmp too small");
+  if ((size_t)nlvl * sizeof(unsigned long long) > 65536) return fail("regularization: level range too large");
   if (fill_u32(s, tmp, 0u, (int64_t)nlvl * 2)) return 1;
   if (n > 0)
-    hipLaunchKernelGGL(k_reg_hist, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, lvl, weights, n, min_lvl, nlvl,
+    hipLaunchKernelGGL(k_reg_hist, dim3((unsigned)min(div_up(n, 256), (int64_t)512)), dim3(256),
+                       (size_t)nlvl * sizeof(unsigned long long), s, lvl, weights, n, min_lvl, nlvl,
                        (unsigned long long*)tmp);
   // 0.01*percentile is formed in double and rounded to float, as the reference's weak typing does
   const float frac = (float)(0.01 * (double)min_percentile);

@@ -58,4 +58,20 @@ __device__ __forceinline__ int block_excl_scan(int v, int* sbuf, int* total) {
   return off + inc - v;
 }
 
+// Grid-stride kernels over the segments of a split array spl[0..n] (non-decreasing, as every
+// split array of the tree and its lists is, with padded slots repeating the final value): the
+// segments from the first index where spl reaches its final value on are empty, so the loop stops
+// there instead of striding through a padded allocation.  Found by a binary search at kernel start
+// (the same value in every lane).
+__device__ __forceinline__ int64_t active_items(const int32_t* __restrict__ spl, int64_t n) {
+  const int32_t last = spl[n];
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (spl[mid] < last) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace jzt

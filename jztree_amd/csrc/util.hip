@@ -131,7 +131,41 @@ int where_flags_i8(hipStream_t s, const int8_t* flags, int64_t n, int32_t* out, 
   return check_launch("where_flags_i8");
 }
 
+// tree.py:868-930 grouped_dense_interaction_list as one device op (the node count lives on the
+// device): super-node splits spl_super[i] = min(i*ngroup, nn) and list splits
+// ispl[i] = min(i*nsuper, nint) for i <= size_super, entries ilist[j] = j % nsuper (and r2[j] = 0)
+// for j < min(nint, size_ilist); entries past the list are not touched (readers stop at ispl).
+__global__ void k_dense_ilist(const int32_t* __restrict__ nn_dev, int ngroup, int32_t* __restrict__ spl_super,
+                              int32_t* __restrict__ ispl, int64_t size_super, int32_t* __restrict__ ilist,
+                              float* __restrict__ r2, int64_t size_ilist, int32_t* __restrict__ nsuper_out) {
+  const int64_t nn = nn_dev[0] > 0 ? nn_dev[0] : 0;
+  const int64_t nsuper = (nn + ngroup - 1) / ngroup;
+  const int64_t nint = nsuper * nsuper;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i <= size_super; i += st) {
+    spl_super[i] = (int32_t)min(i * ngroup, nn);
+    ispl[i] = (int32_t)min(min(i * nsuper, nint), (int64_t)INT32_MAX);
+  }
+  const int64_t nw = min(nint, size_ilist);
+  for (int64_t j = tid; j < nw; j += st) {
+    ilist[j] = (int32_t)(j % nsuper);
+    if (r2) r2[j] = 0.f;
+  }
+  if (tid == 0) nsuper_out[0] = (int32_t)nsuper;
+}
+
 }  // namespace jzt
 
 extern "C" const char* jzt_last_error(void) { return jzt::g_err.c_str(); }
 extern "C" int jzt_abi_version(void) { return JZT_ABI_VERSION; }
+
+extern "C" int jzt_dense_ilist(jzt_stream_t stream, const int32_t* nnodes, int ngroup, int32_t* spl_super,
+                               int32_t* ispl, int64_t size_super, int32_t* ilist, float* r2, int64_t size_ilist,
+                               int32_t* nsuper) {
+  using namespace jzt;
+  if (ngroup < 1) return fail("DenseIlist: ngroup must be >= 1");
+  const int64_t work = size_super + 1 > size_ilist ? size_super + 1 : size_ilist;
+  hipLaunchKernelGGL(k_dense_ilist, dim3(grid_for(div_up(work, 256))), dim3(256), 0, (hipStream_t)stream, nnodes,
+                     ngroup, spl_super, ispl, size_super, ilist, r2, size_ilist, nsuper);
+  return check_launch("DenseIlist");
+}

@@ -134,8 +134,7 @@ def _knn_dual_walk(th: TreeHierarchy, k: int, boxsize: float | None = None,
     nlevels = th.num_planes()
     size = th.size()
     spl, ilist, nsup = grouped_dense_interaction_list(
-        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size)
-    ilist.rad2 = torch.zeros(ilist.isrc.shape, dtype=torch.float32, device=ilist.isrc.device)
+        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size, with_r2=True)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
     rmax2 = None
@@ -254,8 +253,7 @@ def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0.
     nlevels = th.num_planes()
     size = th.size()
     spl, ilist, nsup = grouped_dense_interaction_list(
-        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size)
-    ilist.rad2 = torch.zeros(ilist.isrc.shape, dtype=torch.float32, device=ilist.isrc.device)
+        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size, with_r2=True)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
     for i in range(nlevels):

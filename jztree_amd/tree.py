@@ -381,28 +381,30 @@ zsort_and_tree.jit = zsort_and_tree
 # ------------------------------------------------------------------------------------------------
 
 def grouped_dense_interaction_list(nnodes, size_ilist: int, ngroup: int = 32, size_super: int | None = None,
-                                   node_range=None, dtype=torch.int32):
-    """All-to-all interaction list over super-nodes of `ngroup` top-level nodes (tree.py:868-930).
+                                   node_range=None, dtype=torch.int32, with_r2: bool = False):
+    """All-to-all interaction list over super-nodes of `ngroup` top-level nodes (tree.py:868-930),
+    built on the device (jzt_dense_ilist); only the nsuper^2 live entries are written.
 
-    Returns (spl_super, ilist, nsuper_nodes)."""
+    Returns (spl_super, ilist, nsuper_nodes); ilist.rad2 = zeros when with_r2."""
     if node_range is not None:
         raise NotImplementedError("node_range is a multi-GPU feature")
+    assert dtype == torch.int32
     dev = nnodes.device if torch.is_tensor(nnodes) else torch.device("cuda")
-    nn = _i32(nnodes, dev)[0]
+    nn = _i32(nnodes, dev)
     if size_super is None:
         size_super = int(np.ceil(np.sqrt(size_ilist))) + 2
-    nsuper = (nn + ngroup - 1) // ngroup
-    nint = nsuper * nsuper
-    ar = torch.arange(size_super + 1, dtype=torch.int64, device=dev)
-    spl_super = torch.minimum(ar * ngroup, nn.to(torch.int64)).to(dtype)
-    ispl = torch.minimum(ar * nsuper.to(torch.int64), nint.to(torch.int64)).to(dtype)
-    raise_if(nint.to(torch.int64) > size_ilist,
+    spl_super = torch.empty(size_super + 1, dtype=torch.int32, device=dev)
+    ispl = torch.empty(size_super + 1, dtype=torch.int32, device=dev)
+    ilist = torch.empty(size_ilist, dtype=torch.int32, device=dev)
+    r2 = torch.empty(size_ilist, dtype=torch.float32, device=dev) if with_r2 else None
+    nsuper = torch.empty(1, dtype=torch.int32, device=dev)
+    L.call("jzt_dense_ilist", L.stream(), L.ptr(nn), int(ngroup), L.ptr(spl_super), L.ptr(ispl), size_super,
+           L.ptr(ilist), L.ptr(r2), size_ilist, L.ptr(nsuper))
+    nint = nsuper[0].to(torch.int64) ** 2
+    raise_if(nint > size_ilist,
              "Cannot fit {n} interactions into interaction list with size {size}\nHint: Increase alloc_fac_ilist",
              n=nint, size=size_ilist)
-    idx = torch.arange(size_ilist, dtype=torch.int64, device=dev)
-    ilist = torch.where(idx < nint.to(torch.int64), idx % torch.clamp(nsuper, min=1).to(torch.int64),
-                        torch.zeros_like(idx)).to(dtype)
-    return spl_super, verify_ilist(InteractionList(ispl=ispl, isrc=ilist)), nsuper.to(torch.int32)
+    return spl_super, verify_ilist(InteractionList(ispl=ispl, isrc=ilist, rad2=r2)), nsuper[0]
 
 
 grouped_dense_interaction_list.jit = grouped_dense_interaction_list
