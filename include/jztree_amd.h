@@ -55,6 +55,17 @@ size_t jzt_zsort_tmp_bytes(int64_t n, int dim, int dtype);
 int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out, void* tmp,
                         size_t tmp_bytes, int64_t n, int dim, int dtype);
 
+/* Batched PosZorderSort -- no counterpart in the reference, whose batching is vmap with
+ * vmap_method="sequential" (tree.py:88): one set at a time.  The n points hold nsets independent
+ * sets, set s in rows [set_offsets[s], set_offsets[s+1]) (device int64, nsets+1 entries,
+ * set_offsets[0] = 0, set_offsets[nsets] = n).  Every set stays in its rows and is ordered exactly
+ * as jzt_pos_zorder_sort orders it alone; id_out holds input row indices (subtract the set's
+ * offset for set-local ids).  1 <= nsets <= 65536, n < 2^30. */
+size_t jzt_zsort_segmented_tmp_bytes(int64_t n, int64_t nsets, int dim, int dtype);
+int jzt_pos_zorder_sort_segmented(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out,
+                                  const int64_t* set_offsets, int64_t nsets, void* tmp, size_t tmp_bytes,
+                                  int64_t n, int dim, int dtype);
+
 /* SegmentSort -- replaces ffi target "SegmentSort" (generated/ffi_knn.cu:412-423, knn.cuh:700-722,
  * caller knn.py:112-127).  Sorts (key, val) within segments spl[i]..spl[i+1] by (key, val). */
 int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const float* key, const int32_t* val,
@@ -79,6 +90,21 @@ int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in, const void
                              const int32_t* nleaves, int32_t* nodes_levels, int32_t* nodes_lbound,
                              int32_t* nodes_rbound, int64_t size_nodes, int dim, int dtype, int lvl_max,
                              int lvl_invalid);
+
+/* Batched tree construction -- no counterpart in the reference (see jzt_pos_zorder_sort_segmented).
+ * As jzt_flag_leaf_boundaries (single particle type) and jzt_find_node_boundaries, with set_id
+ * (int32, one per row of posz / pos_in, non-decreasing) separating the sets of a batch: the
+ * boundary between rows of different sets gets level 2^16, above every Morton level, so it is
+ * always a leaf split and no node extends across it.  Inside a set the flags are those of a
+ * single-set call on that set. */
+int jzt_flag_leaf_boundaries_sets(jzt_stream_t stream, const void* posz, const int32_t* set_id,
+                                  const int32_t* lvl_bound, const int32_t* npart, int8_t* split_flags,
+                                  int32_t* lvl, int64_t size_part, int dim, int dtype, int max_size,
+                                  int scan_size);
+int jzt_find_node_boundaries_sets(jzt_stream_t stream, const void* pos_in, const int32_t* set_id,
+                                  const void* pos_boundary, const int32_t* nleaves, int32_t* nodes_levels,
+                                  int32_t* nodes_lbound, int32_t* nodes_rbound, int64_t size_nodes, int dim,
+                                  int dtype, int lvl_invalid);
 
 /* GetNodeGeometry -- replaces ffi target "GetNodeGeometry" (generated/ffi_tree.cu:409-529, kernel
  * tree.cuh:312-372, caller tree.py:286-329).  mode_flags: 1 level, 2 center, 4 extent (outputs not
@@ -113,6 +139,17 @@ int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl, const int3
  * *nsuper (device) receives nsuper.  The caller checks nsuper^2 <= size_ilist. */
 int jzt_dense_ilist(jzt_stream_t stream, const int32_t* nnodes, int ngroup, int32_t* spl_super, int32_t* ispl,
                     int64_t size_super, int32_t* ilist, float* r2, int64_t size_ilist, int32_t* nsuper);
+
+/* jzt_dense_ilist_sets: jzt_dense_ilist for a batch (no counterpart in the reference).  node_set
+ * (int32, non-decreasing over the first *nnodes entries, values in [0, nsets)) names the set of
+ * each top-plane node.  The c_s nodes of set s form ceil(c_s/ngroup) super-nodes (numbered set
+ * by set), and the list of a super-node holds every super-node of its own set, in order.  tmp
+ * holds jzt_dense_ilist_sets_tmp_bytes(nsets) bytes.  The caller checks sum_s nsuper_s^2 <=
+ * size_ilist via ispl[size_super]. */
+size_t jzt_dense_ilist_sets_tmp_bytes(int64_t nsets);
+int jzt_dense_ilist_sets(jzt_stream_t stream, const int32_t* nnodes, const int32_t* node_set, int64_t nsets,
+                         int ngroup, int32_t* spl_super, int32_t* ispl, int64_t size_super, int32_t* ilist,
+                         float* r2, int64_t size_ilist, int32_t* nsuper, void* tmp, size_t tmp_bytes);
 
 /* ------------------------------------------------------------------------------------------ */
 /* kNN                                                                                          */

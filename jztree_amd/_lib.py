@@ -27,13 +27,19 @@ _SIGNATURES = {
     "jzt_abi_version": ([], _i32),
     "jzt_zsort_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_pos_zorder_sort": ([_p, _p, _p, _p, _p, _sz, _i64, _i32, _i32], _i32),
+    "jzt_zsort_segmented_tmp_bytes": ([_i64, _i64, _i32, _i32], _sz),
+    "jzt_pos_zorder_sort_segmented": ([_p, _p, _p, _p, _p, _i64, _p, _sz, _i64, _i32, _i32], _i32),
     "jzt_segment_sort": ([_p, _p, _p, _p, _p, _p, _i64, _i64], _i32),
     "jzt_flag_leaf_boundaries": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32, _i32], _i32),
     "jzt_find_node_boundaries": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
+    "jzt_flag_leaf_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
+    "jzt_find_node_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32], _i32),
     "jzt_get_node_geometry": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32, _i32, _u32, _i32], _i32),
     "jzt_where_tmp_bytes": ([_i64], _sz),
     "jzt_where_flags": ([_p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _sz], _i32),
     "jzt_dense_ilist": ([_p, _p, _i32, _p, _p, _i64, _p, _p, _i64, _p], _i32),
+    "jzt_dense_ilist_sets_tmp_bytes": ([_i64], _sz),
+    "jzt_dense_ilist_sets": ([_p, _p, _p, _i64, _i32, _p, _p, _i64, _p, _p, _i64, _p, _p, _sz], _i32),
     "jzt_regularization_tmp_bytes": ([_i32, _i32], _sz),
     "jzt_regularization_level": ([_p, _p, _p, _i64, _p, _i32, _i32, _f32, _f32, _p, _sz], _i32),
     "jzt_knn_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i32, _f32, _i32,

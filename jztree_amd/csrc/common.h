@@ -82,6 +82,10 @@ __device__ __forceinline__ int msb_diff_level(const Vec<D, T>& a, const Vec<D, T
 template <typename T> __host__ __device__ constexpr int max_level(int dim) {
   return std::is_same<T, float>::value ? dim * 129 : dim * 1025;
 }
+// level given to the boundary between two points of different sets in a batch: above every Morton
+// level, so no cell, leaf or node ever spans two sets
+constexpr int kLvlSet = 1 << 16;
+
 template <typename T> __host__ __device__ constexpr int min_level(int dim) {
   return std::is_same<T, float>::value ? dim * -150 : dim * -1075;
 }

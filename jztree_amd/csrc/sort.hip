@@ -390,7 +390,8 @@ __device__ __forceinline__ bool idx_less(const T* __restrict__ pos, uint32_t a, 
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_fix_runs(const uint64_t* __restrict__ keys, uint32_t* vals, int64_t n,
                                                   const T* __restrict__ pos, const int32_t* __restrict__ runs,
-                                                  const int32_t* __restrict__ nruns_p, int64_t max_runs) {
+                                                  const int32_t* __restrict__ nruns_p, int64_t max_runs,
+                                                  const int32_t* __restrict__ setk) {
   __shared__ unsigned long long s_end;
   const int64_t nruns = min((int64_t)nruns_p[0], max_runs);
   for (int64_t r = blockIdx.x; r < nruns; r += gridDim.x) {
@@ -398,8 +399,9 @@ __global__ __launch_bounds__(256) void k_fix_runs(const uint64_t* __restrict__ k
     const uint64_t k = keys[start];
     if (threadIdx.x == 0) s_end = (unsigned long long)n;
     __syncthreads();
+    const int32_t sk = setk ? setk[start] : 0;
     for (int64_t j = start + 1 + threadIdx.x; j < n; j += blockDim.x) {
-      if (keys[j] != k) {
+      if (keys[j] != k || (setk && setk[j] != sk)) {
         atomicMin(&s_end, (unsigned long long)j);
         break;
       }
@@ -532,9 +534,254 @@ int zsort_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::ty
   const int64_t max_runs = n / 2 + 1;
   hipLaunchKernelGGL(k_run_flags, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, k0, n, rflags);
   if (where_flags_i8(s, rflags, n, runs, max_runs, 0, nullptr, nruns, (int32_t*)(tmp + L.runtmp))) return 1;
-  hipLaunchKernelGGL((k_fix_runs<D, T>), dim3(1024), dim3(256), 0, s, k0, v0, n, src, runs, nruns, max_runs);
+  hipLaunchKernelGGL((k_fix_runs<D, T>), dim3(1024), dim3(256), 0, s, k0, v0, n, src, runs, nruns, max_runs,
+                     (const int32_t*)nullptr);
   hipLaunchKernelGGL((k_gather<D, T>), dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, src, v0, n, pos_out, id_out);
   return check_launch("zsort gather");
+}
+
+// ---------------------------------------------------------------------------------------------
+// 7. segmented sort (many independent point sets, one pipeline)
+// ---------------------------------------------------------------------------------------------
+// Points of set s occupy [off[s], off[s+1]) of the input; the output keeps every set in its range,
+// z-ordered inside it exactly as a single-set sort of that set would order it (the Morton window
+// of each set comes from that set's own bounds).  After the 8 Morton passes a stable pass (two if
+// there are more than 256 sets) on the set id restores the set ranges without disturbing the
+// order inside them; equal-key runs are fixed per set.
+
+constexpr int kSetLds = 4096;  // set offsets staged in LDS up to this many sets
+
+__device__ __forceinline__ int32_t set_of(const int64_t* off, int64_t nsets, int64_t p) {
+  int64_t lo = 0, hi = nsets - 1;  // last s with off[s] <= p
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return (int32_t)lo;
+}
+
+__global__ __launch_bounds__(256) void k_set_ids(const int64_t* __restrict__ off, int64_t nsets, int64_t n,
+                                                 int32_t* __restrict__ setid) {
+  __shared__ int64_t s_off[kSetLds];
+  const bool lds = nsets <= kSetLds;
+  if (lds)
+    for (int64_t i = threadIdx.x; i < nsets; i += blockDim.x) s_off[i] = off[i];
+  __syncthreads();
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    setid[p] = lds ? set_of(s_off, nsets, p) : set_of(off, nsets, p);
+}
+
+// per-set bounds: metas[3*s + {0,1,2}] as meta[] of k_bounds for set s.  Set ids never decrease
+// along the input, so each lane keeps one running accumulator and flushes it when its set changes;
+// the last flush is one atomic per wave when the whole wave ends in one set.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_bounds_seg(const T* __restrict__ pos, int64_t n,
+                                                    const int32_t* __restrict__ setid, uint32_t* metas) {
+  using FB = FBits<T>;
+  using U = typename FB::U;
+  uint32_t emax = 0, neg = 0, inf = 0;
+  int32_t cur = -1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t sid = setid[i];
+    if (sid != cur) {
+      if (cur >= 0) {
+        atomicMax(&metas[3 * cur], emax);
+        if (neg) atomicOr(&metas[3 * cur + 1], 1u);
+        if (inf) atomicOr(&metas[3 * cur + 2], 1u);
+      }
+      emax = neg = inf = 0;
+      cur = sid;
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const T x = pos[i * D + d];
+      const U u = FB::bits(x) & ~((U)1 << (sizeof(U) * 8 - 1));
+      const uint32_t be = (uint32_t)(u >> FB::kMant);
+      const bool fin = be != (uint32_t)FB::kExpMax;
+      if (fin && u != 0) emax = max(emax, be ? be : 1u);
+      neg |= (x < T(0)) ? 1u : 0u;
+      inf |= (!fin && (u & (((U)1 << FB::kMant) - 1)) == 0) ? 1u : 0u;
+    }
+  }
+  const int32_t s0 = __builtin_amdgcn_readfirstlane(cur);
+  if (__ballot(cur != s0) == 0ull) {
+    for (int o = 32; o > 0; o >>= 1) {
+      emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
+      neg |= (uint32_t)__shfl_xor((int)neg, o, 64);
+      inf |= (uint32_t)__shfl_xor((int)inf, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && s0 >= 0) {
+      atomicMax(&metas[3 * s0], emax);
+      if (neg) atomicOr(&metas[3 * s0 + 1], 1u);
+      if (inf) atomicOr(&metas[3 * s0 + 2], 1u);
+    }
+  } else if (cur >= 0) {
+    atomicMax(&metas[3 * cur], emax);
+    if (neg) atomicOr(&metas[3 * cur + 1], 1u);
+    if (inf) atomicOr(&metas[3 * cur + 2], 1u);
+  }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_keygen_seg(const T* __restrict__ pos, int64_t n,
+                                                    const int32_t* __restrict__ setid,
+                                                    const uint32_t* __restrict__ metas, uint64_t* __restrict__ keys,
+                                                    uint64_t* __restrict__ keys_orig, uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t sh[kPasses][kRadix];
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&sh[0][0])[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t* meta = metas + 3 * setid[i];
+    const int Lt = top_level_from_meta<T>(meta);
+    const bool general = (meta[1] | meta[2]) != 0;
+    T x[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) x[d] = pos[i * D + d];
+    const uint64_t k = morton_key<D, T>(x, Lt, general);
+    keys[i] = k;
+    keys_orig[i] = k;
+    vals[i] = (uint32_t)i;
+#pragma unroll
+    for (int p = 0; p < kPasses; p++) atomicAdd(&sh[p][(k >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) {
+    const uint32_t c = (&sh[0][0])[j];
+    if (c) atomicAdd(&hist[j], c);
+  }
+}
+
+// digit histograms of the set ids (pass 0: bits 0-7, pass 1: bits 8-15), from the set sizes
+__global__ void k_set_hist(const int64_t* __restrict__ off, int64_t nsets, uint32_t* __restrict__ hist) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nsets; s += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = (uint32_t)(off[s + 1] - off[s]);
+    if (c) {
+      atomicAdd(&hist[s & 255], c);
+      atomicAdd(&hist[kRadix + ((s >> 8) & 255)], c);
+    }
+  }
+}
+
+// after the Morton passes: set key and Morton key of every sorted position
+__global__ void k_set_keys(const uint32_t* __restrict__ vals, const int32_t* __restrict__ setid, int64_t n,
+                           uint64_t* __restrict__ setkey) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) setkey[i] = (uint64_t)(uint32_t)setid[vals[i] < (uint64_t)n ? vals[i] : 0];
+}
+__global__ void k_final_keys(const uint32_t* __restrict__ vals, const uint64_t* __restrict__ keys_orig,
+                             const int32_t* __restrict__ setid, int64_t n, uint64_t* __restrict__ kf,
+                             int32_t* __restrict__ setk) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = vals[i] < (uint64_t)n ? vals[i] : 0u;
+  kf[i] = keys_orig[v];
+  setk[i] = setid[v];
+}
+__global__ __launch_bounds__(256) void k_run_flags_seg(const uint64_t* __restrict__ keys,
+                                                       const int32_t* __restrict__ setk, int64_t n,
+                                                       int8_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const int32_t sk = setk[i];
+  const bool next = i + 1 < n && keys[i + 1] == k && setk[i + 1] == sk;
+  const bool prev = i > 0 && keys[i - 1] == k && setk[i - 1] == sk;
+  flags[i] = (next && !prev) ? 1 : 0;
+}
+
+struct SegLayout {
+  SortLayout base;
+  size_t setid, setk, korig, metas, total;
+};
+
+SegLayout seg_layout(int64_t n, int64_t nsets, int dim, int dtype) {
+  SegLayout L;
+  L.base = sort_layout(n, dim, dtype);
+  size_t o = L.base.total;
+  L.setid = o; o = align256(o + 4 * (size_t)n);
+  L.setk = o; o = align256(o + 4 * (size_t)n);
+  L.korig = o; o = align256(o + 8 * (size_t)n);
+  L.metas = o; o = align256(o + 4 * 3 * (size_t)(nsets > 0 ? nsets : 1));
+  L.total = o;
+  return L;
+}
+
+template <int D, typename T>
+int zsort_seg_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::type* id_out, const int64_t* off,
+                   int64_t nsets, char* tmp, int64_t n) {
+  const SegLayout SL = seg_layout(n, nsets, D, std::is_same<T, double>::value ? JZT_F64 : JZT_F32);
+  const SortLayout& L = SL.base;
+  uint64_t* k0 = (uint64_t*)(tmp + L.keys0);
+  uint64_t* k1 = (uint64_t*)(tmp + L.keys1);
+  uint32_t* v0 = (uint32_t*)(tmp + L.vals0);
+  uint32_t* v1 = (uint32_t*)(tmp + L.vals1);
+  uint32_t* hist = (uint32_t*)(tmp + L.hist);
+  uint32_t* meta = (uint32_t*)(tmp + L.meta);
+  uint32_t* lookback = (uint32_t*)(tmp + L.lookback);
+  int32_t* setid = (int32_t*)(tmp + SL.setid);
+  uint64_t* korig = (uint64_t*)(tmp + SL.korig);
+  uint32_t* metas = (uint32_t*)(tmp + SL.metas);
+  if (fill_u32(s, hist, 0u, (int64_t)((L.poscopy - L.hist) / 4))) return 1;
+  if (fill_u32(s, metas, 0u, 3 * nsets)) return 1;
+  const T* src = pos_in;
+  if ((const void*)pos_in == (const void*)pos_out) {
+    T* cp = (T*)(tmp + L.poscopy);
+    if (hipMemcpyAsync(cp, pos_in, (size_t)n * D * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail("zsort_segmented: copy of aliased input failed");
+    src = cp;
+  }
+  const unsigned gb = (unsigned)std::min(div_up(n, 256), (int64_t)2048);
+  hipLaunchKernelGGL(k_set_ids, dim3(gb), dim3(256), 0, s, off, nsets, n, setid);
+  hipLaunchKernelGGL((k_bounds_seg<D, T>), dim3(gb), dim3(256), 0, s, src, n, setid, metas);
+  hipLaunchKernelGGL((k_keygen_seg<D, T>), dim3(gb), dim3(256), 0, s, src, n, setid, metas, k0, korig, v0, hist);
+  hipLaunchKernelGGL(k_hist_scan, dim3(1), dim3(256), 0, s, hist);
+  if (check_launch("zsort_segmented keygen")) return 1;
+  for (int p = 0; p < kPasses; p++) {
+    const uint64_t* ki = (p & 1) ? k1 : k0;
+    const uint32_t* vi = (p & 1) ? v1 : v0;
+    uint64_t* ko = (p & 1) ? k0 : k1;
+    uint32_t* vo = (p & 1) ? v0 : v1;
+    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n, 8 * p,
+                       hist + p * kRadix, lookback + (int64_t)p * L.ntiles * kRadix, meta + 8 + p);
+  }
+  if (check_launch("zsort_segmented onesweep")) return 1;
+  // stable set passes: (set key, value) from (k1, v0); histograms, look-back and tile counters of
+  // Morton passes 0 and 1 are reused after clearing
+  const int npass_set = nsets > 256 ? 2 : 1;
+  if (fill_u32(s, hist, 0u, 2 * kRadix)) return 1;
+  if (fill_u32(s, lookback, 0u, 2 * L.ntiles * kRadix)) return 1;
+  if (fill_u32(s, meta + 8, 0u, 2)) return 1;
+  hipLaunchKernelGGL(k_set_hist, dim3((unsigned)std::min(div_up(nsets, 256), (int64_t)1024)), dim3(256), 0, s, off, nsets,
+                     hist);
+  hipLaunchKernelGGL(k_hist_scan, dim3(1), dim3(256), 0, s, hist);  // scans all 8 rows; rows 0-1 matter
+  hipLaunchKernelGGL(k_set_keys, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, v0, setid, n, k1);
+  const uint32_t* vfinal = v0;
+  {
+    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, k1, v0, k0, v1, n, 0, hist,
+                       lookback, meta + 8);
+    vfinal = v1;
+    if (npass_set == 2) {
+      hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, k0, v1, k1, v0, n, 8,
+                         hist + kRadix, lookback + (int64_t)L.ntiles * kRadix, meta + 9);
+      vfinal = v0;
+    }
+  }
+  if (check_launch("zsort_segmented set passes")) return 1;
+  uint32_t* vf = const_cast<uint32_t*>(vfinal);
+  uint64_t* kf = (vf == v1) ? k1 : k0;  // the key buffer not holding set keys any more
+  int32_t* setk = (int32_t*)(tmp + SL.setk);
+  hipLaunchKernelGGL(k_final_keys, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vf, korig, setid, n, kf, setk);
+  int8_t* rflags = (int8_t*)(tmp + L.runflags);
+  int32_t* runs = (int32_t*)(tmp + L.runs);
+  int32_t* nruns = (int32_t*)(tmp + L.nruns);
+  const int64_t max_runs = n / 2 + 1;
+  hipLaunchKernelGGL(k_run_flags_seg, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, kf, setk, n, rflags);
+  if (where_flags_i8(s, rflags, n, runs, max_runs, 0, nullptr, nruns, (int32_t*)(tmp + L.runtmp))) return 1;
+  hipLaunchKernelGGL((k_fix_runs<D, T>), dim3(1024), dim3(256), 0, s, kf, vf, n, src, runs, nruns, max_runs, setk);
+  hipLaunchKernelGGL((k_gather<D, T>), dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, src, vf, n, pos_out, id_out);
+  return check_launch("zsort_segmented gather");
 }
 
 }  // namespace
@@ -563,4 +810,37 @@ extern "C" int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void
     if (dim == 2) return zsort_impl<2, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
   }
   return fail("jzt_pos_zorder_sort: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
+}
+
+extern "C" size_t jzt_zsort_segmented_tmp_bytes(int64_t n, int64_t nsets, int dim, int dtype) {
+  return seg_layout(n, nsets, dim, dtype).total;
+}
+
+extern "C" int jzt_pos_zorder_sort_segmented(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out,
+                                             const int64_t* set_offsets, int64_t nsets, void* tmp, size_t tmp_bytes,
+                                             int64_t n, int dim, int dtype) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0 || nsets < 1) return fail("jzt_pos_zorder_sort_segmented: need n >= 0 and nsets >= 1");
+  if (nsets > 65536) return fail("jzt_pos_zorder_sort_segmented: at most 65536 sets");
+  if (n == 0) return 0;
+  if (n >= (int64_t)kCountMask) return fail("jzt_pos_zorder_sort_segmented: n must be < 2^30");
+  const size_t need = seg_layout(n, nsets, dim, dtype).total;
+  if (tmp_bytes < need)
+    return fail("jzt_pos_zorder_sort_segmented: tmp buffer too small (have " + std::to_string(tmp_bytes) +
+                ", need " + std::to_string(need) + ")");
+  char* t = (char*)tmp;
+  if (dtype == JZT_F32) {
+    if (dim == 3)
+      return zsort_seg_impl<3, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, set_offsets, nsets, t, n);
+    if (dim == 2)
+      return zsort_seg_impl<2, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, set_offsets, nsets, t, n);
+  } else if (dtype == JZT_F64) {
+    if (dim == 3)
+      return zsort_seg_impl<3, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, set_offsets, nsets,
+                                       t, n);
+    if (dim == 2)
+      return zsort_seg_impl<2, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, set_offsets, nsets,
+                                       t, n);
+  }
+  return fail("jzt_pos_zorder_sort_segmented: unsupported dim/dtype");
 }

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
                                                             const int32_t* __restrict__ npart_p,
                                                             int8_t* __restrict__ split_flags, int32_t* __restrict__ lvl,
                                                             int64_t size_part, int max_size, int scan_size,
-                                                            int num_types) {
+                                                            int num_types, const int32_t* __restrict__ seg) {
   extern __shared__ int32_t s_level[];
   const int nload = kTreeThreads + 2 * scan_size + 1;
   uint8_t* s_types = reinterpret_cast<uint8_t*>(s_level + nload);
@@ -41,7 +41,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
     } else if (ip + 1 >= nump) {
       L = lb_r;
     } else {
-      L = msb_diff_level<D, T>(load_vec<D, T>(posz, ip), load_vec<D, T>(posz, ip + 1));
+      L = (seg && seg[ip] != seg[ip + 1]) ? kLvlSet
+                                          : msb_diff_level<D, T>(load_vec<D, T>(posz, ip), load_vec<D, T>(posz, ip + 1));
       if (num_types > 1) ty = ptype[ip];
     }
     s_level[i] = L;
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
     is_split |= (my > lb_l) && (node_idx - lsize <= 0);
     is_split |= (my > lb_r) && (node_idx + rsize >= nump);
   }
+  is_split |= my >= kLvlSet;  // set boundary of a batch
   is_split &= node_idx <= nump;
   is_split |= node_idx == 0;
   is_split |= node_idx == nump;
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_find_node_bounds(const T* __re
                                                                    const int32_t* __restrict__ nleaves_p,
                                                                    int32_t* __restrict__ levels, int32_t* __restrict__ lbound,
                                                                    int32_t* __restrict__ rbound, int64_t size_nodes,
-                                                                   int lvl_invalid) {
+                                                                   int lvl_invalid, const int32_t* __restrict__ seg) {
   int64_t idx = (int64_t)blockIdx.x * kTreeThreads + threadIdx.x;
   const int n = nleaves_p[0];
   const int nnodes = n + 1;
@@ -89,32 +91,38 @@ __global__ __launch_bounds__(kTreeThreads) void k_find_node_bounds(const T* __re
   }
   Vec<D, T> p1 = (idx == 0) ? load_vec<D, T>(pos_boundary, 0) : load_vec<D, T>(pos_in, idx - 1);
   Vec<D, T> p2 = (idx == nnodes - 1) ? load_vec<D, T>(pos_boundary, 1) : load_vec<D, T>(pos_in, idx);
-  const int target = msb_diff_level<D, T>(p1, p2);
   if (n <= 0) {  // no leaves: the single boundary spans nothing (the reference would read pos_in[-1])
-    levels[idx] = target;
+    levels[idx] = msb_diff_level<D, T>(p1, p2);
     lbound[idx] = 0;
     rbound[idx] = 0;
     return;
   }
+  // batch sets: points of different sets are separated at kLvlSet; the array ends take the set of
+  // their neighbour.  Sets are contiguous, so levels stay monotone along the searches below.
+  const int64_t ia = idx > 0 ? idx - 1 : 0, ib = idx < n ? idx : n - 1;
+  auto level = [&](int64_t i, const Vec<D, T>& a, int64_t j, const Vec<D, T>& b) {
+    return (seg && seg[i] != seg[j]) ? kLvlSet : msb_diff_level<D, T>(a, b);
+  };
+  const int target = level(ia, p1, ib, p2);
   int lb, rb;
-  if (msb_diff_level<D, T>(load_vec<D, T>(pos_in, 0), p2) <= target) {
+  if (level(0, load_vec<D, T>(pos_in, 0), ib, p2) <= target) {
     lb = 0;
   } else {
     int ibefore = 0, iinside = (int)idx;
     while (ibefore + 1 < iinside) {
       int it = (ibefore + iinside) / 2;
-      if (msb_diff_level<D, T>(load_vec<D, T>(pos_in, it), p2) > target) ibefore = it;
+      if (level(it, load_vec<D, T>(pos_in, it), ib, p2) > target) ibefore = it;
       else iinside = it;
     }
     lb = iinside;
   }
-  if (msb_diff_level<D, T>(p1, load_vec<D, T>(pos_in, n - 1)) <= target) {
+  if (level(ia, p1, n - 1, load_vec<D, T>(pos_in, n - 1)) <= target) {
     rb = n;
   } else {
     int iinside = (int)idx - 1, iafter = n - 1;
     while (iinside + 1 < iafter) {
       int it = (iinside + iafter) / 2;
-      if (msb_diff_level<D, T>(p1, load_vec<D, T>(pos_in, it)) > target) iafter = it;
+      if (level(ia, p1, it, load_vec<D, T>(pos_in, it)) > target) iafter = it;
       else iinside = it;
     }
     rb = iafter;
@@ -176,21 +184,23 @@ __global__ __launch_bounds__(kTreeThreads) void k_node_geometry(const T* __restr
 
 template <int D, typename T>
 int flag_leaf_impl(hipStream_t s, const void* posz, const uint8_t* ptype, const int32_t* lvl_bound, const int32_t* npart,
-                   int8_t* flags, int32_t* lvl, int64_t size_part, int max_size, int scan_size, int num_types) {
+                   int8_t* flags, int32_t* lvl, int64_t size_part, int max_size, int scan_size, int num_types,
+                   const int32_t* seg) {
   int64_t nb = size_part + 1;
   int nload = kTreeThreads + 2 * scan_size + 1;
   size_t shmem = (size_t)nload * (sizeof(int32_t) + 1);
   if (shmem > 60000) return fail("FlagLeafBoundaries: scan_size too large");
   hipLaunchKernelGGL((k_flag_leaf<D, T>), dim3((unsigned)div_up(nb, kTreeThreads)), dim3(kTreeThreads), shmem, s,
-                     (const T*)posz, ptype, lvl_bound, npart, flags, lvl, size_part, max_size, scan_size, num_types);
+                     (const T*)posz, ptype, lvl_bound, npart, flags, lvl, size_part, max_size, scan_size, num_types,
+                     seg);
   return check_launch("FlagLeafBoundaries");
 }
 
 template <int D, typename T>
 int find_node_impl(hipStream_t s, const void* pos_in, const void* pos_boundary, const int32_t* nleaves, int32_t* lv,
-                   int32_t* lb, int32_t* rb, int64_t size_nodes, int lvl_invalid) {
+                   int32_t* lb, int32_t* rb, int64_t size_nodes, int lvl_invalid, const int32_t* seg) {
   hipLaunchKernelGGL((k_find_node_bounds<D, T>), dim3((unsigned)div_up(size_nodes, kTreeThreads)), dim3(kTreeThreads),
-                     0, s, (const T*)pos_in, (const T*)pos_boundary, nleaves, lv, lb, rb, size_nodes, lvl_invalid);
+                     0, s, (const T*)pos_in, (const T*)pos_boundary, nleaves, lv, lb, rb, size_nodes, lvl_invalid, seg);
   return check_launch("FindNodeBoundaries");
 }
 
@@ -308,7 +318,16 @@ extern "C" int jzt_flag_leaf_boundaries(jzt_stream_t stream, const void* posz, c
                                         int scan_size, int num_types) {
   if (num_types > 1 && ptype == nullptr) return fail("FlagLeafBoundaries: ptype required for num_types > 1");
   JZT_DISPATCH(flag_leaf_impl, (hipStream_t)stream, posz, ptype, lvl_bound, npart, split_flags, lvl, size_part,
-               max_size, scan_size, num_types);
+               max_size, scan_size, num_types, (const int32_t*)nullptr);
+}
+
+extern "C" int jzt_flag_leaf_boundaries_sets(jzt_stream_t stream, const void* posz, const int32_t* set_id,
+                                             const int32_t* lvl_bound, const int32_t* npart, int8_t* split_flags,
+                                             int32_t* lvl, int64_t size_part, int dim, int dtype, int max_size,
+                                             int scan_size) {
+  if (set_id == nullptr) return fail("FlagLeafBoundaries (sets): set_id is required");
+  JZT_DISPATCH(flag_leaf_impl, (hipStream_t)stream, posz, (const uint8_t*)nullptr, lvl_bound, npart, split_flags, lvl,
+               size_part, max_size, scan_size, 1, set_id);
 }
 
 extern "C" int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in, const void* pos_boundary,
@@ -318,7 +337,17 @@ extern "C" int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in,
   (void)lvl_max;  // the reference passes it but its kernel never reads it outside dead stores (tree.cuh:144-196)
   if (size_nodes <= 0) return 0;
   JZT_DISPATCH(find_node_impl, (hipStream_t)stream, pos_in, pos_boundary, nleaves, nodes_levels, nodes_lbound,
-               nodes_rbound, size_nodes, lvl_invalid);
+               nodes_rbound, size_nodes, lvl_invalid, (const int32_t*)nullptr);
+}
+
+extern "C" int jzt_find_node_boundaries_sets(jzt_stream_t stream, const void* pos_in, const int32_t* set_id,
+                                             const void* pos_boundary, const int32_t* nleaves, int32_t* nodes_levels,
+                                             int32_t* nodes_lbound, int32_t* nodes_rbound, int64_t size_nodes, int dim,
+                                             int dtype, int lvl_invalid) {
+  if (set_id == nullptr) return fail("FindNodeBoundaries (sets): set_id is required");
+  if (size_nodes <= 0) return 0;
+  JZT_DISPATCH(find_node_impl, (hipStream_t)stream, pos_in, pos_boundary, nleaves, nodes_levels, nodes_lbound,
+               nodes_rbound, size_nodes, lvl_invalid, set_id);
 }
 
 extern "C" int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* lbound, const int32_t* rbound,

@@ -154,6 +154,93 @@ __global__ void k_dense_ilist(const int32_t* __restrict__ nn_dev, int ngroup, in
   if (tid == 0) nsuper_out[0] = (int32_t)nsuper;
 }
 
+// Dense top-plane lists of a batch: the top nodes of set s (node_set non-decreasing over the first
+// nn nodes) form ceil(c_s/ngroup) super-nodes, and a super-node interacts with every super-node
+// of its own set and with no other.  tmp: F, S, I (int64, nsets+1 each) = exclusive scans of the
+// node counts, super-node counts and list lengths per set.
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t n, int32_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int64_t last_le_i64(const int64_t* a, int64_t n, int64_t v) {  // last i < n with a[i] <= v
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_dense_sets_prep(const int32_t* __restrict__ nn_dev,
+                                                         const int32_t* __restrict__ node_set, int64_t nsets, int ngroup,
+                                                         int64_t* __restrict__ F, int64_t* __restrict__ S,
+                                                         int64_t* __restrict__ I) {
+  __shared__ int64_t s_c[256], s_u[256], s_q[256];
+  __shared__ int64_t carry[3];
+  const int64_t nn = nn_dev[0] > 0 ? nn_dev[0] : 0;
+  if (threadIdx.x == 0) carry[0] = carry[1] = carry[2] = 0;
+  __syncthreads();
+  for (int64_t b = 0; b < nsets; b += 256) {
+    const int64_t s = b + threadIdx.x;
+    int64_t c = 0;
+    if (s < nsets) c = lower_bound_i32(node_set, nn, (int32_t)(s + 1)) - lower_bound_i32(node_set, nn, (int32_t)s);
+    const int64_t u = (c + ngroup - 1) / ngroup;
+    s_c[threadIdx.x] = c;
+    s_u[threadIdx.x] = u;
+    s_q[threadIdx.x] = u * u;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // serial scan of 256 entries per chunk: nsets is at most a few 10^4
+      int64_t c0 = carry[0], c1 = carry[1], c2 = carry[2];
+      for (int t = 0; t < 256 && b + t < nsets; t++) {
+        F[b + t] = c0; S[b + t] = c1; I[b + t] = c2;
+        c0 += s_c[t]; c1 += s_u[t]; c2 += s_q[t];
+      }
+      carry[0] = c0; carry[1] = c1; carry[2] = c2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    F[nsets] = carry[0];
+    S[nsets] = carry[1];
+    I[nsets] = carry[2];
+  }
+}
+
+__global__ void k_dense_sets_fill(const int32_t* __restrict__ nn_dev, int64_t nsets, int ngroup,
+                                  const int64_t* __restrict__ F, const int64_t* __restrict__ S,
+                                  const int64_t* __restrict__ I, int32_t* __restrict__ spl_super,
+                                  int32_t* __restrict__ ispl, int64_t size_super, int32_t* __restrict__ ilist,
+                                  float* __restrict__ r2, int64_t size_ilist, int32_t* __restrict__ nsuper_out) {
+  const int64_t nn = nn_dev[0] > 0 ? nn_dev[0] : 0;
+  const int64_t nsup = S[nsets], nint = I[nsets];
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t a = tid; a <= size_super; a += st) {
+    if (a < nsup) {
+      const int64_t s = last_le_i64(S, nsets + 1, a);
+      const int64_t u = a - S[s], ns = S[s + 1] - S[s];
+      spl_super[a] = (int32_t)min(F[s] + u * ngroup, nn);
+      ispl[a] = (int32_t)min(I[s] + u * ns, (int64_t)INT32_MAX);
+    } else {
+      spl_super[a] = (int32_t)nn;
+      ispl[a] = (int32_t)min(nint, (int64_t)INT32_MAX);
+    }
+  }
+  const int64_t nw = min(nint, size_ilist);
+  for (int64_t j = tid; j < nw; j += st) {
+    const int64_t s = last_le_i64(I, nsets + 1, j);
+    const int64_t ns = S[s + 1] - S[s];
+    ilist[j] = (int32_t)(S[s] + (j - I[s]) % ns);
+    if (r2) r2[j] = 0.f;
+  }
+  if (tid == 0) nsuper_out[0] = (int32_t)nsup;
+}
+
 }  // namespace jzt
 
 extern "C" const char* jzt_last_error(void) { return jzt::g_err.c_str(); }
@@ -168,4 +255,25 @@ extern "C" int jzt_dense_ilist(jzt_stream_t stream, const int32_t* nnodes, int n
   hipLaunchKernelGGL(k_dense_ilist, dim3(grid_for(div_up(work, 256))), dim3(256), 0, (hipStream_t)stream, nnodes,
                      ngroup, spl_super, ispl, size_super, ilist, r2, size_ilist, nsuper);
   return check_launch("DenseIlist");
+}
+
+extern "C" size_t jzt_dense_ilist_sets_tmp_bytes(int64_t nsets) { return (size_t)3 * 8 * (size_t)(nsets + 1); }
+
+extern "C" int jzt_dense_ilist_sets(jzt_stream_t stream, const int32_t* nnodes, const int32_t* node_set,
+                                    int64_t nsets, int ngroup, int32_t* spl_super, int32_t* ispl, int64_t size_super,
+                                    int32_t* ilist, float* r2, int64_t size_ilist, int32_t* nsuper, void* tmp,
+                                    size_t tmp_bytes) {
+  using namespace jzt;
+  if (ngroup < 1) return fail("DenseIlistSets: ngroup must be >= 1");
+  if (nsets < 1 || nsets > INT32_MAX - 1) return fail("DenseIlistSets: need 1 <= nsets < 2^31 - 1");
+  if (tmp_bytes < jzt_dense_ilist_sets_tmp_bytes(nsets)) return fail("DenseIlistSets: tmp buffer too small");
+  int64_t* F = (int64_t*)tmp;
+  int64_t* S = F + (nsets + 1);
+  int64_t* I = S + (nsets + 1);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dense_sets_prep, dim3(1), dim3(256), 0, s, nnodes, node_set, nsets, ngroup, F, S, I);
+  const int64_t work = size_super + 1 > size_ilist ? size_super + 1 : size_ilist;
+  hipLaunchKernelGGL(k_dense_sets_fill, dim3(grid_for(div_up(work, 256))), dim3(256), 0, s, nnodes, nsets, ngroup, F, S,
+                     I, spl_super, ispl, size_super, ilist, r2, size_ilist, nsuper);
+  return check_launch("DenseIlistSets");
 }

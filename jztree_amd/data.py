@@ -237,6 +237,9 @@ class TreeHierarchy():
     geom_cent: PackedArray
     mass: PackedArray | None = None
     mass_cent: PackedArray | None = None
+    # batches of independent sets (extension): set id of every leaf slot and the number of sets
+    leaf_set: torch.Tensor | None = None
+    nsets: int = 1
 
     def splits_leaf_to_part(self, ptype: int = 0, size: int | None = None) -> torch.Tensor:
         if size is None:

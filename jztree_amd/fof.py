@@ -17,7 +17,7 @@ from . import checks
 from .checks import raise_if
 from .config import FofConfig
 from .data import InteractionList, PackedArray, TreeHierarchy, get_pos, verify_ilist
-from .tree import build_tree_hierarchy, grouped_dense_interaction_list, zsort
+from .tree import build_tree_hierarchy, top_dense_interaction_list, zsort
 
 
 @dataclass
@@ -89,9 +89,7 @@ def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_f
     """fof.py:111-151: plane-by-plane node linking and interaction lists."""
     nplanes = th.num_planes()
     size = th.size()
-    spl, ilist, nsup = grouped_dense_interaction_list(th.lvl.num(nplanes - 1),
-                                                      int(th.size_leaves * alloc_fac_ilist), ngroup=32,
-                                                      size_super=size)
+    spl, ilist, nsup = top_dense_interaction_list(th, int(th.size_leaves * alloc_fac_ilist))
     dev = spl.device
     igroup = torch.arange(size, dtype=torch.int32, device=dev)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)

@@ -20,7 +20,7 @@ from .checks import raise_if
 from .config import KNNConfig
 from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
 from .tools import inverse_of_splits, masked_inverse
-from .tree import _tmp, grouped_dense_interaction_list, zsort_and_tree
+from .tree import _tmp, top_dense_interaction_list, zsort_and_tree
 
 
 @dataclass
@@ -133,8 +133,7 @@ def _knn_dual_walk(th: TreeHierarchy, k: int, boxsize: float | None = None,
     node-to-child splits of that plane)."""
     nlevels = th.num_planes()
     size = th.size()
-    spl, ilist, nsup = grouped_dense_interaction_list(
-        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size, with_r2=True)
+    spl, ilist, nsup = top_dense_interaction_list(th, int(th.size_leaves * alloc_fac_ilist), with_r2=True)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
     rmax2 = None
@@ -252,8 +251,7 @@ def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0.
     top-plane list down to leaf lists holding every leaf within each leaf's radius."""
     nlevels = th.num_planes()
     size = th.size()
-    spl, ilist, nsup = grouped_dense_interaction_list(
-        th.lvl.num(nlevels - 1), int(th.size_leaves * alloc_fac_ilist), ngroup=32, size_super=size, with_r2=True)
+    spl, ilist, nsup = top_dense_interaction_list(th, int(th.size_leaves * alloc_fac_ilist), with_r2=True)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
     for i in range(nlevels):

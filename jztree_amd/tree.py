@@ -21,6 +21,10 @@ from .data import (InteractionList, LevelInfo, PackedArray, Pos, TreeHierarchy, 
 from .tools import cumsum_starting_with_zero, div_ceil
 
 
+# level of a boundary between two sets of a batch: above every Morton level
+LVL_SET = 1 << 16
+
+
 def _i32(x, device) -> torch.Tensor:
     """A (1,) int32 device tensor holding x (python int or device scalar)."""
     if torch.is_tensor(x):
@@ -130,9 +134,13 @@ def _where_flags(flags: torch.Tensor, size: int, fill) -> Tuple[torch.Tensor, to
 
 def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None, num_types: int | None = None,
                            leaf_size: int = 32, npart=None, lvl_bound=None, block_size: int = 64,
-                           alloc_size: int | None = None, cfg_reg: RegularizationConfig | None = None) -> torch.Tensor:
+                           alloc_size: int | None = None, cfg_reg: RegularizationConfig | None = None,
+                           set_id: torch.Tensor | None = None) -> torch.Tensor:
     """Leaf splits of z-sorted positions (tree.py:189-262): leaves are the largest binary Morton
-    cells holding no more than leaf_size particles of any type."""
+    cells holding no more than leaf_size particles of any type.
+
+    set_id (extension): int32 set of every row of a batch of independent sets (non-decreasing);
+    set boundaries are always splits and no count runs across them."""
     n, dim = posz.shape
     dev = posz.device
     if alloc_size is None:
@@ -153,9 +161,16 @@ def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None
         ptype_t = ptype.to(torch.uint8).contiguous()
     flags = torch.empty(n + 1, dtype=torch.int8, device=dev)
     lvl = torch.empty(n + 1, dtype=torch.int32, device=dev)
-    L.call("jzt_flag_leaf_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(ptype_t), L.ptr(lvl_bound_t),
-           L.ptr(npart_t), L.ptr(flags), L.ptr(lvl), n, dim, L.dtype_code(posz.dtype), int(leaf_size),
-           int((leaf_size + 1) * num_types), int(num_types))
+    if set_id is not None:
+        assert num_types == 1 and cfg_reg is None, "batches of sets use one particle type and no regularization"
+        assert set_id.dtype == torch.int32 and len(set_id) == n
+        L.call("jzt_flag_leaf_boundaries_sets", L.stream(), L.ptr(posz.contiguous()), L.ptr(set_id.contiguous()),
+               L.ptr(lvl_bound_t), L.ptr(npart_t), L.ptr(flags), L.ptr(lvl), n, dim, L.dtype_code(posz.dtype),
+               int(leaf_size), int(leaf_size + 1))
+    else:
+        L.call("jzt_flag_leaf_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(ptype_t), L.ptr(lvl_bound_t),
+               L.ptr(npart_t), L.ptr(flags), L.ptr(lvl), n, dim, L.dtype_code(posz.dtype), int(leaf_size),
+               int((leaf_size + 1) * num_types), int(num_types))
 
     if cfg_reg is not None:
         splits, _ = _where_flags(flags, alloc_size, npart_t)
@@ -178,8 +193,11 @@ def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None
 detect_leaf_boundaries.jit = detect_leaf_boundaries
 
 
-def determine_znode_boundaries(posz: torch.Tensor, block_size: int = 64, nleaves=None):
-    """tree.py:264-284 -> FindNodeBoundaries.  Returns (lvl, lbound, rbound), each (len+1,)."""
+def determine_znode_boundaries(posz: torch.Tensor, block_size: int = 64, nleaves=None,
+                               set_id: torch.Tensor | None = None):
+    """tree.py:264-284 -> FindNodeBoundaries.  Returns (lvl, lbound, rbound), each (len+1,).
+
+    set_id (extension): set of every row of a batch; boundaries between sets get level LVL_SET."""
     size, dim = posz.shape
     dev = posz.device
     if nleaves is None:
@@ -188,8 +206,14 @@ def determine_znode_boundaries(posz: torch.Tensor, block_size: int = 64, nleaves
     info = LevelInfo(dim, posz.dtype)
     pos_bound = torch.full((2, dim), float("nan"), dtype=posz.dtype, device=dev)
     lvl, lb, rb = (torch.empty(size + 1, dtype=torch.int32, device=dev) for _ in range(3))
-    L.call("jzt_find_node_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(pos_bound), L.ptr(nl),
-           L.ptr(lvl), L.ptr(lb), L.ptr(rb), size + 1, dim, L.dtype_code(posz.dtype), info.max_lvl(), -2000)
+    if set_id is not None:
+        assert set_id.dtype == torch.int32 and len(set_id) == size
+        L.call("jzt_find_node_boundaries_sets", L.stream(), L.ptr(posz.contiguous()), L.ptr(set_id.contiguous()),
+               L.ptr(pos_bound), L.ptr(nl), L.ptr(lvl), L.ptr(lb), L.ptr(rb), size + 1, dim,
+               L.dtype_code(posz.dtype), -2000)
+    else:
+        L.call("jzt_find_node_boundaries", L.stream(), L.ptr(posz.contiguous()), L.ptr(pos_bound), L.ptr(nl),
+               L.ptr(lvl), L.ptr(lb), L.ptr(rb), size + 1, dim, L.dtype_code(posz.dtype), info.max_lvl(), -2000)
     return lvl, lb, rb
 
 
@@ -261,8 +285,10 @@ def _delete_rows(x: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
 
 
 def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None, num_types=None, lvl_bound=None,
-                            cfg_reg: RegularizationConfig | None = None):
-    """tree.py:628-734: leaf splits, then node activity per plane packed into PackedArrays."""
+                            cfg_reg: RegularizationConfig | None = None, set_id: torch.Tensor | None = None):
+    """tree.py:628-734: leaf splits, then node activity per plane packed into PackedArrays.
+
+    set_id (extension): batches of sets; boundaries between sets split every plane."""
     if ptype is not None:
         raise NotImplementedError("multi-type trees are not part of this build")
     n, dim = posz.shape
@@ -273,10 +299,12 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
         lvl_bound = (linfo.max_lvl(), linfo.max_lvl())
 
     ispl = detect_leaf_boundaries(posz, leaf_size=node_sizes[0], npart=npart, alloc_size=alloc_size,
-                                  lvl_bound=lvl_bound, cfg_reg=cfg_reg)
+                                  lvl_bound=lvl_bound, cfg_reg=cfg_reg, set_id=set_id)
     nleaves = torch.argmax(ispl).to(torch.int32)
-    reps = posz[torch.clamp(ispl[:-1].long(), max=n - 1)]
-    lvl, lbound, rbound = determine_znode_boundaries(reps, nleaves=nleaves)
+    first = torch.clamp(ispl[:-1].long(), max=n - 1)
+    reps = posz[first]
+    leaf_set = None if set_id is None else set_id[first].contiguous()
+    lvl, lbound, rbound = determine_znode_boundaries(reps, nleaves=nleaves, set_id=leaf_set)
     lbl, rbl = lbound.long(), rbound.long()
     npins = ispl[rbl] - ispl[lbl]
     ispl_per_type = ispl.reshape((1,) + ispl.shape)
@@ -285,6 +313,8 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     isl = npins[None, :] > sizes[:, None]
     isl = isl | ((lbound <= 0) & (lvl > lvl_bound[0]))[None, :]
     isl = isl | ((rbound >= nleaves) & (lvl > lvl_bound[1]))[None, :]
+    if set_id is not None:
+        isl = isl | (lvl >= LVL_SET)[None, :]
 
     if cfg_reg is not None:
         new = [(npins > 0) | isl[0]]
@@ -317,15 +347,23 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     ispl_n2n = _scatter_drop(alloc_size, offsets, value, torch.int32)
     fill_val = torch.cat([ispl[-1:], nnodes_on_level[:-1]])
     ispl_n2n = PackedArray.from_data(ispl_n2n, level_spl, fill_values=fill_val)
+    if set_id is not None:
+        return ispl, ispl_per_type, ispl_n2l, ispl_n2n, leaf_set
     return ispl, ispl_per_type, ispl_n2l, ispl_n2n
 
 
-def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None, num_types=None) -> TreeHierarchy:
+def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None, num_types=None,
+                         set_id: torch.Tensor | None = None, nsets: int = 1) -> TreeHierarchy:
     """Builds a tree hierarchy from z-order positions (tree.py:756-829).
 
     Plane 0 holds leaves of at most cfg_tree.max_leaf_size particles; plane p nodes hold at most
     max_leaf_size * coarse_fac**p particles.  Geometric centres only (mass-centred trees are not
-    part of this build)."""
+    part of this build).
+
+    set_id, nsets (extension): a batch of nsets independent sets, set_id (int32, non-decreasing)
+    naming the set of every row; the result is a forest whose nodes never span two sets, and
+    the top-plane interaction list pairs only nodes of one set.  Regularization is skipped for
+    batches (it never changes kNN or FoF results, only the leaf shapes)."""
     if cfg_tree.mass_centered:
         raise NotImplementedError("mass-centred trees are not part of this build")
     posz = get_pos(partz)
@@ -335,9 +373,15 @@ def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None
     node_sizes = _define_tree_level_node_sizes(npart_tot, cfg_tree)
     alloc_size = _estimate_node_number(np_per_dev, cfg_tree.max_leaf_size, cfg_tree.alloc_fac_nodes)
 
-    ispl_l2p, ispl_per_type, ispl_n2l, ispl_n2n = _define_split_hierarchy(
-        posz, node_sizes, alloc_size, npart=npart, ptype=ptype, num_types=num_types, lvl_bound=lvl_bound,
-        cfg_reg=cfg_tree.regularization)
+    if set_id is None:
+        ispl_l2p, ispl_per_type, ispl_n2l, ispl_n2n = _define_split_hierarchy(
+            posz, node_sizes, alloc_size, npart=npart, ptype=ptype, num_types=num_types, lvl_bound=lvl_bound,
+            cfg_reg=cfg_tree.regularization)
+        leaf_set = None
+    else:
+        ispl_l2p, ispl_per_type, ispl_n2l, ispl_n2n, leaf_set = _define_split_hierarchy(
+            posz, node_sizes, alloc_size, npart=npart, ptype=ptype, num_types=num_types, lvl_bound=lvl_bound,
+            cfg_reg=None, set_id=set_id)
 
     ispl_n2p = ispl_l2p[ispl_n2l.data.long()]
     lvl, geom_cent = get_node_geometry(posz, ispl_n2p[:-1], ispl_n2p[1:], num=ispl_n2l.nfilled() - 1,
@@ -349,7 +393,8 @@ def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None
     lvl = PackedArray.from_data(lvl, prop_array_spl, fill_values=-2000)
     geom_cent = PackedArray.from_data(geom_cent, prop_array_spl, fill_values=float("nan"))
     size_leaves = _estimate_node_number(np_per_dev, cfg_tree.max_leaf_size)
-    return TreeHierarchy(size_leaves, ispl_n2n, ispl_n2l, ispl_per_type, lvl, geom_cent)
+    return TreeHierarchy(size_leaves, ispl_n2n, ispl_n2l, ispl_per_type, lvl, geom_cent, leaf_set=leaf_set,
+                         nsets=int(nsets))
 
 
 build_tree_hierarchy.jit = build_tree_hierarchy
@@ -381,9 +426,13 @@ zsort_and_tree.jit = zsort_and_tree
 # ------------------------------------------------------------------------------------------------
 
 def grouped_dense_interaction_list(nnodes, size_ilist: int, ngroup: int = 32, size_super: int | None = None,
-                                   node_range=None, dtype=torch.int32, with_r2: bool = False):
+                                   node_range=None, dtype=torch.int32, with_r2: bool = False,
+                                   node_set: torch.Tensor | None = None, nsets: int = 1):
     """All-to-all interaction list over super-nodes of `ngroup` top-level nodes (tree.py:868-930),
     built on the device (jzt_dense_ilist); only the nsuper^2 live entries are written.
+
+    node_set, nsets (extension): set of every top node of a batch; super-nodes then never span
+    two sets and interact only within their set (jzt_dense_ilist_sets).
 
     Returns (spl_super, ilist, nsuper_nodes); ilist.rad2 = zeros when with_r2."""
     if node_range is not None:
@@ -398,9 +447,18 @@ def grouped_dense_interaction_list(nnodes, size_ilist: int, ngroup: int = 32, si
     ilist = torch.empty(size_ilist, dtype=torch.int32, device=dev)
     r2 = torch.empty(size_ilist, dtype=torch.float32, device=dev) if with_r2 else None
     nsuper = torch.empty(1, dtype=torch.int32, device=dev)
-    L.call("jzt_dense_ilist", L.stream(), L.ptr(nn), int(ngroup), L.ptr(spl_super), L.ptr(ispl), size_super,
-           L.ptr(ilist), L.ptr(r2), size_ilist, L.ptr(nsuper))
-    nint = nsuper[0].to(torch.int64) ** 2
+    if node_set is not None:
+        nb = L.LIB.jzt_dense_ilist_sets_tmp_bytes(int(nsets))
+        tmp = _tmp(nb, dev)
+        L.call("jzt_dense_ilist_sets", L.stream(), L.ptr(nn), L.ptr(node_set.to(torch.int32).contiguous()),
+               int(nsets), int(ngroup), L.ptr(spl_super), L.ptr(ispl), size_super, L.ptr(ilist), L.ptr(r2),
+               size_ilist, L.ptr(nsuper), L.ptr(tmp), nb)
+        # the list length itself: sum over sets of nsuper_s^2 (saturated at 2^31-1)
+        nint = tmp.view(torch.int64)[3 * int(nsets) + 2]
+    else:
+        L.call("jzt_dense_ilist", L.stream(), L.ptr(nn), int(ngroup), L.ptr(spl_super), L.ptr(ispl), size_super,
+               L.ptr(ilist), L.ptr(r2), size_ilist, L.ptr(nsuper))
+        nint = nsuper[0].to(torch.int64) ** 2
     raise_if(nint > size_ilist,
              "Cannot fit {n} interactions into interaction list with size {size}\nHint: Increase alloc_fac_ilist",
              n=nint, size=size_ilist)
@@ -408,3 +466,15 @@ def grouped_dense_interaction_list(nnodes, size_ilist: int, ngroup: int = 32, si
 
 
 grouped_dense_interaction_list.jit = grouped_dense_interaction_list
+
+
+def top_dense_interaction_list(th: TreeHierarchy, size_ilist: int, ngroup: int = 32, with_r2: bool = False):
+    """grouped_dense_interaction_list over the top plane of `th` (per set for a batch forest)."""
+    nplanes = th.num_planes()
+    size = th.size()
+    if th.leaf_set is None:
+        return grouped_dense_interaction_list(th.lvl.num(nplanes - 1), size_ilist, ngroup=ngroup, size_super=size,
+                                              with_r2=with_r2)
+    first_leaf = torch.clamp(th.ispl_n2l.get(nplanes - 1, size + 1)[:-1].long(), max=len(th.leaf_set) - 1)
+    return grouped_dense_interaction_list(th.lvl.num(nplanes - 1), size_ilist, ngroup=ngroup, size_super=size,
+                                          with_r2=with_r2, node_set=th.leaf_set[first_leaf], nsets=th.nsets)
