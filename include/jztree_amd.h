@@ -306,6 +306,30 @@ int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32
                       const int32_t* spl, const void* pos, const int32_t* part_igroup_in, int32_t* part_igroup,
                       int64_t size_leaves, int64_t size_part, float r2link, float boxsize, int dim, int dtype);
 
+/* FoF over a batch of sets, each with its own linking length (no counterpart in the reference;
+ * see jzt_pos_zorder_sort_segmented).  As the three calls above with the scalar r2link replaced
+ * by one float r2 per parent node (NodeToChildLabel, FofNode2Node) or per query leaf
+ * (FofLeaf2Leaf): the r2link of the set the node or leaf belongs to.  Every node of a parent's
+ * or leaf's interaction list lies in the same set, so each set is linked exactly as a
+ * single-set call with its own r2link links it. */
+int jzt_node_to_child_label_sets(jzt_stream_t stream, const int32_t* parent_igroup, const int32_t* parent_lvl,
+                                 const int32_t* parent_spl, const float* parent_r2link, int32_t* node_igroup,
+                                 int64_t size_parent, int64_t size_child, int dim, int dtype);
+int jzt_fof_node2node_sets(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist,
+                           const int32_t* parent_spl, const float* parent_r2link, const void* nodes,
+                           const int32_t* node_igroup_in, int32_t* node_igroup, int32_t* node_ilist_spl,
+                           int32_t* node_ilist, int64_t size_parent, int64_t size_node, int64_t parent_ilist_size,
+                           int64_t size_node_ilist, float boxsize, int dim, int dtype);
+int jzt_fof_leaf2leaf_sets(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist, int64_t ilist_size,
+                           const int32_t* spl, const float* leaf_r2link, const void* pos,
+                           const int32_t* part_igroup_in, int32_t* part_igroup, int64_t size_leaves,
+                           int64_t size_part, float boxsize, int dim, int dtype);
+
+/* jzt_graph_count_nodes: diagnostic, no counterpart in the reference.  counts[0..3] = kernel,
+ * memcpy, memset and other nodes of a HIP graph (hipGraph_t passed as void*).  Lets a caller that
+ * captures the pipeline check that it recorded no memset nodes. */
+int jzt_graph_count_nodes(void* graph, int64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

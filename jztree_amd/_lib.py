@@ -39,6 +39,7 @@ _SIGNATURES = {
     "jzt_where_flags": ([_p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _sz], _i32),
     "jzt_dense_ilist": ([_p, _p, _i32, _p, _p, _i64, _p, _p, _i64, _p], _i32),
     "jzt_dense_ilist_sets_tmp_bytes": ([_i64], _sz),
+    "jzt_graph_count_nodes": ([_p, _p], _i32),
     "jzt_dense_ilist_sets": ([_p, _p, _p, _i64, _i32, _p, _p, _i64, _p, _p, _i64, _p, _p, _sz], _i32),
     "jzt_regularization_tmp_bytes": ([_i32, _i32], _sz),
     "jzt_regularization_level": ([_p, _p, _p, _i64, _p, _i32, _i32, _f32, _f32, _p, _sz], _i32),
@@ -65,6 +66,10 @@ _SIGNATURES = {
     "jzt_node_to_child_label": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
     "jzt_fof_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
     "jzt_fof_leaf2leaf": ([_p, _p, _p, _i64, _p, _p, _p, _p, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
+    "jzt_node_to_child_label_sets": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32], _i32),
+    "jzt_fof_node2node_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _i32, _i32],
+                               _i32),
+    "jzt_fof_leaf2leaf_sets": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
 }
 
 

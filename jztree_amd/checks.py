@@ -29,15 +29,25 @@ def clear():
     _pending().clear()
 
 
+def take() -> list:
+    """Hand over the recorded conditions (and forget them here), e.g. to evaluate them after each
+    replay of a captured graph whose buffers they live in."""
+    items = list(_pending())
+    _pending().clear()
+    return items
+
+
 def flush():
     """Raise RuntimeError for the first recorded condition that holds; then forget all."""
-    pend = _pending()
-    if not pend:
+    evaluate(take())
+
+
+def evaluate(items: list):
+    """Raise RuntimeError for the first of the given recorded conditions that holds."""
+    if not items:
         return
-    conds = torch.stack([c.to(torch.bool) for c, _, _ in pend]).cpu()
-    bad = [i for i in range(len(pend)) if bool(conds[i])]
-    items = list(pend)
-    pend.clear()
+    conds = torch.stack([c.to(torch.bool) for c, _, _ in items]).cpu()
+    bad = [i for i in range(len(items)) if bool(conds[i])]
     if bad:
         _, msg, values = items[bad[0]]
         vals = {k: (v.item() if torch.is_tensor(v) else v) for k, v in values.items()}

@@ -85,14 +85,16 @@ template <int D, typename T> __device__ __forceinline__ T norm2(const Vec<D, T>&
 template <int D, typename T>
 __global__ __launch_bounds__(64) void k_node_to_child(const int32_t* __restrict__ pig, const uint8_t* __restrict__ local,
                                                       const int32_t* __restrict__ plvl, const int32_t* __restrict__ pspl,
-                                                      int32_t* __restrict__ cig, int64_t size_parent, float r2link, int64_t nitems) {
+                                                      int32_t* __restrict__ cig, int64_t size_parent, float r2link,
+                                                      const float* __restrict__ r2v, int64_t nitems) {
   const int64_t nact_ = active_items(pspl, nitems);
   for (int64_t node = blockIdx.x; node < nact_; node += gridDim.x) {
     if (node >= size_parent || (local && !local[node])) continue;
     const int root = pig[node];
     if (root >= size_parent || (local && !local[root])) continue;
     const T L2 = norm2<D, T>(lvl_ext<D, T>(plvl[node]));
-    const bool self = (root == node) && (L2 * (T)kSlackHi > (T)r2link);
+    const float r2 = r2v ? r2v[node] : r2link;
+    const bool self = (root == node) && (L2 * (T)kSlackHi > (T)r2);
     const int c0 = pspl[node], c1 = pspl[node + 1];
     const int lab = pspl[root];
     for (int i = c0 + threadIdx.x; i < c1; i += 64) cig[i] = self ? i : lab;
@@ -105,8 +107,8 @@ template <int PASS, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ nodes,
                                                  const int32_t* __restrict__ il_spl, int32_t* ig, int32_t* __restrict__ count,
-                                                 int32_t* __restrict__ il, float r2link, T boxsize, int64_t il_size,
-                                                 int64_t pil_size, int64_t nitems) {
+                                                 int32_t* __restrict__ il, float r2link, const float* __restrict__ r2v,
+                                                 T boxsize, int64_t il_size, int64_t pil_size, int64_t nitems) {
   __shared__ NodeX<D, T> s_n[64];
   __shared__ int s_g[64];
   const int lane = threadIdx.x;
@@ -114,7 +116,8 @@ __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil
   for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
     const int q0 = spl[pq], q1 = spl[pq + 1];
     const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
-    const T rl_hi = (T)r2link * (T)kSlackHi, rl_lo = (T)r2link * (T)kSlackLo;
+    const float r2 = r2v ? r2v[pq] : r2link;
+    const T rl_hi = (T)r2 * (T)kSlackHi, rl_lo = (T)r2 * (T)kSlackLo;
     for (int qoff = q0; qoff < q1; qoff += 64) {
       const int iq = min(qoff + lane, q1 - 1);
       const bool valid = qoff + lane < q1;
@@ -191,7 +194,8 @@ template <typename T> __device__ __forceinline__ T wave_max_t(T v) {
 template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
-                                                 float r2link, T boxsize, int64_t il_size, int64_t nitems) {
+                                                 float r2link_all, const float* __restrict__ r2v, T boxsize,
+                                                 int64_t il_size, int64_t nitems) {
   __shared__ FofRec<D, T> s_r[64];
   const int lane = threadIdx.x;
   const int64_t nact_ = active_items(spl, nitems);
@@ -199,6 +203,7 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
     const int q0 = spl[lq], q1 = spl[lq + 1];
     const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
     const T half = (T)(float)(T(0.5) * boxsize);
+    const float r2link = r2v ? r2v[lq] : r2link_all;
     const T r2skip = (T)(r2link * (1.f + kBoundSlack));
     for (int qoff = q0; qoff < q1; qoff += 64) {
       const int iq = min(qoff + lane, q1 - 1);
@@ -267,18 +272,18 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
 
 template <int D, typename T>
 int node2child_impl(hipStream_t s, const int32_t* pig, const uint8_t* local, const int32_t* plvl, const int32_t* pspl,
-                    int32_t* cig, int64_t size_parent, int64_t size_child, float r2link) {
+                    int32_t* cig, int64_t size_parent, int64_t size_child, float r2link, const float* r2v) {
   if (fill_u32(s, cig, 0u, size_child)) return 1;
   if (size_parent <= 0) return 0;
   hipLaunchKernelGGL((k_node_to_child<D, T>), dim3(grid_for(size_parent)), dim3(64), 0, s, pig, local, plvl, pspl, cig,
-                     size_parent, r2link, size_parent);
+                     size_parent, r2link, r2v, size_parent);
   return check_launch("NodeToChildLabel");
 }
 
 template <int D, typename T>
 int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, const int32_t* pspl, const void* nodes,
                   const int32_t* ig_in, int32_t* ig, int32_t* il_spl, int32_t* il, int64_t size_parent, int64_t size_node,
-                  int64_t pil_size, int64_t il_size, float r2link, float boxsize) {
+                  int64_t pil_size, int64_t il_size, float r2link, const float* r2v, float boxsize) {
   const T* nd = (const T*)nodes;
   if (fill_u32(s, il_spl, 0u, size_node + 1)) return 1;
   if (fill_u32(s, il, 0u, il_size)) return 1;
@@ -287,10 +292,10 @@ int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, con
   if (size_parent > 0) {
     dim3 g(grid_for(size_parent)), b(64);
     hipLaunchKernelGGL((k_fof_node<0, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
-                       (int32_t*)nullptr, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size, size_parent);
+                       (int32_t*)nullptr, (int32_t*)nullptr, r2link, r2v, (T)boxsize, il_size, pil_size, size_parent);
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_node, 256)), dim3(256), 0, s, ig, size_node);
     hipLaunchKernelGGL((k_fof_node<1, D, T>), g, b, 0, s, pil_spl, pil, pspl, nd, (const int32_t*)nullptr, ig,
-                       il_spl + 1, (int32_t*)nullptr, r2link, (T)boxsize, il_size, pil_size, size_parent);
+                       il_spl + 1, (int32_t*)nullptr, r2link, r2v, (T)boxsize, il_size, pil_size, size_parent);
     if (check_launch("FofNode2Node link/count")) return 1;
   }
   if (scan_tmp_words(size_node) > il_size) return fail("FofNode2Node: interaction list too small for scan temp");
@@ -298,7 +303,7 @@ int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, con
   if (fill_u32(s, il, 0u, scan_tmp_words(size_node))) return 1;
   if (size_parent > 0) {
     hipLaunchKernelGGL((k_fof_node<2, D, T>), dim3(grid_for(size_parent)), dim3(64), 0, s, pil_spl, pil, pspl, nd,
-                       il_spl, ig, (int32_t*)nullptr, il, r2link, (T)boxsize, il_size, pil_size, size_parent);
+                       il_spl, ig, (int32_t*)nullptr, il, r2link, r2v, (T)boxsize, il_size, pil_size, size_parent);
   }
   return check_launch("FofNode2Node insert");
 }
@@ -306,16 +311,16 @@ int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, con
 template <int D, typename T>
 int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64_t il_size, const int32_t* spl,
                   const void* pos, const int32_t* ig_in, int32_t* ig, int64_t size_leaves, int64_t size_part,
-                  float r2link, float boxsize) {
+                  float r2link, const float* r2v, float boxsize) {
   if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_part * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return fail("FofLeaf2Leaf: copy failed");
   if (size_leaves > 0) {
     if (boxsize > 0.f)
       hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, (T)boxsize, il_size, size_leaves);
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves);
     else
       hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, (T)boxsize, il_size, size_leaves);
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves);
   }
   if (size_part > 0)
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);
@@ -340,7 +345,15 @@ extern "C" int jzt_node_to_child_label(jzt_stream_t stream, const int32_t* paren
                                        const int32_t* parent_lvl, const int32_t* parent_spl, int32_t* node_igroup,
                                        int64_t size_parent, int64_t size_child, float r2link, int dim, int dtype) {
   JZT_DISPATCH2(node2child_impl, (hipStream_t)stream, parent_igroup, parent_is_local, parent_lvl, parent_spl, node_igroup,
-                size_parent, size_child, r2link);
+                size_parent, size_child, r2link, (const float*)nullptr);
+}
+
+extern "C" int jzt_node_to_child_label_sets(jzt_stream_t stream, const int32_t* parent_igroup, const int32_t* parent_lvl,
+                                            const int32_t* parent_spl, const float* parent_r2link, int32_t* node_igroup,
+                                            int64_t size_parent, int64_t size_child, int dim, int dtype) {
+  if (parent_r2link == nullptr) return fail("NodeToChildLabel (sets): parent_r2link is required");
+  JZT_DISPATCH2(node2child_impl, (hipStream_t)stream, parent_igroup, (const uint8_t*)nullptr, parent_lvl, parent_spl,
+                node_igroup, size_parent, size_child, 0.f, parent_r2link);
 }
 
 extern "C" int jzt_fof_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist,
@@ -350,7 +363,19 @@ extern "C" int jzt_fof_node2node(jzt_stream_t stream, const int32_t* parent_ilis
                                  float boxsize, int dim, int dtype) {
   JZT_DISPATCH2(fof_node_impl, (hipStream_t)stream, parent_ilist_spl, parent_ilist, parent_spl, nodes, node_igroup_in,
                 node_igroup, node_ilist_spl, node_ilist, size_parent, size_node, parent_ilist_size, size_node_ilist,
-                r2link, boxsize);
+                r2link, (const float*)nullptr, boxsize);
+}
+
+extern "C" int jzt_fof_node2node_sets(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist,
+                                      const int32_t* parent_spl, const float* parent_r2link, const void* nodes,
+                                      const int32_t* node_igroup_in, int32_t* node_igroup, int32_t* node_ilist_spl,
+                                      int32_t* node_ilist, int64_t size_parent, int64_t size_node,
+                                      int64_t parent_ilist_size, int64_t size_node_ilist, float boxsize, int dim,
+                                      int dtype) {
+  if (parent_r2link == nullptr) return fail("FofNode2Node (sets): parent_r2link is required");
+  JZT_DISPATCH2(fof_node_impl, (hipStream_t)stream, parent_ilist_spl, parent_ilist, parent_spl, nodes, node_igroup_in,
+                node_igroup, node_ilist_spl, node_ilist, size_parent, size_node, parent_ilist_size, size_node_ilist,
+                0.f, parent_r2link, boxsize);
 }
 
 extern "C" int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist,
@@ -358,5 +383,14 @@ extern "C" int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, 
                                  int32_t* part_igroup, int64_t size_leaves, int64_t size_part, float r2link,
                                  float boxsize, int dim, int dtype) {
   JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
-                size_leaves, size_part, r2link, boxsize);
+                size_leaves, size_part, r2link, (const float*)nullptr, boxsize);
+}
+
+extern "C" int jzt_fof_leaf2leaf_sets(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist,
+                                      int64_t ilist_size, const int32_t* spl, const float* leaf_r2link, const void* pos,
+                                      const int32_t* part_igroup_in, int32_t* part_igroup, int64_t size_leaves,
+                                      int64_t size_part, float boxsize, int dim, int dtype) {
+  if (leaf_r2link == nullptr) return fail("FofLeaf2Leaf (sets): leaf_r2link is required");
+  JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
+                size_leaves, size_part, 0.f, leaf_r2link, boxsize);
 }

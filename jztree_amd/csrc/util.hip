@@ -3,6 +3,8 @@
 #include <string>
 
 #include "util.h"
+
+#include <vector>
 #include "../../include/jztree_amd.h"
 
 namespace jzt {
@@ -276,4 +278,21 @@ extern "C" int jzt_dense_ilist_sets(jzt_stream_t stream, const int32_t* nnodes, 
   hipLaunchKernelGGL(k_dense_sets_fill, dim3(grid_for(div_up(work, 256))), dim3(256), 0, s, nnodes, nsets, ngroup, F, S,
                      I, spl_super, ispl, size_super, ilist, r2, size_ilist, nsuper);
   return check_launch("DenseIlistSets");
+}
+
+extern "C" int jzt_graph_count_nodes(void* graph, int64_t* counts) {
+  using namespace jzt;
+  if (graph == nullptr || counts == nullptr) return fail("GraphCountNodes: null argument");
+  for (int i = 0; i < 4; i++) counts[i] = 0;
+  size_t n = 0;
+  if (hipGraphGetNodes((hipGraph_t)graph, nullptr, &n) != hipSuccess) return fail("GraphCountNodes: hipGraphGetNodes failed");
+  std::vector<hipGraphNode_t> nodes(n);
+  if (n && hipGraphGetNodes((hipGraph_t)graph, nodes.data(), &n) != hipSuccess)
+    return fail("GraphCountNodes: hipGraphGetNodes failed");
+  for (size_t i = 0; i < n; i++) {
+    hipGraphNodeType t;
+    if (hipGraphNodeGetType(nodes[i], &t) != hipSuccess) return fail("GraphCountNodes: hipGraphNodeGetType failed");
+    counts[t == hipGraphNodeTypeKernel ? 0 : t == hipGraphNodeTypeMemcpy ? 1 : t == hipGraphNodeTypeMemset ? 2 : 3]++;
+  }
+  return 0;
 }

@@ -143,7 +143,8 @@ class PackedArray:
         if fill_value is None:
             fill = self.fill_values[level]
         else:
-            fill = torch.as_tensor(fill_value, dtype=self.data.dtype, device=self.data.device)
+            fill = (fill_value.to(device=self.data.device, dtype=self.data.dtype) if torch.is_tensor(fill_value)
+                    else torch.full((), fill_value, dtype=self.data.dtype, device=self.data.device))
         if self.data.ndim > 1:
             valid = valid.reshape((-1,) + (1,) * (self.data.ndim - 1))
         return torch.where(valid, vals, fill)
@@ -166,7 +167,10 @@ class PackedArray:
         fv = self.fill_values
         if fill_value is not None:
             fv = fv.clone()
-            fv[level] = torch.as_tensor(fill_value).to(fv.dtype)
+            if torch.is_tensor(fill_value):
+                fv[level] = fill_value.to(device=fv.device, dtype=fv.dtype)
+            else:
+                fv[level:level + 1].fill_(fill_value)  # a fill kernel, no host copy
         lf = torch.full((1,), level + 1, dtype=torch.int32, device=self.data.device)
         return PackedArray(new_data, new_spl, fv, lf)
 

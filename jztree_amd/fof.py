@@ -47,6 +47,12 @@ def _r2link(rlink: float) -> float:
     return float(np.float32(rlink * rlink))
 
 
+def _r2_items(r2: torch.Tensor, n: int) -> torch.Tensor:
+    """A float32 r2link per item, at least n entries (batches of sets pass these instead of rlink)."""
+    assert r2.dtype == torch.float32 and r2.numel() >= n
+    return r2.contiguous()
+
+
 def _fof_node2node(ilist: InteractionList, spl_parent, node_data: PosLvl, node_igroup, rlink: float,
                    boxsize: float = 0., block_size: int = 32):
     """fof.py:31-70 -> FofNode2Node.  Returns (node_igroup, node_ilist)."""
@@ -57,10 +63,17 @@ def _fof_node2node(ilist: InteractionList, spl_parent, node_data: PosLvl, node_i
     ig = node_igroup.to(torch.int32).contiguous()
     il_spl = torch.empty(size + 1, dtype=torch.int32, device=dev)
     il = torch.empty(ilist.size(), dtype=torch.int32, device=dev)
-    L.call("jzt_fof_node2node", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(spl_parent.contiguous()),
-           L.ptr(node_data.pos_lvl()), L.ptr(ig), L.ptr(ig), L.ptr(il_spl), L.ptr(il), spl_parent.shape[0] - 1,
-           size, ilist.size(), ilist.size(), _r2link(rlink), float(boxsize or 0.), node_data.pos.shape[-1],
-           L.dtype_code(node_data.pos.dtype))
+    if torch.is_tensor(rlink):  # r2link per parent node (batches of sets)
+        L.call("jzt_fof_node2node_sets", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc),
+               L.ptr(spl_parent.contiguous()), L.ptr(_r2_items(rlink, spl_parent.shape[0] - 1)),
+               L.ptr(node_data.pos_lvl()), L.ptr(ig), L.ptr(ig), L.ptr(il_spl), L.ptr(il), spl_parent.shape[0] - 1,
+               size, ilist.size(), ilist.size(), float(boxsize or 0.), node_data.pos.shape[-1],
+               L.dtype_code(node_data.pos.dtype))
+    else:
+        L.call("jzt_fof_node2node", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), L.ptr(spl_parent.contiguous()),
+               L.ptr(node_data.pos_lvl()), L.ptr(ig), L.ptr(ig), L.ptr(il_spl), L.ptr(il), spl_parent.shape[0] - 1,
+               size, ilist.size(), ilist.size(), _r2link(rlink), float(boxsize or 0.), node_data.pos.shape[-1],
+               L.dtype_code(node_data.pos.dtype))
     node_ilist = verify_ilist(InteractionList(il_spl, il))
     raise_if(node_ilist.nfilled() > il.numel(),
              "The interaction list allocation is too small. (need: {n1} have: {n2})\n"
@@ -77,19 +90,45 @@ def _node_to_child_label(igroup, lvl, spl, rlink: float, dim: int, num=None, siz
         size_child = igroup.numel()
     out = torch.empty(size_child, dtype=torch.int32, device=dev)
     loc = None if flag_local is None else flag_local.to(torch.uint8).contiguous()
-    L.call("jzt_node_to_child_label", L.stream(), L.ptr(igroup.to(torch.int32).contiguous()), L.ptr(loc),
-           L.ptr(lvl.to(torch.int32).contiguous()), L.ptr(spl.to(torch.int32).contiguous()), L.ptr(out),
-           igroup.numel(), size_child, _r2link(rlink), int(dim), L.dtype_code(dtype))
+    if torch.is_tensor(rlink):  # r2link per parent node (batches of sets)
+        assert flag_local is None
+        L.call("jzt_node_to_child_label_sets", L.stream(), L.ptr(igroup.to(torch.int32).contiguous()),
+               L.ptr(lvl.to(torch.int32).contiguous()), L.ptr(spl.to(torch.int32).contiguous()),
+               L.ptr(_r2_items(rlink, igroup.numel())), L.ptr(out), igroup.numel(), size_child, int(dim),
+               L.dtype_code(dtype))
+    else:
+        L.call("jzt_node_to_child_label", L.stream(), L.ptr(igroup.to(torch.int32).contiguous()), L.ptr(loc),
+               L.ptr(lvl.to(torch.int32).contiguous()), L.ptr(spl.to(torch.int32).contiguous()), L.ptr(out),
+               igroup.numel(), size_child, _r2link(rlink), int(dim), L.dtype_code(dtype))
     if num is not None:
         out = torch.where(torch.arange(size_child, device=dev) < num, out, torch.zeros_like(out))
     return out
 
 
-def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_fac_ilist: float = 128):
-    """fof.py:111-151: plane-by-plane node linking and interaction lists."""
+def _plane_node_sets(th: TreeHierarchy, spl_super: torch.Tensor) -> list:
+    """Set of every node slot on each plane of a batch forest, plus the top super-nodes last."""
+    size = th.size()
+    nl = len(th.leaf_set)
+    sets = []
+    for p in range(th.num_planes()):
+        first_leaf = torch.clamp(th.ispl_n2l.get(p, size + 1)[:-1].long(), max=nl - 1)
+        sets.append(th.leaf_set[first_leaf])
+    top = sets[-1]
+    sets.append(top[torch.clamp(spl_super[:-1].long(), max=len(top) - 1)])
+    return sets
+
+
+def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_fac_ilist: float = 128,
+                   r2link_set: torch.Tensor | None = None):
+    """fof.py:111-151: plane-by-plane node linking and interaction lists.
+
+    r2link_set (extension, batch forests): float32 r2link of every set, used in place of rlink."""
     nplanes = th.num_planes()
     size = th.size()
     spl, ilist, nsup = top_dense_interaction_list(th, int(th.size_leaves * alloc_fac_ilist))
+    if r2link_set is not None:
+        assert th.leaf_set is not None
+        r2_plane = [r2link_set[s.long()].contiguous() for s in _plane_node_sets(th, spl)]
     dev = spl.device
     igroup = torch.arange(size, dtype=torch.int32, device=dev)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
@@ -100,27 +139,37 @@ def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_f
     for i in range(nplanes):
         level = nplanes - 1 - i
         parent_spl = spl_n2n.get(level + 1, size + 1)
-        igroup = _node_to_child_label(igroup, nlvl.get(level + 1, size), parent_spl, num=th.num(level), rlink=rlink,
+        rl = rlink if r2link_set is None else r2_plane[level + 1]
+        igroup = _node_to_child_label(igroup, nlvl.get(level + 1, size), parent_spl, num=th.num(level), rlink=rl,
                                       dim=dim, dtype=th.geom_cent.data.dtype)
         node_data = PosLvl(pos=th.geom_cent.get(level, size), lvl=nlvl.get(level, size))
-        igroup, ilist = _fof_node2node(ilist, parent_spl, node_data, igroup, rlink=rlink, boxsize=boxsize)
+        igroup, ilist = _fof_node2node(ilist, parent_spl, node_data, igroup, rlink=rl, boxsize=boxsize)
     node_data = FofNodeData(nlvl.get(0, size), igroup, spl_n2n.get(0, size + 1))
     return node_data, ilist
 
 
 def _fof_leaf2leaf(leaf_data: FofNodeData, ilist: InteractionList, posz, rlink: float, boxsize: float = 0.,
-                   block_size=32):
-    """fof.py:153-169 -> NodeToChildLabel (leaf -> particle) + FofLeaf2Leaf."""
+                   block_size=32, leaf_r2link: torch.Tensor | None = None):
+    """fof.py:153-169 -> NodeToChildLabel (leaf -> particle) + FofLeaf2Leaf.
+
+    leaf_r2link (extension, batch forests): float32 r2link of every leaf slot, used in place of rlink."""
     assert len(leaf_data.spl) == len(ilist.ispl) == len(leaf_data.igroup) + 1 == len(leaf_data.lvl) + 1
     dev = posz.device
-    part_igroup = _node_to_child_label(leaf_data.igroup, leaf_data.lvl, leaf_data.spl, rlink=rlink,
+    rl = rlink if leaf_r2link is None else leaf_r2link
+    part_igroup = _node_to_child_label(leaf_data.igroup, leaf_data.lvl, leaf_data.spl, rlink=rl,
                                        size_child=len(posz), num=leaf_data.spl[-1], dim=posz.shape[-1],
                                        dtype=posz.dtype)
     out = torch.empty(len(posz), dtype=torch.int32, device=dev)
-    L.call("jzt_fof_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
-           L.ptr(leaf_data.spl.contiguous()),
-           L.ptr(posz.contiguous()), L.ptr(part_igroup), L.ptr(out), len(leaf_data.spl) - 1, len(posz),
-           _r2link(rlink), float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    if leaf_r2link is not None:
+        L.call("jzt_fof_leaf2leaf_sets", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
+               L.ptr(leaf_data.spl.contiguous()), L.ptr(_r2_items(leaf_r2link, len(leaf_data.spl) - 1)),
+               L.ptr(posz.contiguous()), L.ptr(part_igroup), L.ptr(out), len(leaf_data.spl) - 1, len(posz),
+               float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    else:
+        L.call("jzt_fof_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
+               L.ptr(leaf_data.spl.contiguous()),
+               L.ptr(posz.contiguous()), L.ptr(part_igroup), L.ptr(out), len(leaf_data.spl) - 1, len(posz),
+               _r2link(rlink), float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
     return out
 
 

@@ -26,10 +26,19 @@ LVL_SET = 1 << 16
 
 
 def _i32(x, device) -> torch.Tensor:
-    """A (1,) int32 device tensor holding x (python int or device scalar)."""
+    """A (1,) int32 device tensor holding x (python int or device scalar); written by a fill
+    kernel, never copied from the host, so that the pipeline can be captured as a HIP graph."""
     if torch.is_tensor(x):
         return x.reshape(1).to(device=device, dtype=torch.int32)
-    return torch.tensor([int(x)], dtype=torch.int32, device=device)
+    return torch.full((1,), int(x), dtype=torch.int32, device=device)
+
+
+def _i32_list(vals, device) -> torch.Tensor:
+    """int32 device tensor of python ints, filled on the device (capture-safe)."""
+    out = torch.empty(len(vals), dtype=torch.int32, device=device)
+    for i, v in enumerate(vals):
+        out[i:i + 1].fill_(int(v))
+    return out
 
 
 def _tmp(nbytes: int, device) -> torch.Tensor:
@@ -150,7 +159,8 @@ def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None
     linfo = LevelInfo(dim, posz.dtype)
     if lvl_bound is None:
         lvl_bound = (linfo.max_lvl(), linfo.max_lvl())
-    lvl_bound_t = torch.as_tensor(lvl_bound, dtype=torch.int32, device=dev).reshape(2)
+    lvl_bound_t = (lvl_bound.to(device=dev, dtype=torch.int32).reshape(2) if torch.is_tensor(lvl_bound)
+                   else _i32_list(lvl_bound, dev))
     npart_t = _i32(npart, dev)
     if ptype is None:
         num_types = 1
@@ -276,7 +286,7 @@ def _delete_rows(x: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
     m = idx.numel()
     mark = torch.zeros(n, dtype=torch.int64, device=x.device)
     # in range even when an (already recorded) allocation overflow made the indices meaningless
-    mark[torch.clamp(idx.long(), 0, n - 1)] = 1
+    mark.index_fill_(0, torch.clamp(idx.long(), 0, n - 1).reshape(-1), 1)  # scalar as kernel argument
     dest = torch.arange(n, device=x.device) - (torch.cumsum(mark, 0) - mark)
     dest = torch.where(mark.bool(), torch.full_like(dest, n - m), dest)
     out = torch.empty((n - m + 1,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)
@@ -300,7 +310,9 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
 
     ispl = detect_leaf_boundaries(posz, leaf_size=node_sizes[0], npart=npart, alloc_size=alloc_size,
                                   lvl_bound=lvl_bound, cfg_reg=cfg_reg, set_id=set_id)
-    nleaves = torch.argmax(ispl).to(torch.int32)
+    # first index of the largest split (= the leaf count); the splits never decrease, so a search
+    # finds it without a multi-block reduction (which would record a memset in a captured graph)
+    nleaves = torch.searchsorted(ispl, ispl[-1:]).reshape(()).to(torch.int32)
     first = torch.clamp(ispl[:-1].long(), max=n - 1)
     reps = posz[first]
     leaf_set = None if set_id is None else set_id[first].contiguous()
@@ -309,12 +321,16 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     npins = ispl[rbl] - ispl[lbl]
     ispl_per_type = ispl.reshape((1,) + ispl.shape)
 
-    sizes = torch.tensor(node_sizes, dtype=torch.int32, device=dev)
+    sizes = _i32_list(node_sizes, dev)
     isl = npins[None, :] > sizes[:, None]
     isl = isl | ((lbound <= 0) & (lvl > lvl_bound[0]))[None, :]
     isl = isl | ((rbound >= nleaves) & (lvl > lvl_bound[1]))[None, :]
     if set_id is not None:
-        isl = isl | (lvl >= LVL_SET)[None, :]
+        # set boundaries split every plane; so do the two ends of the array, which a single-set
+        # tree only splits through the size or level tests (both can miss a small end set, whose
+        # top-plane node would then never be closed)
+        ends = torch.arange(len(lvl), device=dev)
+        isl = isl | ((lvl >= LVL_SET) | (ends == 0) | (ends == nleaves))[None, :]
 
     if cfg_reg is not None:
         new = [(npins > 0) | isl[0]]

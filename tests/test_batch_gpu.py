@@ -81,6 +81,37 @@ def test_forest_never_spans_sets(dev):
         assert np.all(segh[a] == segh[b - 1]), f"a plane-{p} node spans two sets"
 
 
+def test_small_sets_at_both_ends(dev):
+    """Sets at the two ends of the batch smaller than a top-plane node (whose size follows the
+    total count) still get their own nodes on every plane: regression test."""
+    import jztree_amd as jz
+    cfg = jz.config.KNNConfig()
+    sizes = np.array([1100] + [1 << 19] * 7 + [1500])
+    top = jz.tree._define_tree_level_node_sizes(int(sizes.sum()), cfg.tree)[-1]
+    assert sizes[0] < top and sizes[-1] < top
+    parts = _batch(sizes, 70)
+    print(f"[ends] sizes={sizes.tolist()} top node size {top}", flush=True)
+    x = torch.from_numpy(np.concatenate(parts)).to(dev)
+    rl = [0.2 * float(s) ** (-1.0 / 3.0) for s in sizes]
+    rnn, inn, lab = jz.batch.pipeline_sets(x, sizes, 8, rl, boxsize=1.0)
+    posz, idz, seg, off, th = jz.batch.tree_sets(x, sizes, cfg.tree)
+    torch.cuda.synchronize()
+    ls = th.leaf_set.cpu().numpy()
+    for p in range(th.num_planes()):
+        m = int(th.num(p))
+        n2l = th.ispl_n2l.get(p, th.size() + 1).cpu().numpy()
+        assert n2l[0] == 0 and n2l[m] == int(th.num(0)), f"plane {p} does not cover every leaf"
+        assert len(np.unique(ls[n2l[:m]])) == len(sizes), f"plane {p} lost a set"
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for s in (0, len(sizes) - 1, 3):
+        a, b = off[s], off[s + 1]
+        xs = torch.from_numpy(parts[s]).to(dev)
+        r1, i1 = jz.knn.knn(xs, k=8, boxsize=1.0)
+        _, l1 = jz.fof.fof_labels(xs, rl[s], boxsize=1.0, output_order="input")
+        assert torch.equal(inn[a:b], i1) and torch.equal(rnn[a:b], r1), f"set {s}: kNN differs"
+        assert torch.equal(lab[a:b], l1), f"set {s}: labels differ"
+
+
 @pytest.mark.parametrize("k,box,kind", [(8, 1.0, "uniform"), (16, 0.0, "uniform"), (8, 0.0, "mixed")])
 def test_knn_sets_equals_single(dev, k, box, kind):
     import jztree_amd as jz
@@ -102,21 +133,58 @@ def test_knn_sets_equals_single(dev, k, box, kind):
         assert torch.equal(rnn[a:b], r1), f"set {s}: radii differ"
 
 
-@pytest.mark.parametrize("box", [1.0, 0.0])
-def test_fof_sets_equals_single(dev, box):
+@pytest.mark.parametrize("box,per_set", [(1.0, False), (0.0, False), (1.0, True), (0.0, True)])
+def test_fof_sets_equals_single(dev, box, per_set):
+    """One linking length for the batch, or b = 0.2 of each set's own mean spacing."""
     import jztree_amd as jz
     sizes = _sizes(24, 10, 14, 6)
     parts = _batch(sizes, 60)
-    rlink = 0.2 * float(np.mean(sizes)) ** (-1.0 / 3.0)
-    print(f"[fof_sets] box={box} nsets={len(sizes)} n={int(sizes.sum())} rlink={rlink}", flush=True)
+    if per_set:
+        rlink = [0.2 * float(s) ** (-1.0 / 3.0) for s in sizes]
+    else:
+        rlink = 0.2 * float(np.mean(sizes)) ** (-1.0 / 3.0)
+    print(f"[fof_sets] box={box} per_set={per_set} nsets={len(sizes)} n={int(sizes.sum())}", flush=True)
     x = torch.from_numpy(np.concatenate(parts)).to(dev)
     lab = jz.batch.fof_sets(x, sizes, rlink, boxsize=box)
     torch.cuda.synchronize()
     off = np.concatenate([[0], np.cumsum(sizes)])
     ngroups = 0
     for s, p in enumerate(parts):
-        _, l1 = jz.fof.fof_labels(torch.from_numpy(p).to(dev), rlink, boxsize=box, output_order="input")
+        rl = rlink[s] if per_set else rlink
+        _, l1 = jz.fof.fof_labels(torch.from_numpy(p).to(dev), rl, boxsize=box, output_order="input")
         a, b = off[s], off[s + 1]
         assert torch.equal(lab[a:b], l1), f"set {s}: labels differ"
         ngroups += int(torch.unique(l1).numel())
     assert ngroups < int(sizes.sum())  # some linking happened
+
+
+@pytest.mark.parametrize("big", [False, True])
+def test_pipeline_and_captured_graph(dev, big):
+    """One forest for kNN and FoF equals knn_sets/fof_sets; the HIP graph of the whole pipeline
+    records no memset node and every replay (new positions, eager work in between) equals the
+    eager result.  The big case has sets up to 2^19 points (multi-block paths everywhere)."""
+    import jztree_amd as jz
+    sizes = _sizes(8, 16, 19.5, 9) if big else _sizes(30, 10, 13, 8)
+    k = 8
+    rl = [0.2 * float(s) ** (-1.0 / 3.0) for s in sizes]
+    xs = [torch.from_numpy(np.concatenate(_batch(sizes, 100 * t))).to(dev) for t in range(3)]
+    print(f"[pipeline] nsets={len(sizes)} n={int(sizes.sum())}", flush=True)
+    rnn, inn, lab = jz.batch.pipeline_sets(xs[0], sizes, k, rl, boxsize=1.0)
+    r1, i1 = jz.batch.knn_sets(xs[0], sizes, k, boxsize=1.0)
+    l1 = jz.batch.fof_sets(xs[0], sizes, rl, boxsize=1.0)
+    assert torch.equal(inn, i1) and torch.equal(rnn, r1) and torch.equal(lab, l1)
+    print("[pipeline] capture", flush=True)
+    g = jz.batch.CapturedSets(xs[0], sizes, k, rl, boxsize=1.0)
+    print(f"[pipeline] graph nodes {g.node_counts}", flush=True)
+    assert g.memset_nodes() == 0
+    for rep in range(4):
+        x = xs[rep % 3]
+        out = g.run(x)
+        torch.cuda.synchronize()
+        g.check()
+        got = [t.clone() for t in out]
+        ref = jz.batch.pipeline_sets(x, sizes, k, rl, boxsize=1.0)
+        torch.cuda.synchronize()
+        print(f"[pipeline] replay {rep}", flush=True)
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), f"replay {rep} differs from the eager pipeline"
