@@ -475,6 +475,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZ
       for (int i = 0; i < K; i++) best[i] = start_key;
       int cnt = 0;  // entries in this lane's buffer row
       bool done = false;
+      // largest k-th distance of the wave (slack included); best[] changes only on flushes
+      float kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
       for (int eb = e0; eb < e1 && !done; eb += 64) {
         const int ne = min(64, e1 - eb);
         int m_t0 = 0, m_t1 = 0;
@@ -518,7 +520,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZ
           // the largest current k-th distance of the wave (or a periodic wrap is possible); the
           // kept records are compacted into LDS.  Exact: by monotone rounding every canonical r2
           // from the box is >= the gap^2 computed here with the same FMA chain.
-          const float kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
           bool need = false, reach = false;
           if (lane < n_cur) {
             float g2 = 0.f;
@@ -546,7 +547,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZ
               cr.id = src_ids ? (uint32_t)src_ids[t0 + lane] : (uint32_t)(t0 + lane);
             }
           }
-          consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_keep, wrap, boxsize, half, floor_key, row, cnt);
+          if (consume_chunk<K, PER, FLOOR, D>(best, q, s_rec, n_keep, wrap, boxsize, half, floor_key, row, cnt))
+            kmax = wave_maxf(key_r2(best[K - 1])) * kRmaxSlack;
           __syncthreads();
           i = ni;
           off = noff;

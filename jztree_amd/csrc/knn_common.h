@@ -78,11 +78,18 @@ template <int K> constexpr int kBufCap = K < JZT_TOPK_BUF ? K : JZT_TOPK_BUF;
 constexpr int kBufStride = 17;
 constexpr int kBufWords = 64 * kBufStride;  // u64 per wave
 
+// Keys compared as doubles: the high word of a key is the bit pattern of a non-negative float r2
+// (at most +inf, 0x7f800000), so read as an IEEE double every key is non-negative, never NaN or
+// infinite, and the double order equals the unsigned integer order (denormal doubles included,
+// which the FP64 mode keeps).  A compare-exchange is then one v_min_f64 and one v_max_f64
+// instead of a 64-bit compare and four 32-bit selects.
+__device__ __forceinline__ double key_as_f64(uint64_t k) { return __longlong_as_double((long long)k); }
+__device__ __forceinline__ uint64_t f64_as_key(double d) { return (uint64_t)__double_as_longlong(d); }
+
 __device__ __forceinline__ void ce_asc(uint64_t& a, uint64_t& b) {
-  const bool sw = b < a;
-  const uint64_t lo = sw ? b : a, hi = sw ? a : b;
-  a = lo;
-  b = hi;
+  const double x = key_as_f64(a), y = key_as_f64(b);
+  a = f64_as_key(__builtin_fmin(x, y));
+  b = f64_as_key(__builtin_fmax(x, y));
 }
 
 // ascending bitonic sort of a register array (N a power of two)
@@ -117,8 +124,7 @@ __device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* 
   // half-cleaner of best ++ reverse(b padded to K): the low half holds the K smallest, bitonic
 #pragma unroll
   for (int i = 0; i < M; i++) {
-    const uint64_t x = best[K - M + i], y = b[M - 1 - i];
-    best[K - M + i] = y < x ? y : x;
+    best[K - M + i] = f64_as_key(__builtin_fmin(key_as_f64(best[K - M + i]), key_as_f64(b[M - 1 - i])));
   }
   // bitonic merge to ascending order
 #pragma unroll
@@ -133,26 +139,48 @@ __device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* 
 
 // Stream n staged source particles into the lane's buffer (and the top-k on flushes).  `wrap`
 // (wave-uniform) applies the periodic wrap; the caller clears it only when no difference of the
-// chunk needs it.  row = this lane's buffer row, cnt = its fill.
+// chunk needs it.  row = this lane's buffer row, cnt = its fill (at most kBufCap on entry and on
+// return).  Sources go JZT_CONSUME_UNROLL at a time: their LDS reads are issued together and the
+// wave flushes before a group whenever some lane has less room than a whole group.  Returns
+// whether best[] changed (a flush happened).
+#ifndef JZT_CONSUME_UNROLL
+#define JZT_CONSUME_UNROLL 2
+#endif
 template <int K, bool PER, bool FLOOR, int D>
-__device__ __forceinline__ void consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
+__device__ __forceinline__ bool consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
                                               int n, bool wrap, float boxsize, float half, uint64_t floor_key,
                                               uint64_t* row, int& cnt) {
-  for (int j = 0; j < n; j++) {
-    const SrcRec c = s_rec[j];
-    Vec<D, float> p;
-#pragma unroll
-    for (int d = 0; d < D; d++) p[d] = c.p[d];
-    float r2;
-    if (PER && wrap) r2 = dist2P<true, D, float>(q, p, boxsize, half);
-    else r2 = dist2P<false, D, float>(q, p, boxsize, half);
-    const uint64_t key = ((uint64_t)__float_as_uint(r2) << 32) | c.id;
-    if (key < best[K - 1] && (!FLOOR || key > floor_key)) {
-      row[cnt] = key;
-      cnt++;
+  constexpr int U = JZT_CONSUME_UNROLL;
+  constexpr int CAP = kBufCap<K>;
+  static_assert(U >= 1 && U <= CAP, "unroll must fit a buffer row");
+  bool flushed = false;
+  for (int j = 0; j < n; j += U) {  // one flush site: the network is inlined once
+    if (wave_any(cnt > CAP - U)) {
+      topk_flush<K>(best, row, cnt);
+      flushed = true;
     }
-    if (wave_any(cnt >= kBufCap<K>)) topk_flush<K>(best, row, cnt);
+    SrcRec c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) c[u] = s_rec[j + u < n ? j + u : j];
+    float r2[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      Vec<D, float> p;
+#pragma unroll
+      for (int d = 0; d < D; d++) p[d] = c[u].p[d];
+      if (PER && wrap) r2[u] = dist2P<true, D, float>(q, p, boxsize, half);
+      else r2[u] = dist2P<false, D, float>(q, p, boxsize, half);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t key = ((uint64_t)__float_as_uint(r2[u]) << 32) | c[u].id;
+      if (j + u < n && key < best[K - 1] && (!FLOOR || key > floor_key)) {
+        row[cnt] = key;
+        cnt++;
+      }
+    }
   }
+  return flushed;
 }
 
 }  // namespace jzt
