@@ -161,6 +161,14 @@ __global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pi
   }
 }
 
+// One wave per parent node; its children are staged in LDS, and for every entry of the parent's
+// list the children of that entry are staged too.  Lanes take (child, target) pairs: with nt targets
+// staged, lane l tests child c0 + l / nt against target l % nt, so a parent with 8 children and an
+// entry with 8 targets keeps all 64 lanes busy (one lane per child would leave 56 idle).  A child's
+// hits in one step are lanes of one contiguous block: counted with a ballot, and in the insert pass
+// placed at the child's running offset plus the rank of the hit in its block.  The order inside a
+// node's list is therefore not the reference's, but every list is sorted by (r2, src) afterwards
+// (pairs are unique), so the lists are identical.
 template <int PASS, int D, typename T>
 __global__ __launch_bounds__(64) void k_count_insert(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src,
                                                      const float* __restrict__ pil_r2,
@@ -169,57 +177,72 @@ __global__ __launch_bounds__(64) void k_count_insert(const int32_t* __restrict__
                                                      const int32_t* __restrict__ out_spl, int32_t* __restrict__ out_src,
                                                      float* __restrict__ out_r2, int64_t nmax, T boxsize,
                                                      int64_t pil_size, int64_t nitems) {
-  __shared__ Vec<D, T> s_x[64];
-  __shared__ Vec<D, T> s_e[64];
+  __shared__ Vec<D, T> s_qc[64], s_qe[64], s_tc[64], s_te[64];
+  __shared__ float s_qr[64];
+  __shared__ int64_t s_run[64];  // per child: hits so far (count pass) or next output slot (insert pass)
   const int lane = threadIdx.x;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   const int64_t nact_ = active_items(parent_spl, nitems);
   for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
     const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
     const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
     for (int qoff = q0; qoff < q1; qoff += 64) {
-      const int iq = min(qoff + lane, q1 - 1);
-      const bool valid = qoff + lane < q1;
-      NodeT<D, T> nq = load_node<D, T>(nodes, iq);
-      Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
-      const float rq = rmax2[iq];
-      int64_t off = (PASS == 1) ? (int64_t)out_spl[iq] : 0;
-      int cnt = 0;
+      const int nc = min(q1 - qoff, 64);
+      float rq = -INFINITY;
+      if (lane < nc) {
+        const NodeT<D, T> nq = load_node<D, T>(nodes, qoff + lane);
+        s_qc[lane] = nq.cent;
+        s_qe[lane] = lvl_half_ext<D, T>(nq.level);
+        rq = rmax2[qoff + lane];
+        s_qr[lane] = rq;
+        s_run[lane] = (PASS == 1) ? (int64_t)out_spl[qoff + lane] : 0;
+      }
+      const float rq_max = wave_maxf(rq);
+      __syncthreads();
       for (int e = e0; e < e1; e++) {
-        const int pt = pil_src[e];
         const float r2t = pil_r2[e];
-        if (!wave_any(r2t <= rq)) break;
+        if (!(r2t <= rq_max)) break;  // entries ascend by r2; no child reaches this one or any later
+        const int pt = pil_src[e];
         const int t0 = parent_spl[pt], t1 = parent_spl[pt + 1];
         for (int toff = t0; toff < t1; toff += 64) {
-          const int it = toff + lane;
-          if (it < t1) {
-            NodeT<D, T> nt = load_node<D, T>(nodes, it);
-            s_x[lane] = nt.cent;
-            s_e[lane] = lvl_half_ext<D, T>(nt.level);
+          const int nt = min(t1 - toff, 64);
+          if (lane < nt) {
+            const NodeT<D, T> ntn = load_node<D, T>(nodes, toff + lane);
+            s_tc[lane] = ntn.cent;
+            s_te[lane] = lvl_half_ext<D, T>(ntn.level);
           }
           __syncthreads();
-          const int nl = min(t1 - toff, 64);
-          for (int j = 0; j < nl; j++) {
-            Vec<D, T> ew = add<D, T>(eq, s_e[j]);
-            float r2 = (float)mindist2<D, T>(s_x[j], nq.cent, ew, boxsize);
-            if (valid && r2 <= rq) {
-              if (PASS == 1) {
-                int64_t o = off + cnt;
-                if (o < nmax) {
-                  // touching nodes get a tiny positive radius so the node itself sorts first
-                  if (r2 == 0.f) r2 = 1e-10f * (float)maxdist2<D, T>(s_x[j], nq.cent, ew, boxsize);
-                  out_r2[o] = r2;
-                  out_src[o] = toff + j;
-                }
-              }
-              cnt++;
+          const int cstep = nt >= 32 ? 1 : 64 / nt;
+          const int my_c = lane / nt, my_t = lane - my_c * nt;
+          const uint64_t block = (nt == 64 ? ~0ull : ((1ull << nt) - 1ull)) << (my_c * nt);
+          for (int c0 = 0; c0 < nc; c0 += cstep) {
+            const int c = c0 + my_c;
+            const bool active = my_c < cstep && c < nc;
+            bool hit = false;
+            float r2 = 0.f;
+            if (active) {
+              const Vec<D, T> ew = add<D, T>(s_qe[c], s_te[my_t]);
+              r2 = (float)mindist2<D, T>(s_tc[my_t], s_qc[c], ew, boxsize);
+              hit = r2 <= s_qr[c];
+              if (PASS == 1 && hit && r2 == 0.f)  // touching nodes get a tiny radius: the node itself sorts first
+                r2 = 1e-10f * (float)maxdist2<D, T>(s_tc[my_t], s_qc[c], ew, boxsize);
             }
+            const uint64_t hits = __ballot(hit) & block;
+            if (PASS == 1 && hit) {
+              const int64_t o = s_run[c] + __popcll(hits & lt_mask);
+              if (o < nmax) {
+                out_r2[o] = r2;
+                out_src[o] = toff + my_t;
+              }
+            }
+            if (active && my_t == 0 && hits) s_run[c] += __popcll(hits);
           }
           __syncthreads();
         }
       }
-      if (PASS == 0 && valid) count_out[iq] = cnt;
+      if (PASS == 0 && lane < nc) count_out[qoff + lane] = (int32_t)s_run[lane];
+      __syncthreads();
     }
-  
   }
 }
 

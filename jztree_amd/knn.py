@@ -266,17 +266,18 @@ def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0.
 
 def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
                  alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None,
-                 grow: float = 16.0, stats: dict | None = None):
+                 grow: float | None = None, stats: dict | None = None):
     """Exact kNN from a proposed search radius^2 per leaf (extension; k <= 64).
 
     Round 1 walks the tree with the proposed radii on every plane (a node searches as far as its
     farthest-reaching child; each radius clipped to the guaranteed one, _guaranteed_leaf_r2) and
     runs the leaf kernel with each leaf's radius as starting bound.  Queries that find fewer than k
-    neighbours inside it are flagged; round 2 redoes their leaves with the radius^2 grown by `grow`
-    (still clipped) while every other leaf sits out, round 3 the leaves still short with the
-    guaranteed radius.  Whatever is left -- or everything, should a round's lists overflow their
-    allocation -- is recomputed without any proposal: by the fused retry pass on guaranteed-radius
-    lists, or by the plain walk when round 1 itself overflowed.
+    neighbours inside it are flagged; the next round redoes their leaves with the guaranteed radius
+    while every other leaf sits out (with `grow`, a round with the radius^2 grown by that factor,
+    still clipped, comes first: each round walks the tree once, so an extra round only pays when
+    most short leaves are close misses).  Whatever is left -- or everything, should a round's lists
+    overflow their allocation -- is recomputed without any proposal: by the fused retry pass on
+    guaranteed-radius lists, or by the plain walk when round 1 itself overflowed.
     The result is therefore independent of the proposal: identical to knn() for any leaf_r2.
     Returns (rnn, inn, retry flags of round 1)."""
     assert k <= 64, "a start bound needs k <= 64"
@@ -317,10 +318,11 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     if int(h[1]) == 0:
         return rnn, inn, first
     # Later rounds: the leaves holding a flagged query (segment sums of the flags over the leaf
-    # splits) with the radius grown by `grow`, then with the guaranteed radius; each round's lists
-    # and flags are read back once.
+    # splits), optionally with the radius grown by `grow`, then with the guaranteed radius; each
+    # round's lists and flags are read back once.
     sl = spl.long().clamp(0, n)
-    for rnd, r2_next in enumerate((lambda: torch.minimum(r2 * grow, cap), lambda: cap), start=2):
+    later = ((lambda: torch.minimum(r2 * grow, cap)),) if grow else ()
+    for rnd, r2_next in enumerate(later + ((lambda: cap),), start=2):
         cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(retry, 0)])
         short = (cs[sl[1:]] - cs[sl[:-1]]) > 0
         ovf = run(torch.where(short, r2_next(), torch.full_like(r2, -1.0)).contiguous())
@@ -344,6 +346,28 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
            L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
            int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
     return rnn, inn, first
+
+
+def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, leaf_r2, src_ids=None,
+                          dst_rows=None):
+    """Leaf kernel over the guaranteed-radius lists with each leaf's guaranteed k-th radius^2
+    (from KnnNode2Node, widened by 2^-10) as the start bound of its queries (extension, k <= 64):
+    candidates beyond it are never buffered, so the top-k fills with fewer flushes.  The lists hold
+    every leaf within that radius, so a query can only come up short if the float bound rounds
+    below its k-th neighbour; such leaves are redone without a bound.  Results equal the
+    unbounded kernel's."""
+    n = len(posz)
+    size = len(spl) - 1
+    init = (leaf_r2.to(torch.float64) * (1.0 + 2.0 ** -10)).to(torch.float32).contiguous()
+    rnn, inn, retry = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
+                                     dst_rows=dst_rows, init_r2=init)
+    if bool((retry != 0).any()):
+        short = torch.zeros(size + 1, dtype=torch.bool, device=posz.device)
+        short[inverse_of_splits(spl, n).long()[retry != 0]] = True
+        redo = torch.where(short[:size], torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
+        _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
+                       dst_rows=dst_rows, init_r2=redo, out=(rnn, inn, retry))
+    return rnn, inn
 
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,
@@ -386,6 +410,11 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
             # neighbours are reported (and tie-broken) by their origin index directly
             rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                         src_ids=originz.idx, dst_rows=originz.idx if direct else None)
+        elif k <= 64:
+            ilist, leaf_r2, _ = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
+                                               return_parents=True)
+            rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx,
+                                                originz.idx if direct else None)
         else:
             ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
             rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,

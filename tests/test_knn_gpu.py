@@ -170,9 +170,10 @@ def test_bounded_matches_plain(dev, oracle, k, boxsize):
     exact = _leaf_max_r2k(jz, th, rz, k)
     mixed = exact.clone()
     mixed[::3] = 0.0
-    for fac, bound in [(0.25, exact * 0.25), (1.0, exact * (1 + 1e-6)), (3.0, exact * 3.0), ("mixed", mixed)]:
+    for fac, bound, grow in [(0.25, exact * 0.25, None), (0.25, exact * 0.25, 16.0), (1.0, exact * (1 + 1e-6), None),
+                             (3.0, exact * 3.0, None), ("mixed", mixed, None)]:
         st = {}
-        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize,
+        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize, grow=grow,
                                             alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx, stats=st)
         print(f"[bounded] k={k} box={boxsize} factor {fac}: {st}", flush=True)
         jz.checks.flush()
@@ -189,3 +190,26 @@ def test_bounded_matches_plain(dev, oracle, k, boxsize):
     rnn = torch.empty_like(rz)
     rnn[originz.idx.long()] = rz
     _check(x, k, rnn.cpu().numpy(), inn.cpu().numpy(), oracle, boxsize=boxsize)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,boxsize", [(16, 1.0), (32, 0.0)])
+def test_start_bound_fallback(dev, k, boxsize):
+    """The plain search starts every query at its leaf's guaranteed radius; a start bound that
+    rounds too small (forced here by shrinking it) sends the short leaves through the unbounded
+    pass, and the result equals the unbounded kernel's."""
+    import jztree_amd as jz
+    x = clustered(1 << 16, nhalos=64, seed=23)
+    t = torch.from_numpy(x).to(dev)
+    cfg = _clustered_cfg()
+    partz, originz, th = jz.tree.zsort_and_tree(t, cfg.tree, data=jz.data.RankIdx(
+        rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev)))
+    il, leaf_r2, _ = jz.knn._knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
+                                           return_parents=True)
+    spl = th.splits_leaf_to_part()
+    rz, iz = jz.knn._knn_leaf2leaf(il, spl, partz, k=k, boxsize=boxsize, src_ids=originz.idx)
+    for fac in (1.0, 0.3):
+        print(f"[start bound] k={k} box={boxsize} factor {fac}", flush=True)
+        r1, i1 = jz.knn._knn_leaf_start_bound(il, spl, partz, k, boxsize, leaf_r2 * fac, originz.idx)
+        torch.cuda.synchronize()
+        assert torch.equal(i1, iz) and torch.equal(r1, rz), f"factor {fac}"
