@@ -43,6 +43,18 @@ enum { JZT_F32 = 0, JZT_F64 = 1 };
 const char* jzt_last_error(void);
 int jzt_abi_version(void);
 
+/* Tuning knobs (no counterpart in the reference, which has no autotuner): process-wide settings
+ * read at each launch; results never depend on them.  Set them between calls, not while another
+ * host thread launches.
+ *   JZT_TUNE_LEAF_WAVES  waves per EU the leaf kNN kernel is compiled for when k <= 32 (1, 2, 4;
+ *                        default 4): the register budget per lane against occupancy
+ *   JZT_TUNE_SORT_KPT    keys per thread of the onesweep radix passes (8, 12, 16; default 12):
+ *                        tile = 256 * kpt keys */
+#define JZT_TUNE_LEAF_WAVES 0
+#define JZT_TUNE_SORT_KPT 1
+int jzt_set_tuning(int knob, int value);
+int jzt_get_tuning(int knob);
+
 /* ------------------------------------------------------------------------------------------ */
 /* sort                                                                                         */
 /* ------------------------------------------------------------------------------------------ */

@@ -25,6 +25,8 @@ _sz = _c.c_size_t
 _SIGNATURES = {
     "jzt_last_error": ([], _c.c_char_p),
     "jzt_abi_version": ([], _i32),
+    "jzt_set_tuning": ([_i32, _i32], _i32),
+    "jzt_get_tuning": ([_i32], _i32),
     "jzt_zsort_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_pos_zorder_sort": ([_p, _p, _p, _p, _p, _sz, _i64, _i32, _i32], _i32),
     "jzt_zsort_segmented_tmp_bytes": ([_i64, _i64, _i32, _i32], _sz),

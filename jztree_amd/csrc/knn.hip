@@ -459,8 +459,8 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // than K real neighbours is flagged in `retry` (its row holds the sentinel entries); every other
 // query's row is exact, because all its K neighbours lie within r2_0 and the list holds every
 // leaf within r2_0 (the caller's contract).
-template <int K, bool PER, bool FLOOR, int D>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? JZT_LEAF_WAVES : 1))) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
+template <int K, bool PER, bool FLOOR, int D, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_knn_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il_src,
                                                  const float* __restrict__ il_r2, const int32_t* __restrict__ splT,
                                                  const float* __restrict__ xT, const int32_t* __restrict__ splQ,
                                                  const float* __restrict__ xQ, const int32_t* __restrict__ src_ids,
@@ -673,27 +673,38 @@ int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, 
   dim3 g(grid_for(nleaves_q)), b(64);
   const float* xt = (const float*)xT;
   const float* xq = (const float*)xQ;
+  const int waves = tune_get(kTuneLeafWaves);
   for (int koff = 0; koff < k; koff += 64) {
     const int kn = min(k - koff, 64);
     const bool floor = resume && koff > 0;
-#define JZT_KNN_LAUNCH(KK, PER, FLOOR)                                                                              \
-  hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
+#define JZT_KNN_LAUNCH(KK, PER, FLOOR, W)                                                                              \
+  hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D, W>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
                      dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q)
-#define JZT_KNN_LAUNCH_K(KK)                                      \
-  do {                                                            \
-    if (boxsize > 0.f) {                                          \
-      if (floor) JZT_KNN_LAUNCH(KK, true, true);                  \
-      else JZT_KNN_LAUNCH(KK, true, false);                       \
-    } else {                                                      \
-      if (floor) JZT_KNN_LAUNCH(KK, false, true);                 \
-      else JZT_KNN_LAUNCH(KK, false, false);                      \
-    }                                                             \
+  // waves per EU (the register budget of the kernel): tunable for K <= 32; the multi-pass tail
+  // (FLOOR) and K = 64 keep their fixed choice
+#define JZT_KNN_LAUNCH_W(KK, PER)                                                  \
+  do {                                                                             \
+    constexpr int w2 = KK > 32 ? 1 : 2, w4 = KK > 32 ? 1 : JZT_LEAF_WAVES;         \
+    if (KK > 32 || waves == 1) JZT_KNN_LAUNCH(KK, PER, false, 1);                  \
+    else if (waves == 2) JZT_KNN_LAUNCH(KK, PER, false, w2);                       \
+    else JZT_KNN_LAUNCH(KK, PER, false, w4);                                       \
+  } while (0)
+#define JZT_KNN_LAUNCH_K(KK)                                                            \
+  do {                                                                                  \
+    if (boxsize > 0.f) {                                                                \
+      if (floor) JZT_KNN_LAUNCH(KK, true, true, (KK > 32 ? 1 : JZT_LEAF_WAVES));        \
+      else JZT_KNN_LAUNCH_W(KK, true);                                                  \
+    } else {                                                                            \
+      if (floor) JZT_KNN_LAUNCH(KK, false, true, (KK > 32 ? 1 : JZT_LEAF_WAVES));       \
+      else JZT_KNN_LAUNCH_W(KK, false);                                                 \
+    }                                                                                   \
   } while (0)
     if (kn <= 8) JZT_KNN_LAUNCH_K(8);
     else if (kn <= 16) JZT_KNN_LAUNCH_K(16);
     else if (kn <= 32) JZT_KNN_LAUNCH_K(32);
     else JZT_KNN_LAUNCH_K(64);
 #undef JZT_KNN_LAUNCH_K
+#undef JZT_KNN_LAUNCH_W
 #undef JZT_KNN_LAUNCH
     if (check_launch("KnnLeaf2Leaf")) return 1;
   }

@@ -30,8 +30,8 @@ namespace {
 
 constexpr int kSortThreads = 256;  // 4 waves
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kKPT = 12;                          // keys per thread
-constexpr int kTile = kSortThreads * kKPT;        // 3072 keys per tile
+constexpr int kKPTMin = 8;                        // keys per thread: 8, 12 (default) or 16, tuned at run
+                                                  // time; look-back state is sized for the smallest tile
 constexpr int kRadix = 256;
 constexpr int kPasses = 8;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kCountMask = (1u << 30) - 1;
@@ -254,6 +254,7 @@ __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
 // ---------------------------------------------------------------------------------------------
 // 4. onesweep pass
 // ---------------------------------------------------------------------------------------------
+template <int KPT>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -263,8 +264,8 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
   __shared__ uint32_t s_lstart[kRadix];
   __shared__ uint32_t s_gbase[kRadix];
-  __shared__ uint64_t s_keys[kTile];
-  __shared__ uint8_t s_dig[kTile];
+  __shared__ uint64_t s_keys[(kSortThreads * KPT)];
+  __shared__ uint8_t s_dig[(kSortThreads * KPT)];
   __shared__ int s_scan[kSortThreads / 64];
   __shared__ uint32_t s_tile;
 
@@ -273,21 +274,21 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   for (int i = tid; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
-  const int64_t base = (int64_t)tile * kTile;
-  const int64_t wbase = base + (int64_t)w * 64 * kKPT;
+  const int64_t base = (int64_t)tile * (kSortThreads * KPT);
+  const int64_t wbase = base + (int64_t)w * 64 * KPT;
 
-  uint64_t key[kKPT];
-  uint32_t dig[kKPT];
-  uint32_t rank[kKPT];
+  uint64_t key[KPT];
+  uint32_t dig[KPT];
+  uint32_t rank[KPT];
 #pragma unroll
-  for (int r = 0; r < kKPT; r++) {
+  for (int r = 0; r < KPT; r++) {
     int64_t idx = wbase + r * 64 + lane;
     key[r] = idx < n ? kin[idx] : ~0ull;
     dig[r] = (uint32_t)(key[r] >> shift) & 255u;
   }
   const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int r = 0; r < kKPT; r++) {
+  for (int r = 0; r < KPT; r++) {
     uint32_t d = dig[r];
     uint64_t m = ~0ull;
 #pragma unroll
@@ -338,14 +339,14 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
 
   // scatter keys into LDS in tile-local digit order
 #pragma unroll
-  for (int r = 0; r < kKPT; r++) {
+  for (int r = 0; r < KPT; r++) {
     uint32_t dd = dig[r];
     uint32_t p = s_lstart[dd] + s_wcnt[w][dd] + rank[r];
     s_keys[p] = key[r];
     s_dig[p] = (uint8_t)dd;
   }
   __syncthreads();
-  const int tile_n = (int)((n - base) < kTile ? (n - base) : kTile);
+  const int tile_n = (int)((n - base) < (kSortThreads * KPT) ? (n - base) : (kSortThreads * KPT));
   for (int i = tid; i < tile_n; i += kSortThreads) {
     const uint32_t g = s_gbase[s_dig[i]] + (uint32_t)i;
     if (g < (uint64_t)n) kout[g] = s_keys[i];
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   __syncthreads();
   uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
-  for (int r = 0; r < kKPT; r++) {
+  for (int r = 0; r < KPT; r++) {
     int64_t idx = wbase + r * 64 + lane;
     uint32_t v = idx < n ? vin[idx] : 0u;
     uint32_t dd = dig[r];
@@ -464,6 +465,22 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ pos, const
   id_out[i] = (typename IntOf<T>::type)v;
 }
 
+// one onesweep pass with the tuned tile (keys per thread); look-back slots are `stride` tiles apart
+int onesweep(hipStream_t s, const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, int64_t n, int shift,
+             const uint32_t* bucket, uint32_t* lookback, uint32_t* counter) {
+  const int kpt = tune_get(kTuneSortKpt);
+  if (kpt == 8)
+    hipLaunchKernelGGL(k_onesweep<8>, dim3((unsigned)div_up(n, kSortThreads * 8)), dim3(kSortThreads), 0, s, ki, vi, ko,
+                       vo, n, shift, bucket, lookback, counter);
+  else if (kpt == 16)
+    hipLaunchKernelGGL(k_onesweep<16>, dim3((unsigned)div_up(n, kSortThreads * 16)), dim3(kSortThreads), 0, s, ki, vi,
+                       ko, vo, n, shift, bucket, lookback, counter);
+  else
+    hipLaunchKernelGGL(k_onesweep<12>, dim3((unsigned)div_up(n, kSortThreads * 12)), dim3(kSortThreads), 0, s, ki, vi,
+                       ko, vo, n, shift, bucket, lookback, counter);
+  return 0;
+}
+
 struct SortLayout {
   size_t keys0, vals0, keys1, vals1, hist, meta, lookback, poscopy, runflags, runtmp, runs, nruns, total;
   int64_t ntiles;
@@ -474,7 +491,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 SortLayout sort_layout(int64_t n, int dim, int dtype) {
   SortLayout L;
   size_t o = 0;
-  L.ntiles = div_up(n > 0 ? n : 1, kTile);
+  L.ntiles = div_up(n > 0 ? n : 1, kSortThreads * kKPTMin);  // look-back stride of every pass
   L.keys0 = o; o = align256(o + 8 * (size_t)n);
   L.vals0 = o; o = align256(o + 4 * (size_t)n);
   L.keys1 = o; o = align256(o + 8 * (size_t)n);
@@ -523,7 +540,7 @@ int zsort_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::ty
     const uint32_t* vi = (p & 1) ? v1 : v0;
     uint64_t* ko = (p & 1) ? k0 : k1;
     uint32_t* vo = (p & 1) ? v0 : v1;
-    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n, 8 * p,
+    onesweep(s, ki, vi, ko, vo, n, 8 * p,
                        hist + p * kRadix, lookback + (int64_t)p * L.ntiles * kRadix, meta + 8 + p);
   }
   if (check_launch("zsort onesweep")) return 1;
@@ -743,7 +760,7 @@ int zsort_seg_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>
     const uint32_t* vi = (p & 1) ? v1 : v0;
     uint64_t* ko = (p & 1) ? k0 : k1;
     uint32_t* vo = (p & 1) ? v0 : v1;
-    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo, n, 8 * p,
+    onesweep(s, ki, vi, ko, vo, n, 8 * p,
                        hist + p * kRadix, lookback + (int64_t)p * L.ntiles * kRadix, meta + 8 + p);
   }
   if (check_launch("zsort_segmented onesweep")) return 1;
@@ -759,11 +776,11 @@ int zsort_seg_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>
   hipLaunchKernelGGL(k_set_keys, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, v0, setid, n, k1);
   const uint32_t* vfinal = v0;
   {
-    hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, k1, v0, k0, v1, n, 0, hist,
+    onesweep(s, k1, v0, k0, v1, n, 0, hist,
                        lookback, meta + 8);
     vfinal = v1;
     if (npass_set == 2) {
-      hipLaunchKernelGGL(k_onesweep, dim3((unsigned)L.ntiles), dim3(kSortThreads), 0, s, k0, v1, k1, v0, n, 8,
+      onesweep(s, k0, v1, k1, v0, n, 8,
                          hist + kRadix, lookback + (int64_t)L.ntiles * kRadix, meta + 9);
       vfinal = v0;
     }

@@ -17,6 +17,11 @@ inline int64_t scan_tmp_words(int64_t n) { return div_up(n, kScanTile) + 1; }
 // tmp must hold scan_tmp_words(n) int32.
 int scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, bool inclusive, int32_t* tmp);
 
+// process-wide tuning knobs (set through jzt_set_tuning; the values the kernels were written for
+// are the defaults)
+enum TuneKnob { kTuneLeafWaves = 0, kTuneSortKpt = 1, kTuneCount = 2 };
+int tune_get(int knob);
+
 // fill n 32-bit words with v
 int fill_u32(hipStream_t s, void* p, uint32_t v, int64_t n);
 

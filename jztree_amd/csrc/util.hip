@@ -243,7 +243,23 @@ __global__ void k_dense_sets_fill(const int32_t* __restrict__ nn_dev, int64_t ns
   if (tid == 0) nsuper_out[0] = (int32_t)nsup;
 }
 
+static int g_tune[kTuneCount] = {4, 12};  // leaf kNN waves per EU (K <= 32), onesweep keys per thread
+int tune_get(int knob) { return (knob >= 0 && knob < kTuneCount) ? __atomic_load_n(&g_tune[knob], __ATOMIC_RELAXED) : 0; }
+
 }  // namespace jzt
+
+extern "C" int jzt_set_tuning(int knob, int value) {
+  using namespace jzt;
+  bool ok = false;
+  if (knob == kTuneLeafWaves) ok = value == 1 || value == 2 || value == 4;
+  else if (knob == kTuneSortKpt) ok = value == 8 || value == 12 || value == 16;
+  else return fail("jzt_set_tuning: unknown knob " + std::to_string(knob));
+  if (!ok) return fail("jzt_set_tuning: value " + std::to_string(value) + " not allowed for knob " + std::to_string(knob));
+  __atomic_store_n(&g_tune[knob], value, __ATOMIC_RELAXED);
+  return 0;
+}
+
+extern "C" int jzt_get_tuning(int knob) { return jzt::tune_get(knob); }
 
 extern "C" const char* jzt_last_error(void) { return jzt::g_err.c_str(); }
 extern "C" int jzt_abi_version(void) { return JZT_ABI_VERSION; }

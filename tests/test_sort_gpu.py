@@ -85,3 +85,25 @@ def test_zsort_gradient(dev):
     g = torch.empty_like(w)
     g[idz.long()] = w
     assert torch.equal(x.grad, g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kpt", [8, 16])
+def test_zsort_tile_sizes(dev, oracle, kpt):
+    """Every onesweep tile (keys per thread) sorts exactly as the default one and the oracle."""
+    import jztree_amd as jz
+    from gen import clustered
+    x = clustered(200_003, nhalos=50, seed=9)
+    x[:1000] = x[1000:2000]  # equal keys
+    t = torch.from_numpy(x).to(dev)
+    before = jz.autotune.get_knobs()
+    try:
+        jz.autotune.set_knobs(sort_kpt=kpt)
+        print(f"[zsort tile] kpt={kpt}", flush=True)
+        pz, iz = jz.tree.zsort(t)
+        torch.cuda.synchronize()
+    finally:
+        jz.autotune.set_knobs(**before)
+    p0, i0 = jz.tree.zsort(t)
+    assert torch.equal(iz, i0) and torch.equal(pz, p0)
+    assert np.array_equal(iz.cpu().numpy().astype(np.int64), oracle.zsort(x)[1])
