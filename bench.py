@@ -60,6 +60,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--batch-sets", type=int, default=256, help="sets of the batched workload (0 = skip it)")
     p.add_argument("--batch-k", type=int, default=8)
+    p.add_argument("--no-tune", action="store_true", help="skip the autotuner (fixed leaf 128/64, waves 4, tile 16)")
     return p.parse_args()
 
 
@@ -94,9 +95,11 @@ def bench_batch(jz, torch, dev, nsets, k, steps, warmup, ev):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / reps
 
-    eager = lambda: jz.batch.pipeline_sets(x, sizes, k, rl, boxsize=box)
+    ctree, tres = jz.autotune.tune_pipeline_sets(x, sizes, k, rl, boxsize=box, leaf_sizes=(16, 24, 32, 48),
+                                                 leaf_waves=(2, 4), sort_kpts=(12, 16), reps=2)
+    eager = lambda: jz.batch.pipeline_sets(x, sizes, k, rl, boxsize=box, cfg_tree=ctree)
     ref = [t.clone() for t in eager()]
-    g = jz.batch.CapturedSets(x, sizes, k, rl, boxsize=box)
+    g = jz.batch.CapturedSets(x, sizes, k, rl, boxsize=box, cfg_tree=ctree)
     got = g.run()
     torch.cuda.synchronize()
     g.check()
@@ -105,7 +108,8 @@ def bench_batch(jz, torch, dev, nsets, k, steps, warmup, ev):
     def sequential():
         outs = []
         for s in range(nsets):
-            outs.append(jz.batch.pipeline_sets(x[off[s]:off[s + 1]], sizes[s:s + 1], k, rl[s:s + 1], boxsize=box))
+            outs.append(jz.batch.pipeline_sets(x[off[s]:off[s + 1]], sizes[s:s + 1], k, rl[s:s + 1], boxsize=box,
+                                               cfg_tree=ctree))
         return outs
 
     seq = sequential()
@@ -128,6 +132,7 @@ def bench_batch(jz, torch, dev, nsets, k, steps, warmup, ev):
         "graph_points_per_s": n / (t_graph * 1e-3), "batched_points_per_s": n / (t_eager * 1e-3),
         "one_set_at_a_time_points_per_s": n / (t_seq * 1e-3),
         "graph_identical": same_graph, "one_set_at_a_time_identical": same_seq, "graph_nodes": nodes,
+        "autotune": {"leaf_size": tres.leaf_size, "leaf_waves": tres.leaf_waves, "sort_kpt": tres.sort_kpt},
     }
 
 
@@ -159,14 +164,31 @@ def main():
     dev = torch.device("cuda:0")
     x = torch.from_numpy(xh).to(dev)
 
-    # tree settings for this workload: leaves of <= 64 points (one per lane of a wave64) and a
-    # leaf allocation large enough for strongly clustered sets; results do not depend on either
+    # tree settings: a leaf allocation large enough for strongly clustered sets; the leaf size, the
+    # leaf kernel's waves per EU and the sort tile are picked by the autotuner on these points
+    # (results do not depend on any of them, which the tuner checks)
     kcfg = jz.config.KNNConfig()
-    kcfg = dataclasses.replace(kcfg, tree=dataclasses.replace(kcfg.tree, alloc_fac_nodes=2.0, max_leaf_size=64))
+    kcfg = dataclasses.replace(kcfg, tree=dataclasses.replace(kcfg.tree, alloc_fac_nodes=2.0))
     fcfg = jz.config.FofConfig()
-    fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0, max_leaf_size=64))
+    fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0))
     rlink = args.b * n ** (-1.0 / 3.0)
     box = 1.0
+    t0 = time.time()
+    if args.no_tune:
+        kcfg = dataclasses.replace(kcfg, tree=dataclasses.replace(kcfg.tree, max_leaf_size=128))
+        fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, max_leaf_size=64))
+        jz.autotune.set_knobs(leaf_waves=4, sort_kpt=16)
+        tune_info = {"skipped": True, "leaf_size_knn": 128, "leaf_size_fof": 64, "leaf_waves": 4, "sort_kpt": 16}
+    else:
+        fcfg, ftune = jz.autotune.tune_fof(x, rlink, boxsize=box, cfg=fcfg, leaf_sizes=(32, 64, 96, 128),
+                                           sort_kpts=(12, 16), reps=2)
+        kcfg, ktune = jz.autotune.tune_knn(x, k, boxsize=box, cfg=kcfg, leaf_sizes=(64, 96, 128, 160),
+                                           leaf_waves=(2, 4), sort_kpts=(12, 16), reps=2)
+        tune_info = {"leaf_size_knn": ktune.leaf_size, "leaf_waves": ktune.leaf_waves, "sort_kpt": ktune.sort_kpt,
+                     "leaf_size_fof": ftune.leaf_size, "fof_sort_kpt": ftune.sort_kpt,
+                     "knn_candidates": len(ktune.table), "fof_candidates": len(ftune.table),
+                     "seconds": round(time.time() - t0, 1)}
+    print(f"# autotune: {tune_info}", file=sys.stderr, flush=True)
 
     # ---- proposer: train on other seeds (or load) ----
     t0 = time.time()
@@ -301,7 +323,9 @@ def main():
         "config": {"workload": f"clustered 2^{int(np.log2(n))} points ({args.halos} Plummer halos, 70% in halos), "
                                f"periodic unit box; exact kNN k={k} with the learned radius proposer, "
                                f"then FoF b={args.b}",
-                   "n_points": n, "k": k, "fof_b": args.b, "leaf_size": 64},
+                   "n_points": n, "k": k, "fof_b": args.b, "leaf_size_knn": kcfg.tree.max_leaf_size,
+                   "leaf_size_fof": fcfg.tree.max_leaf_size},
+        "autotune": tune_info,
         "plain_queries_per_s": n / (t_plain * 1e-3),
         "learned_speedup": t_plain / t_learn,
         "learned_identical": identical,

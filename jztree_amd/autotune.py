@@ -192,3 +192,23 @@ def tune_pipeline_sets(pos: torch.Tensor, sizes, k: int, rlink, boxsize: float =
     res = _tune("pipeline_sets", len(pos), k, pos.dtype, pos.shape[-1], boxsize, nsets, run, cache, reps,
                 sort_kpts, leaf_sizes, leaf_waves)
     return _with_leaf(cfg_tree, res.leaf_size), res
+
+
+def tune_fof(pos: torch.Tensor, rlink: float, boxsize: float = 0., cfg=None,
+             leaf_sizes: Sequence[int] = (32, 64, 96, 128), sort_kpts: Sequence[int] = (8, 12, 16), reps: int = 3,
+             cache: str | None = None):
+    """Tune fof_labels on these points over leaf size and sort tile (the leaf kNN waves knob is
+    left as it is).  Returns (FofConfig with the chosen leaf size, TuneResult)."""
+    from .config import FofConfig
+    from .fof import fof_labels
+    cfg = cfg if cfg is not None else FofConfig()
+    waves = get_knobs()["leaf_waves"]
+
+    def run(s):
+        set_knobs(sort_kpt=s["sort_kpt"])
+        return fof_labels(pos, rlink, boxsize=boxsize,
+                          cfg=dataclasses.replace(cfg, tree=_with_leaf(cfg.tree, s["leaf_size"])))[1]
+
+    res = _tune("fof", len(pos), 0, pos.dtype, pos.shape[-1], boxsize, 1, run, cache, reps, sort_kpts,
+                leaf_sizes, (waves,))
+    return dataclasses.replace(cfg, tree=_with_leaf(cfg.tree, res.leaf_size)), res

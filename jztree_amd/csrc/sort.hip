@@ -589,54 +589,76 @@ __global__ __launch_bounds__(256) void k_set_ids(const int64_t* __restrict__ off
     setid[p] = lds ? set_of(s_off, nsets, p) : set_of(off, nsets, p);
 }
 
-// per-set bounds: metas[3*s + {0,1,2}] as meta[] of k_bounds for set s.  Set ids never decrease
-// along the input, so each lane keeps one running accumulator and flushes it when its set changes;
-// the last flush is one atomic per wave when the whole wave ends in one set.
+// per-set bounds: metas[3*s + {0,1,2}] as meta[] of k_bounds for set s.  Each wave scans one
+// contiguous range of points, 64 at a time (coalesced), so it meets only the few sets its range
+// touches: lanes accumulate while the whole wave stays in the current set, and the wave reduces
+// and flushes once per set (one atomic per set and wave).  A step that straddles a set boundary
+// is flushed lane by lane.
+template <typename T>
+__device__ __forceinline__ void bounds_accum(T x, uint32_t& emax, uint32_t& neg, uint32_t& inf) {
+  using FB = FBits<T>;
+  using U = typename FB::U;
+  const U u = FB::bits(x) & ~((U)1 << (sizeof(U) * 8 - 1));
+  const uint32_t be = (uint32_t)(u >> FB::kMant);
+  const bool fin = be != (uint32_t)FB::kExpMax;
+  if (fin && u != 0) emax = max(emax, be ? be : 1u);
+  neg |= (x < T(0)) ? 1u : 0u;
+  inf |= (!fin && (u & (((U)1 << FB::kMant) - 1)) == 0) ? 1u : 0u;
+}
+__device__ __forceinline__ void bounds_flush(uint32_t* metas, int32_t set, uint32_t emax, uint32_t neg, uint32_t inf) {
+  atomicMax(&metas[3 * set], emax);
+  if (neg) atomicOr(&metas[3 * set + 1], 1u);
+  if (inf) atomicOr(&metas[3 * set + 2], 1u);
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_bounds_seg(const T* __restrict__ pos, int64_t n,
                                                     const int32_t* __restrict__ setid, uint32_t* metas) {
-  using FB = FBits<T>;
-  using U = typename FB::U;
-  uint32_t emax = 0, neg = 0, inf = 0;
-  int32_t cur = -1;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t sid = setid[i];
-    if (sid != cur) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int64_t per = ((n + nwaves - 1) / nwaves + 63) & ~(int64_t)63;
+  const int64_t r0 = wave * per, r1 = min(n, r0 + per);
+  uint32_t emax = 0, neg = 0, inf = 0;  // this lane's share of the current set
+  int32_t cur = -1;                     // wave-uniform
+  for (int64_t b = r0; b < r1; b += 64) {
+    const int64_t i = b + lane;
+    const bool valid = i < r1;
+    const int32_t sid = valid ? setid[i] : -1;
+    const int32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+    const bool uniform = __ballot(valid && sid != s0) == 0ull;
+    if (!uniform || s0 != cur) {  // leave the current set: flush it once per wave
       if (cur >= 0) {
-        atomicMax(&metas[3 * cur], emax);
-        if (neg) atomicOr(&metas[3 * cur + 1], 1u);
-        if (inf) atomicOr(&metas[3 * cur + 2], 1u);
+        for (int o = 32; o > 0; o >>= 1) {
+          emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
+          neg |= (uint32_t)__shfl_xor((int)neg, o, 64);
+          inf |= (uint32_t)__shfl_xor((int)inf, o, 64);
+        }
+        if (lane == 0) bounds_flush(metas, cur, emax, neg, inf);
       }
       emax = neg = inf = 0;
-      cur = sid;
+      cur = uniform ? s0 : -1;
     }
+    if (valid) {
+      uint32_t e = 0, ng = 0, nf = 0;
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      const T x = pos[i * D + d];
-      const U u = FB::bits(x) & ~((U)1 << (sizeof(U) * 8 - 1));
-      const uint32_t be = (uint32_t)(u >> FB::kMant);
-      const bool fin = be != (uint32_t)FB::kExpMax;
-      if (fin && u != 0) emax = max(emax, be ? be : 1u);
-      neg |= (x < T(0)) ? 1u : 0u;
-      inf |= (!fin && (u & (((U)1 << FB::kMant) - 1)) == 0) ? 1u : 0u;
+      for (int d = 0; d < D; d++) bounds_accum<T>(pos[i * D + d], e, ng, nf);
+      if (uniform) {
+        emax = max(emax, e);
+        neg |= ng;
+        inf |= nf;
+      } else {
+        bounds_flush(metas, sid, e, ng, nf);  // a step across a set boundary
+      }
     }
   }
-  const int32_t s0 = __builtin_amdgcn_readfirstlane(cur);
-  if (__ballot(cur != s0) == 0ull) {
+  if (cur >= 0) {
     for (int o = 32; o > 0; o >>= 1) {
       emax = max(emax, (uint32_t)__shfl_xor((int)emax, o, 64));
       neg |= (uint32_t)__shfl_xor((int)neg, o, 64);
       inf |= (uint32_t)__shfl_xor((int)inf, o, 64);
     }
-    if ((threadIdx.x & 63) == 0 && s0 >= 0) {
-      atomicMax(&metas[3 * s0], emax);
-      if (neg) atomicOr(&metas[3 * s0 + 1], 1u);
-      if (inf) atomicOr(&metas[3 * s0 + 2], 1u);
-    }
-  } else if (cur >= 0) {
-    atomicMax(&metas[3 * cur], emax);
-    if (neg) atomicOr(&metas[3 * cur + 1], 1u);
-    if (inf) atomicOr(&metas[3 * cur + 2], 1u);
+    if (lane == 0) bounds_flush(metas, cur, emax, neg, inf);
   }
 }
 
