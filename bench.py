@@ -38,8 +38,8 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the leaf kernel per launch on this workload, from two rocprofv3 --pmc passes
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain copy kernel of known size reads);
-# keyed by (kernel, n, k, halos); None where not measured for the current kernel.
-LEAF_TRAFFIC_BYTES = {}
+# keyed by (kernel, n, k, halos, leaf size); None where not measured for the current kernel.
+LEAF_TRAFFIC_BYTES = {("k_knn_leaf", 1 << 26, 32, 65536, 128): 91_678_102_323}
 
 
 def parse():
@@ -286,7 +286,7 @@ def main():
     t_leaf = ka.elapsed_time(kb) / reps
     alg_bytes = n * (16 + 8 * k)  # read query (12) + origin index (4), write k (radius, id) pairs
     achieved = alg_bytes / (t_leaf * 1e-3) / 1e9
-    traffic = LEAF_TRAFFIC_BYTES.get(("k_knn_leaf", n, k, args.halos))
+    traffic = LEAF_TRAFFIC_BYTES.get(("k_knn_leaf", n, k, args.halos, kcfg.tree.max_leaf_size))
 
     batched = None
     if args.batch_sets > 0:
