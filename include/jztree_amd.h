@@ -192,6 +192,22 @@ int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_ilist_spl, c
                          int64_t size_parents, int64_t size_nodes, int64_t parent_ilist_size,
                          int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
 
+/* jzt_knn_group (extension; k <= 64, float32): the leaf plane of the kNN walk over query groups.
+ * A group is a plane-1 node: its queries (the particles of its child leaves, ranges from
+ * parent_to_leaf and leaf_to_part) go 64 at a time through one wave, which walks the node's
+ * r2-sorted plane-1 list (parent_ilist_*, as KnnNode2Node writes it for plane 1) and evaluates
+ * the listed nodes' child leaves (cells in leaf_nodes: centre + level bits, dim+1 floats per
+ * leaf) on the fly.  Replaces the last KnnNode2Node + KnnLeaf2Leaf (knn.cuh:602-694, :309-356)
+ * with the same result; lanes stay busy across leaf boundaries.  init_r2 (per group, NULL = none;
+ * negative = skip the group) starts every query's top-k at that bound; queries left with fewer
+ * than k neighbours inside it are flagged in retry (required with init_r2).  Output rows as
+ * jzt_knn_leaf2leaf_ex (dst_rows, out_rows). */
+int jzt_knn_group(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
+                  const float* parent_ilist_r2, int64_t parent_ilist_size, const int32_t* parent_to_leaf,
+                  const void* leaf_nodes, const int32_t* leaf_to_part, const void* pos, const int32_t* src_ids,
+                  const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* knn_rad, int32_t* knn_id,
+                  int64_t ngroups, int64_t out_rows, int k, float boxsize, int dim, int dtype);
+
 /* Segment maximum (extension, used to pass search radii up the tree planes): out[i] = max of
  * vals over [spl[i], spl[i+1]) clamped to [0, nvals), or `empty` for an empty segment.  spl has
  * nseg+1 entries. */

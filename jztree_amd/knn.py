@@ -195,6 +195,62 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     return rnn, inn, retry
 
 
+def _knn_group(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, alloc_fac_ilist: float = 256.,
+               src_ids=None, dst_rows=None, out_rows: int | None = None, walk=None, init_r2=None, out=None):
+    """Leaf plane over query groups (extension, k <= 64; see csrc/knn_group.hip): the queries of
+    each plane-1 node go 64 at a time through one wave, over the node's plane-1 list.
+
+    walk = (plane-1 lists, plane-1 radii, plane-1 -> leaf splits); default: the guaranteed-radius
+    walk down to plane 1.  init_r2 (per plane-1 node) is the start bound, default the node's
+    guaranteed radius widened by 2^-10.  Without an explicit init_r2, queries left short by
+    rounding are redone without a bound, so the result equals knn()'s.  With one, returns the retry
+    flags as well (rows of flagged queries are incomplete)."""
+    assert k <= 64, "query groups need k <= 64"
+    dev = posz.device
+    n = len(posz)
+    size = th.size()
+    boxsize = 0. if boxsize is None else boxsize
+    if walk is None:
+        walk = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                              return_parents=True)
+    ilist_p, rad_p, n2c = walk
+    explicit = init_r2 is not None
+    if not explicit and rad_p is not None:
+        init_r2 = (rad_p.to(torch.float64) * (1.0 + 2.0 ** -10)).to(torch.float32)
+    # a one-plane tree has no guaranteed radii (its groups are the top-level super-nodes): no bound
+    init_r2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
+    leaf_nodes = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=th.lvl.get(0, size)).pos_lvl()
+    l2p = th.splits_leaf_to_part().contiguous()
+    n2c = n2c.to(torch.int32).contiguous()
+    rows = n if out_rows is None else int(out_rows)
+    if out is not None:
+        rnn, inn, retry = out
+    else:
+        rnn = torch.empty((rows, k), dtype=posz.dtype, device=dev)
+        inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
+        retry = torch.empty(n, dtype=torch.int32, device=dev)
+    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
+    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+
+    def launch(r2):
+        L.call("jzt_knn_group", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
+               ilist_p.size(), L.ptr(n2c), L.ptr(leaf_nodes), L.ptr(l2p), L.ptr(posz.contiguous()), L.ptr(sid),
+               L.ptr(drow), L.ptr(r2), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, rows, int(k), float(boxsize),
+               posz.shape[-1], L.dtype_code(posz.dtype))
+
+    launch(init_r2)
+    if explicit:
+        return rnn, inn, retry
+    if init_r2 is not None and bool((retry != 0).any()):
+        from .tools import inverse_of_splits
+        gspl = l2p[n2c.long().clamp(0, len(l2p) - 1)].contiguous()
+        short = torch.zeros(size + 1, dtype=torch.bool, device=dev)
+        short[inverse_of_splits(gspl, n).long().clamp(0, size)[retry != 0]] = True
+        launch(torch.where(short[:size], torch.full_like(init_r2, float("inf")),
+                           torch.full_like(init_r2, -1.0)).contiguous())
+    return rnn, inn
+
+
 def _cell_diag2(lvl: torch.Tensor, dim: int) -> torch.Tensor:
     """Squared diagonal of the Morton cell of each level (float64): per-dim sides 2^l_i with the
     level split as in lvl_vec (math.cuh:524-541)."""
@@ -372,14 +428,16 @@ def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, le
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,
         result: str = "rad_globalidx", reduce_func: Callable | None = None, output_order: str = "input",
-        cfg: KNNConfig = KNNConfig(), fused: bool = False):
+        cfg: KNNConfig = KNNConfig(), fused: bool | str = False):
     """Exact k nearest neighbours (knn.py:198-347).
 
     Returns (rnn, inn) by default: radii and indices of the k nearest neighbours of every particle,
     ascending by (distance, index).  result: "_"-separated keys among rad, drad, rankidx,
     globalidx, part, reduce, or a particle attribute name.  output_order: "input" or "z".
-    fused (extension): use the fused leaf-plane kernel for k <= 64 (same results; False runs the
-    reference's KnnNode2Node + KnnLeaf2Leaf decomposition).
+    fused (extension, k <= 64, same results): True = the fused leaf-plane kernel (one wave per query
+    leaf over its parent's list), "group" = query groups (the queries of each plane-1 node 64 at a
+    time, csrc/knn_group.hip: faster for small leaves); False runs the reference's KnnNode2Node +
+    KnnLeaf2Leaf decomposition (the default: fastest with the leaf sizes the autotuner picks).
     """
     assert output_order in ("z", "input")
     if part_query is not None:
@@ -406,7 +464,10 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
         # rows can be written straight to input order when only radii/indices are requested
         direct = (output_order == "input" and set(keys) <= {"rad", "globalidx"}
                   and isinstance(num_origin, int) and num_origin == size)
-        if fused and k <= 64:
+        if fused == "group" and k <= 64:
+            rnnz, inn_g = _knn_group(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
+                                     src_ids=originz.idx, dst_rows=originz.idx if direct else None)
+        elif fused and k <= 64:
             # neighbours are reported (and tie-broken) by their origin index directly
             rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                         src_ids=originz.idx, dst_rows=originz.idx if direct else None)

@@ -116,9 +116,35 @@ def test_fused_equals_reference_decomposition(dev, oracle, case):
     cfg = _clustered_cfg()
     r1, i1 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=True)
     r2_, i2 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=False)
+    r3, i3 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused="group")
     torch.cuda.synchronize()
     assert torch.equal(i1, i2) and torch.equal(r1, r2_)
+    assert torch.equal(i3, i2) and torch.equal(r3, r2_), "query groups differ"
     _check(x, k, r1.cpu().numpy(), i1.cpu().numpy(), oracle, boxsize=box)
+
+
+def test_group_bound_and_retry(dev):
+    """Query groups with a start bound too small for some queries flag them; the default call redoes
+    them and equals the reference decomposition."""
+    import jztree_amd as jz
+    x = clustered(1 << 19, nhalos=256, seed=24)  # two planes: the plane-1 nodes have guaranteed radii
+    t = torch.from_numpy(x).to(dev)
+    cfg = _clustered_cfg()
+    partz, originz, th = jz.tree.zsort_and_tree(t, cfg.tree, data=jz.data.RankIdx(
+        rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev)))
+    walk = jz.knn._knn_dual_walk(th, 16, boxsize=1.0, alloc_fac_ilist=cfg.alloc_fac_ilist, stop_level=1,
+                                 return_parents=True)
+    print("[group] bound x0.3", flush=True)
+    r1, i1, retry = jz.knn._knn_group(th, partz, 16, boxsize=1.0, src_ids=originz.idx, walk=walk,
+                                      init_r2=walk[1] * 0.3)
+    torch.cuda.synchronize()
+    assert int(retry.sum()) > 0
+    rz, iz = jz.knn._knn_leaf2leaf(jz.knn._knn_dual_walk(th, 16, boxsize=1.0, alloc_fac_ilist=cfg.alloc_fac_ilist),
+                                   th.splits_leaf_to_part(), partz, k=16, boxsize=1.0, src_ids=originz.idx)
+    ok = retry == 0
+    assert torch.equal(i1[ok], iz[ok]) and torch.equal(r1[ok], rz[ok]), "unflagged rows must be exact"
+    r2_, i2 = jz.knn._knn_group(th, partz, 16, boxsize=1.0, src_ids=originz.idx, walk=walk)
+    assert torch.equal(i2, iz) and torch.equal(r2_, rz)
 
 
 def test_fused_bound_and_retry(dev, oracle):
