@@ -183,4 +183,40 @@ __device__ __forceinline__ bool consume_chunk(uint64_t (&best)[K], const Vec<D, 
   return flushed;
 }
 
+// Write a lane's k results (ascending keys) to its output row: radius = sqrt(r2), id = low word.
+// Rows of whole float4/int4 groups that are 16-byte aligned go out as 16-byte stores (a quarter of
+// the store instructions; with rows in input order every lane writes a different row, and the
+// address processing of 64 scattered rows per instruction dominates), the rest element-wise.
+template <int K>
+__device__ __forceinline__ void write_row(const uint64_t (&best)[K], float* __restrict__ rad_out,
+                                          int32_t* __restrict__ id_out, int64_t o, int nk) {
+  if constexpr (K % 4 == 0) {
+    if (nk >= K && ((reinterpret_cast<uintptr_t>(rad_out + o) | reinterpret_cast<uintptr_t>(id_out + o)) & 15) == 0) {
+#pragma unroll
+      for (int i = 0; i < K; i += 4) {
+        float4 r;
+        int4 d;
+        r.x = sqrtf(key_r2(best[i]));
+        r.y = sqrtf(key_r2(best[i + 1]));
+        r.z = sqrtf(key_r2(best[i + 2]));
+        r.w = sqrtf(key_r2(best[i + 3]));
+        d.x = (int32_t)(uint32_t)best[i];
+        d.y = (int32_t)(uint32_t)best[i + 1];
+        d.z = (int32_t)(uint32_t)best[i + 2];
+        d.w = (int32_t)(uint32_t)best[i + 3];
+        *reinterpret_cast<float4*>(rad_out + o + i) = r;
+        *reinterpret_cast<int4*>(id_out + o + i) = d;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < K; i++) {
+    if (i < nk) {
+      rad_out[o + i] = sqrtf(key_r2(best[i]));
+      id_out[o + i] = (int32_t)(uint32_t)best[i];
+    }
+  }
+}
+
 }  // namespace jzt

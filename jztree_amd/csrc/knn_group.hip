@@ -142,16 +142,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
       if (wave_any(cnt > 0)) topk_flush<K>(best, row, cnt);
       if (qoff + lane < p1) {
         const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
-        if (orow >= 0 && orow < out_rows) {
-          const int64_t o = orow * ktot;
-#pragma unroll
-          for (int i = 0; i < K; i++) {
-            if (i < ktot) {
-              rad_out[o + i] = sqrtf(key_r2(best[i]));
-              id_out[o + i] = (int32_t)(uint32_t)best[i];
-            }
-          }
-        }
+        if (orow >= 0 && orow < out_rows) write_row<K>(best, rad_out, id_out, orow * ktot, ktot);
         if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
       }
     }
