@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--b", type=float, default=0.2)
     p.add_argument("--train-seeds", type=int, default=4)
     p.add_argument("--train-steps", type=int, default=2000)
-    p.add_argument("--quantile", type=float, default=0.98)
+    p.add_argument("--quantile", type=float, default=0.99)
     p.add_argument("--proposer", default=None, help="load a trained proposer (.npz) instead of training")
     p.add_argument("--save-proposer", default=None)
     p.add_argument("--cpu-queries", type=int, default=1 << 21)

@@ -322,18 +322,19 @@ def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0.
 
 def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize: float | None = None,
                  alloc_fac_ilist: float = 256., src_ids=None, dst_rows=None, out_rows: int | None = None,
-                 grow: float | None = None, stats: dict | None = None):
+                 grow: float | None = None, stats: dict | None = None, direct_retry: bool = True):
     """Exact kNN from a proposed search radius^2 per leaf (extension; k <= 64).
 
     Round 1 walks the tree with the proposed radii on every plane (a node searches as far as its
     farthest-reaching child; each radius clipped to the guaranteed one, _guaranteed_leaf_r2) and
     runs the leaf kernel with each leaf's radius as starting bound.  Queries that find fewer than k
-    neighbours inside it are flagged; the next round redoes their leaves with the guaranteed radius
-    while every other leaf sits out (with `grow`, a round with the radius^2 grown by that factor,
-    still clipped, comes first: each round walks the tree once, so an extra round only pays when
-    most short leaves are close misses).  Whatever is left -- or everything, should a round's lists
-    overflow their allocation -- is recomputed without any proposal: by the fused retry pass on
-    guaranteed-radius lists, or by the plain walk when round 1 itself overflowed.
+    neighbours inside it are flagged.  With direct_retry (default) the flagged queries go straight
+    to the fused retry pass (guaranteed-radius walk down to plane 1 only, then only the flagged
+    queries).  Otherwise the next round redoes their leaves with the guaranteed radius while every
+    other leaf sits out (with `grow`, a round with the radius^2 grown by that factor, still
+    clipped, comes first); each such round walks the whole tree again.  Whatever is left -- or
+    everything, should a round's lists overflow their allocation -- is recomputed without any
+    proposal: by the fused retry pass, or by the plain walk when round 1 itself overflowed.
     The result is therefore independent of the proposal: identical to knn() for any leaf_r2.
     Returns (rnn, inn, retry flags of round 1)."""
     assert k <= 64, "a start bound needs k <= 64"
@@ -378,7 +379,8 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     # round's lists and flags are read back once.
     sl = spl.long().clamp(0, n)
     later = ((lambda: torch.minimum(r2 * grow, cap)),) if grow else ()
-    for rnd, r2_next in enumerate(later + ((lambda: cap),), start=2):
+    later = () if direct_retry else later + ((lambda: cap),)
+    for rnd, r2_next in enumerate(later, start=2):
         cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(retry, 0)])
         short = (cs[sl[1:]] - cs[sl[:-1]]) > 0
         ovf = run(torch.where(short, r2_next(), torch.full_like(r2, -1.0)).contiguous())

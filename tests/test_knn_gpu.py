@@ -196,10 +196,12 @@ def test_bounded_matches_plain(dev, oracle, k, boxsize):
     exact = _leaf_max_r2k(jz, th, rz, k)
     mixed = exact.clone()
     mixed[::3] = 0.0
-    for fac, bound, grow in [(0.25, exact * 0.25, None), (0.25, exact * 0.25, 16.0), (1.0, exact * (1 + 1e-6), None),
-                             (3.0, exact * 3.0, None), ("mixed", mixed, None)]:
+    for fac, bound, grow, direct in [(0.25, exact * 0.25, None, True), (0.25, exact * 0.25, None, False),
+                                     (0.25, exact * 0.25, 16.0, False), (1.0, exact * (1 + 1e-6), None, True),
+                                     (3.0, exact * 3.0, None, True), ("mixed", mixed, None, True),
+                                     ("mixed", mixed, None, False)]:
         st = {}
-        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize, grow=grow,
+        r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize, grow=grow, direct_retry=direct,
                                             alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx, stats=st)
         print(f"[bounded] k={k} box={boxsize} factor {fac}: {st}", flush=True)
         jz.checks.flush()
