@@ -53,14 +53,37 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
   const int idx = threadIdx.x + scan_size;
   const int my = s_level[idx];
   bool is_split = false;
-  for (int type = 0; type < num_types; type++) {
-    int lsize = 0;
-    for (int i = idx - scan_size; i < idx; i++) lsize = (s_level[i] < my ? lsize : 0) + (s_types[i] == type ? 1 : 0);
-    int rsize = 0;
-    for (int i = idx + scan_size; i > idx; i--) rsize = (s_level[i] < my ? rsize : 0) + (s_types[i] == type ? 1 : 0);
+  if (num_types == 1) {
+    // one type: lsize = idx - (the nearest i < idx in the window with s_level[i] >= my), or the whole
+    // window if there is none -- what the reset-and-count loop below leaves; scanning outward from
+    // idx stops there instead of reading the whole window (likewise rsize to the right)
+    int lsize = scan_size;
+    for (int i = idx - 1; i >= idx - scan_size; i--) {
+      if (s_level[i] >= my) {
+        lsize = idx - i;
+        break;
+      }
+    }
+    int rsize = scan_size;
+    for (int i = idx + 1; i <= idx + scan_size; i++) {
+      if (s_level[i] >= my) {
+        rsize = i - idx;
+        break;
+      }
+    }
     is_split |= (lsize + rsize > max_size);
     is_split |= (my > lb_l) && (node_idx - lsize <= 0);
     is_split |= (my > lb_r) && (node_idx + rsize >= nump);
+  } else {
+    for (int type = 0; type < num_types; type++) {
+      int lsize = 0;
+      for (int i = idx - scan_size; i < idx; i++) lsize = (s_level[i] < my ? lsize : 0) + (s_types[i] == type ? 1 : 0);
+      int rsize = 0;
+      for (int i = idx + scan_size; i > idx; i--) rsize = (s_level[i] < my ? rsize : 0) + (s_types[i] == type ? 1 : 0);
+      is_split |= (lsize + rsize > max_size);
+      is_split |= (my > lb_l) && (node_idx - lsize <= 0);
+      is_split |= (my > lb_r) && (node_idx + rsize >= nump);
+    }
   }
   is_split |= my >= kLvlSet;  // set boundary of a batch
   is_split &= node_idx <= nump;
