@@ -102,16 +102,26 @@ __global__ __launch_bounds__(64) void k_node_to_child(const int32_t* __restrict_
   }
 }
 
-// PASS 0: link guaranteed pairs; PASS 1: count uncertain pairs; PASS 2: insert them
+// PASS 0: link guaranteed pairs; PASS 1: count uncertain pairs; PASS 2: insert them.
+// One wave per parent node; its children are staged in LDS, and for every entry of the parent's list
+// (each parent pair once) the entry's children too.  Lanes take (child, target) pairs: with nt
+// targets staged, lane l tests child c0 + l / nt against target l % nt (one lane per child would
+// leave most lanes idle: nodes have about coarse_fac children).  A child's hits in one step are
+// lanes of one contiguous block: counted with a ballot and, in the insert pass, placed at the
+// child's running offset plus the rank of the hit in its block.  Lists are sets (the leaf kernel and
+// the next plane visit every entry), and union-find with min-hooking reaches the same roots in any
+// link order, so the labels do not depend on the order inside a list or of the links.
 template <int PASS, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ nodes,
                                                  const int32_t* __restrict__ il_spl, int32_t* ig, int32_t* __restrict__ count,
                                                  int32_t* __restrict__ il, float r2link, const float* __restrict__ r2v,
                                                  T boxsize, int64_t il_size, int64_t pil_size, int64_t nitems) {
-  __shared__ NodeX<D, T> s_n[64];
-  __shared__ int s_g[64];
+  __shared__ NodeX<D, T> s_q[64], s_n[64];
+  __shared__ int s_qg[64], s_g[64];
+  __shared__ int64_t s_run[64];  // per child: hits so far (count pass) or next list slot (insert pass)
   const int lane = threadIdx.x;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   const int64_t nact_ = active_items(spl, nitems);
   for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
     const int q0 = spl[pq], q1 = spl[pq + 1];
@@ -119,49 +129,61 @@ __global__ __launch_bounds__(64) void k_fof_node(const int32_t* __restrict__ pil
     const float r2 = r2v ? r2v[pq] : r2link;
     const T rl_hi = (T)r2 * (T)kSlackHi, rl_lo = (T)r2 * (T)kSlackLo;
     for (int qoff = q0; qoff < q1; qoff += 64) {
-      const int iq = min(qoff + lane, q1 - 1);
-      const bool valid = qoff + lane < q1;
-      NodeX<D, T> nq = load_nodex<D, T>(nodes, iq);
-      const int gq = ig[iq];
-      int64_t off = (PASS == 2) ? (int64_t)il_spl[iq] : 0;
-      int cnt = 0;
+      const int nc = min(q1 - qoff, 64);
+      if (lane < nc) {
+        s_q[lane] = load_nodex<D, T>(nodes, qoff + lane);
+        s_qg[lane] = ig[qoff + lane];
+        s_run[lane] = (PASS == 2) ? (int64_t)il_spl[qoff + lane] : 0;
+      }
+      __syncthreads();
       for (int e = e0; e < e1; e++) {
         const int pt = pil[e];
         if (pt < pq) continue;  // each parent pair once
         const int t0 = spl[pt], t1 = spl[pt + 1];
         for (int toff = t0; toff < t1; toff += 64) {
-          const int it = toff + lane;
-          if (it < t1) {
-            s_n[lane] = load_nodex<D, T>(nodes, it);
-            s_g[lane] = ig[it];
+          const int nt = min(t1 - toff, 64);
+          if (lane < nt) {
+            s_n[lane] = load_nodex<D, T>(nodes, toff + lane);
+            s_g[lane] = ig[toff + lane];
           }
           __syncthreads();
-          const int nl = min(t1 - toff, 64);
-          if (valid) {
-            for (int j = 0; j < nl; j++) {
-              const int jt = toff + j;
-              if (gq == s_g[j] && iq != jt) continue;  // already linked
-              NodeX<D, T> nt = s_n[j];
+          const int cstep = nt >= 32 ? 1 : 64 / nt;
+          const int my_c = lane / nt, my_t = lane - my_c * nt;
+          const uint64_t block = (nt == 64 ? ~0ull : ((1ull << nt) - 1ull)) << (my_c * nt);
+          for (int c0 = 0; c0 < nc; c0 += cstep) {
+            const int c = c0 + my_c;
+            const bool active = my_c < cstep && c < nc;
+            const int iq = qoff + c, jt = toff + my_t;
+            bool hit = false;
+            if (active && !(s_qg[c] == s_g[my_t] && iq != jt)) {  // equal labels: already linked
+              const NodeX<D, T> nq = s_q[c], nt_ = s_n[my_t];
               Vec<D, T> w;
-  #pragma unroll
-              for (int d = 0; d < D; d++) w[d] = nq.h[d] + nt.h[d];
-              T r2max = maxdist2<D, T>(nt.c, nq.c, w, boxsize);
-              T r2min = mindist2<D, T>(nt.c, nq.c, w, boxsize);
-              T L2 = norm2<D, T>(nt.h) * T(4);
+#pragma unroll
+              for (int d = 0; d < D; d++) w[d] = nq.h[d] + nt_.h[d];
+              const T r2max = maxdist2<D, T>(nt_.c, nq.c, w, boxsize);
+              const T r2min = mindist2<D, T>(nt_.c, nq.c, w, boxsize);
+              const T L2 = norm2<D, T>(nt_.h) * T(4);
               if (max(r2max, L2) <= rl_lo) {
                 if (PASS == 0) uf_link(ig, iq, jt);
               } else if (PASS >= 1 && r2min <= rl_hi) {
-                if (PASS == 2 && off + cnt < il_size) il[off + cnt] = jt;
-                cnt++;
+                hit = true;
               }
+            }
+            if (PASS >= 1) {
+              const uint64_t hits = __ballot(hit) & block;
+              if (PASS == 2 && hit) {
+                const int64_t o = s_run[c] + __popcll(hits & lt_mask);
+                if (o < il_size) il[o] = jt;
+              }
+              if (active && my_t == 0 && hits) s_run[c] += __popcll(hits);
             }
           }
           __syncthreads();
         }
       }
-      if (PASS == 1 && valid) count[iq] = cnt;
+      if (PASS == 1 && lane < nc) count[qoff + lane] = (int32_t)s_run[lane];
+      __syncthreads();
     }
-  
   }
 }
 
