@@ -37,17 +37,20 @@ __device__ __forceinline__ int uf_find(const int32_t* p, int x) {
   }
 }
 
-__device__ __forceinline__ void uf_link(int32_t* p, int a, int b) {
+// Joins the sets of a and b; returns the root the two were joined under (a root at the time of the
+// join: later links elsewhere may hook it further).
+__device__ __forceinline__ int uf_link(int32_t* p, int a, int b) {
   a = uf_find(p, a);
   b = uf_find(p, b);
   while (a != b) {
     int hi = a > b ? a : b, lo = a > b ? b : a;
     int old = atomicMin(p + hi, lo);
-    if (old == hi || old == lo) return;  // hooked (or already joined to lo)
+    if (old == hi || old == lo) return lo;  // hooked (or already joined to lo)
     // hi had been hooked to `old` meanwhile; join old's tree with lo and go on
     a = uf_find(p, old);
     b = uf_find(p, lo);
   }
+  return a;
 }
 
 __global__ void k_contract(int32_t* p, int64_t n) {
@@ -237,8 +240,10 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
         qlo[d] = wave_min_t<T>(xq[d]);
         qhi[d] = wave_max_t<T>(xq[d]);
       }
-      const int gq = ig[iq];
-      int glast = -1;
+      // labels known to be in q's set: its own label, refreshed to the root after each join, and
+      // the last two labels it was joined with
+      int gq = ig[iq];
+      int glast = -1, glast2 = -1;
       for (int e = e0; e < e1; e++) {
         const int lt = il[e];
         if (lt < lq) continue;  // each leaf pair once
@@ -273,14 +278,16 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
               const int gj = s_r[j].g;
               // labels are snapshots of set members; a label equal to q's own, or to one q was
               // already joined with, belongs to q's set by now
-              if (gj == gq || gj == glast) continue;
+              if (gj == gq || gj == glast || gj == glast2) continue;
               Vec<D, T> xp;
   #pragma unroll
               for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
               const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
               if (r2 <= (T)r2link) {
-                uf_link(ig, iq, toff + j);
+                const int root = uf_link(ig, iq, toff + j);
+                glast2 = glast;
                 glast = gj;
+                gq = root;
               }
             }
           }
