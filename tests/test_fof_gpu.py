@@ -66,3 +66,34 @@ def test_fof_input_order_and_canonical(dev, oracle):
     zpos = np.empty(len(x), np.int64)
     zpos[idz] = np.arange(len(x))
     assert np.all(zpos[ig] <= zpos)  # label points at the group member that comes first in z-order
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("boxsize", [0.0, 1.0])
+def test_fof_float64(dev, boxsize):
+    """FoF on float64 positions (sort, tree and FoF kernels in double): the partition equals float64
+    brute-force connected components of r2 <= float32(rlink^2) (random points: no pair lies within
+    an ulp of the threshold)."""
+    import numpy as np
+    import torch
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    import jztree_amd as jz
+    rng = np.random.default_rng(31)
+    x = rng.random((6000, 3))
+    rl = 0.7 * len(x) ** (-1 / 3)
+    print(f"[fof f64] box={boxsize}", flush=True)
+    _, lab = jz.fof.fof_labels(torch.from_numpy(x).to(dev), rl, boxsize=boxsize, output_order="input")
+    lab = lab.cpu().numpy()
+    d = x[:, None, :] - x[None, :, :]
+    if boxsize:
+        d = d - boxsize * np.round(d / boxsize)
+    d2 = (d * d).sum(-1)
+    i, j = np.nonzero(d2 <= float(np.float32(rl * rl)))
+    ncomp, comp = connected_components(coo_matrix((np.ones(len(i)), (i, j)), shape=(len(x), len(x))), directed=False)
+    # same partition: canonical label = smallest member index
+    def canon(l):
+        first = {}
+        return np.array([first.setdefault(v, k) for k, v in enumerate(l)])
+    assert np.array_equal(canon(lab), canon(comp))
+    assert ncomp < len(x)
