@@ -284,39 +284,52 @@ def _guaranteed_leaf_r2(th: TreeHierarchy, k: int) -> torch.Tensor:
     return cap
 
 
-def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor) -> list:
+def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor, upper_r2: torch.Tensor | None = None) -> list:
     """Search radius^2 for every plane: leaf_r2 for the leaves (negative = no search), and for a
     node the largest radius among its children.  A node's list then holds every node that a leaf
-    inside it may reach: mindist(child, T') >= mindist(node, T) for children of T."""
+    inside it may reach: mindist(child, T') >= mindist(node, T) for children of T.
+
+    upper_r2 (per leaf, >= leaf_r2): the planes above the leaves take their radii from these
+    instead (e.g. guaranteed radii, so that the plane-1 lists also serve an unbounded search)."""
     size = th.size()
     dev = leaf_r2.device
     ar = torch.arange(size, device=dev)
     neg = torch.full((size,), -1.0, dtype=torch.float32, device=dev)
     radii = [torch.where(ar < th.num(0), leaf_r2.float(), neg).contiguous()]
+    chain = radii[0] if upper_r2 is None else torch.where(ar < th.num(0), upper_r2.float(), neg).contiguous()
     for p in range(1, th.num_planes()):
         n2c = th.ispl_n2n.get(p, size + 1).to(torch.int32).contiguous()
         r = torch.empty(size, dtype=torch.float32, device=dev)
-        L.call("jzt_segment_max_f32", L.stream(), L.ptr(n2c), L.ptr(radii[-1]), size, L.ptr(r), size, -1.0)
-        radii.append(torch.where(ar < th.num(p), r, neg).contiguous())
+        L.call("jzt_segment_max_f32", L.stream(), L.ptr(n2c), L.ptr(chain), size, L.ptr(r), size, -1.0)
+        chain = torch.where(ar < th.num(p), r, neg).contiguous()
+        radii.append(chain)
     return radii
 
 
 def _knn_radius_walk(th: TreeHierarchy, k: int, radii: list, boxsize: float = 0., alloc_fac_ilist: float = 256.,
-                     overflow: list | None = None):
+                     overflow: list | None = None, keep_plane1: bool = False):
     """Dual walk with given search radii on every plane (no guaranteed-radius search): from the dense
-    top-plane list down to leaf lists holding every leaf within each leaf's radius."""
+    top-plane list down to leaf lists holding every leaf within each leaf's radius.
+
+    keep_plane1: also return (the plane-1 lists, plane-1 -> leaf splits) -- with a one-plane tree
+    the dense top-level list and its super-node splits."""
     nlevels = th.num_planes()
     size = th.size()
     spl, ilist, nsup = top_dense_interaction_list(th, int(th.size_leaves * alloc_fac_ilist), with_r2=True)
     spl_n2n = th.ispl_n2n.append(spl, nsup + 1, fill_value=spl[-1], resize=True)
     raise_if(spl_n2n.nfilled() > spl_n2n.size(), "No space for top-level splits\nHint: Increase alloc_fac_nodes.")
+    plane1 = None
     for i in range(nlevels):
         level = nlevels - i - 1
         parent_spl = spl_n2n.get(level + 1, size + 1)
+        if level == 0:
+            plane1 = (ilist, parent_spl)
         node_data = PosLvlNum(pos=th.geom_cent.get(level, size), lvl=th.lvl.get(level, size),
                               npart=th.npart(level, size=size, ptype=0))
         ilist = _knn_node2node_ilist(ilist, parent_spl, node_data, k=k, boxsize=boxsize, rmax2_in=radii[level],
                                      overflow=overflow)
+    if keep_plane1:
+        return ilist, plane1
     return ilist
 
 
@@ -325,16 +338,18 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
                  grow: float | None = None, stats: dict | None = None, direct_retry: bool = True):
     """Exact kNN from a proposed search radius^2 per leaf (extension; k <= 64).
 
-    Round 1 walks the tree with the proposed radii on every plane (a node searches as far as its
-    farthest-reaching child; each radius clipped to the guaranteed one, _guaranteed_leaf_r2) and
-    runs the leaf kernel with each leaf's radius as starting bound.  Queries that find fewer than k
-    neighbours inside it are flagged.  With direct_retry (default) the flagged queries go straight
-    to the fused retry pass (guaranteed-radius walk down to plane 1 only, then only the flagged
-    queries).  Otherwise the next round redoes their leaves with the guaranteed radius while every
-    other leaf sits out (with `grow`, a round with the radius^2 grown by that factor, still
-    clipped, comes first); each such round walks the whole tree again.  Whatever is left -- or
-    everything, should a round's lists overflow their allocation -- is recomputed without any
-    proposal: by the fused retry pass, or by the plain walk when round 1 itself overflowed.
+    Round 1 walks the tree with the proposed radii at the leaf plane (each clipped to the guaranteed
+    one, _guaranteed_leaf_r2) and runs the leaf kernel with each leaf's radius as starting bound.
+    Queries that find fewer than k neighbours inside it are flagged.  With direct_retry (default)
+    the planes above the leaves search the guaranteed radii (a node as far as its farthest-reaching
+    child), so round 1's plane-1 lists already hold every node a flagged query can need, and the
+    flagged queries go straight to the fused retry pass over those lists.  Without it every plane
+    searches the proposed radii, and the next round redoes the flagged queries' leaves with the
+    guaranteed radius while every other leaf sits out (with `grow`, a round with the radius^2 grown
+    by that factor, still clipped, comes first); each such round walks the whole tree again.
+    Whatever is left -- or everything, should a round's lists overflow their allocation -- is
+    recomputed without any proposal: by the fused retry pass (after its own walk to plane 1 when
+    round 1 did not keep its plane-1 lists), or by the plain walk when round 1 itself overflowed.
     The result is therefore independent of the proposal: identical to knn() for any leaf_r2.
     Returns (rnn, inn, retry flags of round 1)."""
     assert k <= 64, "a start bound needs k <= 64"
@@ -353,15 +368,21 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     cap = (_guaranteed_leaf_r2(th, k) * (1.0 + 2.0 ** -10)).to(torch.float32)
     r2 = torch.minimum(leaf_r2.to(torch.float32), cap).contiguous()
 
-    def run(r2):
+    plane1 = []
+
+    def run(r2, upper=None):
         ovf = []
-        il = _knn_radius_walk(th, k, _plane_radii(th, r2), boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist,
-                              overflow=ovf)
+        il, p1 = _knn_radius_walk(th, k, _plane_radii(th, r2, upper), boxsize=boxsize,
+                                  alloc_fac_ilist=alloc_fac_ilist, overflow=ovf, keep_plane1=True)
+        if upper is not None:
+            plane1.append(p1)
         _knn_leaf2leaf(il, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids, dst_rows=dst_rows, out_rows=rows,
                        init_r2=r2, out=(rnn, inn, retry))
         return torch.stack(ovf).any()
 
-    ovf1 = run(r2)
+    # with direct_retry the planes above the leaves search the guaranteed radii: their plane-1 lists
+    # then serve the fused retry pass directly (no second walk)
+    ovf1 = run(r2, cap if direct_retry else None)
     first = retry.clone()
     # one read-back per round: (lists overflowed, queries left)
     h = torch.stack([ovf1.to(torch.int64), first.sum().to(torch.int64)]).cpu()
@@ -394,8 +415,11 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
         if int(h[1]) == 0:
             return rnn, inn, first
     st["fallback"] = "fused retry"
-    ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
-                                     return_parents=True)
+    if plane1:
+        ilist_p, n2c = plane1[0]
+    else:
+        ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
+                                         return_parents=True)
     leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
