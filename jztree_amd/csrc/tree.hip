@@ -7,7 +7,8 @@
 //                                        [lbound, rbound) extent of the node it splits (binary search)
 //  GetNodeGeometry     tree.cuh:312-372  level, centre and extent of the smallest cell holding a range
 //
-// All integer/bit exact.  Launch: 256 threads (4 waves) per workgroup, one boundary/node per lane.
+// All integer/bit exact.  Launch: 256 threads (4 waves) per workgroup, one node per lane (four
+// boundaries per lane in FlagLeafBoundaries).
 #include "util.h"
 #include "../../include/jztree_amd.h"
 
@@ -15,6 +16,9 @@ namespace jzt {
 namespace {
 
 constexpr int kTreeThreads = 256;
+constexpr int kFlagPer = 4;  // boundaries per thread in k_flag_leaf: the LDS window's halo is shared by more of them
+constexpr int kFlagSpan = kTreeThreads * kFlagPer;
+constexpr int kFlagChunk = 8;  // levels read per step of k_flag_leaf's outward scans
 
 template <int D, typename T>
 __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict__ posz, const uint8_t* __restrict__ ptype,
@@ -23,13 +27,14 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
                                                             int8_t* __restrict__ split_flags, int32_t* __restrict__ lvl,
                                                             int64_t size_part, int max_size, int scan_size,
                                                             int num_types, const int32_t* __restrict__ seg) {
-  extern __shared__ int32_t s_level[];
-  const int nload = kTreeThreads + 2 * scan_size + 1;
-  uint8_t* s_types = reinterpret_cast<uint8_t*>(s_level + nload);
+  extern __shared__ int32_t s_mem[];
+  const int nload = kFlagSpan + 2 * scan_size + 1;
+  // kFlagChunk unused slots on either side: a scan step may read past the window (masked out)
+  int32_t* const s_level = s_mem + kFlagChunk;
+  uint8_t* s_types = reinterpret_cast<uint8_t*>(s_mem + nload + 2 * kFlagChunk);
   const int64_t nump = npart_p[0];
   const int lb_l = lvl_bound[0], lb_r = lvl_bound[1];
-  const int64_t node_idx = (int64_t)blockIdx.x * kTreeThreads + threadIdx.x;
-  const int64_t ioff = (int64_t)blockIdx.x * kTreeThreads - scan_size - 1;
+  const int64_t ioff = (int64_t)blockIdx.x * kFlagSpan - scan_size - 1;
 
   // s_level[i] = level of boundary ioff+i+1, i.e. between particles ioff+i and ioff+i+1
   for (int i = threadIdx.x; i < nload - 1; i += kTreeThreads) {
@@ -50,48 +55,69 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
   }
   __syncthreads();
 
-  const int idx = threadIdx.x + scan_size;
-  const int my = s_level[idx];
-  bool is_split = false;
-  if (num_types == 1) {
-    // one type: lsize = idx - (the nearest i < idx in the window with s_level[i] >= my), or the whole
-    // window if there is none -- what the reset-and-count loop below leaves; scanning outward from
-    // idx stops there instead of reading the whole window (likewise rsize to the right)
-    int lsize = scan_size;
-    for (int i = idx - 1; i >= idx - scan_size; i--) {
-      if (s_level[i] >= my) {
-        lsize = idx - i;
-        break;
+  for (int rep = 0; rep < kFlagPer; rep++) {
+    const int64_t node_idx = (int64_t)blockIdx.x * kFlagSpan + rep * kTreeThreads + threadIdx.x;
+    const int idx = rep * kTreeThreads + threadIdx.x + scan_size;
+    const int my = s_level[idx];
+    bool is_split = false;
+    if (num_types == 1) {
+      // one type: lsize = idx - (the nearest i < idx in the window with s_level[i] >= my), or the whole
+      // window if there is none -- what the reset-and-count loop below leaves; scanning outward from
+      // idx stops there instead of reading the whole window (likewise rsize to the right).  The scan
+      // reads kFlagChunk levels per step (those past the window land in the pad slots and are
+      // masked out), all issued before any test and without branches, so a wave waits for one LDS
+      // round trip per step instead of one per level.
+      const int32_t* const here = s_level + idx;
+      int lsize = scan_size;
+      for (int base = 1; base <= scan_size; base += kFlagChunk) {
+        int lv[kFlagChunk];
+#pragma unroll
+        for (int u = 0; u < kFlagChunk; u++) lv[u] = here[-(base + u)];
+        const int room = scan_size - base;  // u <= room stays inside the window
+        int hit = 0;
+#pragma unroll
+        for (int u = kFlagChunk - 1; u >= 0; u--) hit = ((u <= room) & (lv[u] >= my)) ? base + u : hit;
+        if (hit) {
+          lsize = hit;
+          break;
+        }
       }
-    }
-    int rsize = scan_size;
-    for (int i = idx + 1; i <= idx + scan_size; i++) {
-      if (s_level[i] >= my) {
-        rsize = i - idx;
-        break;
+      int rsize = scan_size;
+      for (int base = 1; base <= scan_size; base += kFlagChunk) {
+        int lv[kFlagChunk];
+#pragma unroll
+        for (int u = 0; u < kFlagChunk; u++) lv[u] = here[base + u];
+        const int room = scan_size - base;
+        int hit = 0;
+#pragma unroll
+        for (int u = kFlagChunk - 1; u >= 0; u--) hit = ((u <= room) & (lv[u] >= my)) ? base + u : hit;
+        if (hit) {
+          rsize = hit;
+          break;
+        }
       }
-    }
-    is_split |= (lsize + rsize > max_size);
-    is_split |= (my > lb_l) && (node_idx - lsize <= 0);
-    is_split |= (my > lb_r) && (node_idx + rsize >= nump);
-  } else {
-    for (int type = 0; type < num_types; type++) {
-      int lsize = 0;
-      for (int i = idx - scan_size; i < idx; i++) lsize = (s_level[i] < my ? lsize : 0) + (s_types[i] == type ? 1 : 0);
-      int rsize = 0;
-      for (int i = idx + scan_size; i > idx; i--) rsize = (s_level[i] < my ? rsize : 0) + (s_types[i] == type ? 1 : 0);
       is_split |= (lsize + rsize > max_size);
       is_split |= (my > lb_l) && (node_idx - lsize <= 0);
       is_split |= (my > lb_r) && (node_idx + rsize >= nump);
+    } else {
+      for (int type = 0; type < num_types; type++) {
+        int lsize = 0;
+        for (int i = idx - scan_size; i < idx; i++) lsize = (s_level[i] < my ? lsize : 0) + (s_types[i] == type ? 1 : 0);
+        int rsize = 0;
+        for (int i = idx + scan_size; i > idx; i--) rsize = (s_level[i] < my ? rsize : 0) + (s_types[i] == type ? 1 : 0);
+        is_split |= (lsize + rsize > max_size);
+        is_split |= (my > lb_l) && (node_idx - lsize <= 0);
+        is_split |= (my > lb_r) && (node_idx + rsize >= nump);
+      }
     }
-  }
-  is_split |= my >= kLvlSet;  // set boundary of a batch
-  is_split &= node_idx <= nump;
-  is_split |= node_idx == 0;
-  is_split |= node_idx == nump;
-  if (node_idx < size_part + 1) {
-    split_flags[node_idx] = (int8_t)is_split;
-    lvl[node_idx] = my;
+    is_split |= my >= kLvlSet;  // set boundary of a batch
+    is_split &= node_idx <= nump;
+    is_split |= node_idx == 0;
+    is_split |= node_idx == nump;
+    if (node_idx < size_part + 1) {
+      split_flags[node_idx] = (int8_t)is_split;
+      lvl[node_idx] = my;
+    }
   }
 }
 
@@ -210,10 +236,10 @@ int flag_leaf_impl(hipStream_t s, const void* posz, const uint8_t* ptype, const 
                    int8_t* flags, int32_t* lvl, int64_t size_part, int max_size, int scan_size, int num_types,
                    const int32_t* seg) {
   int64_t nb = size_part + 1;
-  int nload = kTreeThreads + 2 * scan_size + 1;
-  size_t shmem = (size_t)nload * (sizeof(int32_t) + 1);
+  int nload = kFlagSpan + 2 * scan_size + 1;
+  size_t shmem = (size_t)(nload + 2 * kFlagChunk) * sizeof(int32_t) + (size_t)nload;
   if (shmem > 60000) return fail("FlagLeafBoundaries: scan_size too large");
-  hipLaunchKernelGGL((k_flag_leaf<D, T>), dim3((unsigned)div_up(nb, kTreeThreads)), dim3(kTreeThreads), shmem, s,
+  hipLaunchKernelGGL((k_flag_leaf<D, T>), dim3((unsigned)div_up(nb, kFlagSpan)), dim3(kTreeThreads), shmem, s,
                      (const T*)posz, ptype, lvl_bound, npart, flags, lvl, size_part, max_size, scan_size, num_types,
                      seg);
   return check_launch("FlagLeafBoundaries");
@@ -259,43 +285,57 @@ __global__ __launch_bounds__(256) void k_reg_hist(const int32_t* __restrict__ lv
     if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
 }
 
-// FP32 evaluation in ascending level order by one thread (the reference reduces under XLA's
-// order; only borderline ties can differ).
-__global__ void k_reg_level(const unsigned long long* __restrict__ hist, int nlvl, int min_lvl, int max_lvl, float fac,
-                            float frac, int32_t* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  float total = 0.f;
-  for (int b = 0; b < nlvl; b++) total += (float)(int64_t)hist[b];
+// FP32 evaluation in ascending level order (the reference reduces under XLA's order; only
+// borderline ties can differ).  The running sums are formed by one thread in that order; the two
+// "first level whose running sum reaches ..." searches then test every level in parallel on those
+// same values (a block-wide minimum of the matching indices), instead of a serial loop with an
+// early exit.
+constexpr int kRegThreads = 256;
+__global__ __launch_bounds__(kRegThreads) void k_reg_level(const unsigned long long* __restrict__ hist, int nlvl,
+                                                           int min_lvl, int max_lvl, float fac, float frac,
+                                                           int32_t* out) {
+  extern __shared__ float s_f[];  // [0, nlvl) counts, [nlvl, 2 nlvl) running sums
+  __shared__ int s_first[2];
+  __shared__ float s_lim;
+  float* s_cnt = s_f;
+  float* s_cum = s_f + nlvl;
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nlvl; b += kRegThreads) s_cnt[b] = (float)(int64_t)hist[b];
+  if (tid == 0) s_first[0] = s_first[1] = INT_MAX;
+  __syncthreads();
+  if (tid == 0) {
+    float c = 0.f;
+    for (int b = 0; b < nlvl; b++) {
+      c += s_cnt[b];
+      s_cum[b] = c;
+    }
+  }
+  __syncthreads();
+  const float total = nlvl > 0 ? s_cum[nlvl - 1] : 0.f;
   const float thresh = frac * total;
-  float csum = 0.f;
-  int ref = max_lvl;
-  for (int b = 0; b < nlvl; b++) {
-    csum += (float)(int64_t)hist[b];
-    if (csum >= thresh) {
-      ref = b + min_lvl;
-      break;
+  for (int b = tid; b < nlvl; b += kRegThreads)
+    if (s_cum[b] >= thresh) atomicMin(&s_first[0], b);
+  __syncthreads();
+  const int ref = s_first[0] == INT_MAX ? max_lvl : s_first[0] + min_lvl;
+  if (tid == 0) {
+    float vol_sum = 0.f, cts_sum = 0.f;
+    for (int b = 0; b < nlvl && b + min_lvl <= ref; b++) {
+      float c = s_cnt[b];
+      float v = ldexpf(1.0f, b + min_lvl - ref);
+      vol_sum += v * c;
+      cts_sum += c;
     }
+    const float avg = vol_sum / cts_sum;
+    s_lim = fac * avg;
   }
-  float vol_sum = 0.f, cts_sum = 0.f;
-  for (int b = 0; b < nlvl && b + min_lvl <= ref; b++) {
-    float c = (float)(int64_t)hist[b];
+  __syncthreads();
+  const float lim = s_lim;
+  for (int b = tid; b < nlvl; b += kRegThreads) {
     float v = ldexpf(1.0f, b + min_lvl - ref);
-    vol_sum += v * c;
-    cts_sum += c;
+    if (s_cum[b] >= thresh && v >= lim) atomicMin(&s_first[1], b);
   }
-  const float avg = vol_sum / cts_sum;
-  const float lim = fac * avg;
-  int res = max_lvl;
-  csum = 0.f;
-  for (int b = 0; b < nlvl; b++) {
-    csum += (float)(int64_t)hist[b];
-    float v = ldexpf(1.0f, b + min_lvl - ref);
-    if (csum >= thresh && v >= lim) {
-      res = b + min_lvl;
-      break;
-    }
-  }
-  out[0] = res;
+  __syncthreads();
+  if (tid == 0) out[0] = s_first[1] == INT_MAX ? max_lvl : s_first[1] + min_lvl;
 }
 
 }  // namespace
@@ -321,8 +361,8 @@ extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl,
                        (unsigned long long*)tmp);
   // 0.01*percentile is formed in double and rounded to float, as the reference's weak typing does
   const float frac = (float)(0.01 * (double)min_percentile);
-  hipLaunchKernelGGL(k_reg_level, dim3(1), dim3(64), 0, s, (const unsigned long long*)tmp, nlvl, min_lvl, max_lvl,
-                     max_volume_fac, frac, out_level);
+  hipLaunchKernelGGL(k_reg_level, dim3(1), dim3(kRegThreads), (size_t)nlvl * 2 * sizeof(float), s,
+                     (const unsigned long long*)tmp, nlvl, min_lvl, max_lvl, max_volume_fac, frac, out_level);
   return check_launch("regularization level");
 }
 
