@@ -278,12 +278,14 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   const int64_t wbase = base + (int64_t)w * 64 * KPT;
 
   uint64_t key[KPT];
+  uint32_t val[KPT];  // loaded with the keys, so the tile waits for global memory once
   uint32_t dig[KPT];
   uint32_t rank[KPT];
 #pragma unroll
   for (int r = 0; r < KPT; r++) {
     int64_t idx = wbase + r * 64 + lane;
     key[r] = idx < n ? kin[idx] : ~0ull;
+    val[r] = idx < n ? vin[idx] : 0u;
     dig[r] = (uint32_t)(key[r] >> shift) & 255u;
   }
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -355,10 +357,8 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
   for (int r = 0; r < KPT; r++) {
-    int64_t idx = wbase + r * 64 + lane;
-    uint32_t v = idx < n ? vin[idx] : 0u;
     uint32_t dd = dig[r];
-    s_vals[s_lstart[dd] + s_wcnt[w][dd] + rank[r]] = v;
+    s_vals[s_lstart[dd] + s_wcnt[w][dd] + rank[r]] = val[r];
   }
   __syncthreads();
   for (int i = tid; i < tile_n; i += kSortThreads) {
