@@ -208,6 +208,13 @@ int jzt_knn_group(jzt_stream_t stream, const int32_t* parent_ilist_spl, const in
                   const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* knn_rad, int32_t* knn_id,
                   int64_t ngroups, int64_t out_rows, int k, float boxsize, int dim, int dtype);
 
+/* One plane of a per-plane packed array (extension; the data.py:251-345 PackedArray.get of the
+ * reference, which jax evaluates as an index + select): out row i = data row ispl[level] + i
+ * while that is below ispl[level + 1], else every element of the row = *fill.  Rows are
+ * row_elems elements of elem_bytes (4 or 8) bytes; ispl is int64 when ispl64, else int32 (device). */
+int jzt_packed_get(jzt_stream_t stream, const void* data, int64_t nrows, const void* ispl, int ispl64, int level,
+                   const void* fill, int elem_bytes, int row_elems, void* out, int64_t size);
+
 /* Segment maximum (extension, used to pass search radii up the tree planes): out[i] = max of
  * vals over [spl[i], spl[i+1]) clamped to [0, nvals), or `empty` for an empty segment.  spl has
  * nseg+1 entries. */

@@ -243,6 +243,22 @@ __global__ void k_dense_sets_fill(const int32_t* __restrict__ nn_dev, int64_t ns
   if (tid == 0) nsuper_out[0] = (int32_t)nsup;
 }
 
+// One plane of a packed per-plane array: out row i = data row (off + i) while off + i < end, else
+// every element of the row = *fill.  off, end = ispl[level], ispl[level + 1] (int32 or int64 on the
+// device).  Rows are row_words 32-bit words; the fill element is elem_words words.
+__global__ void k_packed_get(const uint32_t* __restrict__ data, int64_t nrows, const void* __restrict__ ispl,
+                             int ispl64, int level, const uint32_t* __restrict__ fill, int elem_words, int row_words,
+                             uint32_t* __restrict__ out, int64_t size) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= size * row_words) return;
+  const int64_t i = t / row_words;
+  const int w = (int)(t - i * row_words);
+  const int64_t off = ispl64 ? ((const int64_t*)ispl)[level] : (int64_t)((const int32_t*)ispl)[level];
+  const int64_t end = ispl64 ? ((const int64_t*)ispl)[level + 1] : (int64_t)((const int32_t*)ispl)[level + 1];
+  const int64_t r = off + i;
+  out[t] = (r < end && r >= 0 && r < nrows) ? data[r * row_words + w] : fill[w % elem_words];
+}
+
 static int g_tune[kTuneCount] = {4, 12};  // leaf kNN waves per EU (K <= 32), onesweep keys per thread
 int tune_get(int knob) { return (knob >= 0 && knob < kTuneCount) ? __atomic_load_n(&g_tune[knob], __ATOMIC_RELAXED) : 0; }
 
@@ -311,4 +327,18 @@ extern "C" int jzt_graph_count_nodes(void* graph, int64_t* counts) {
     counts[t == hipGraphNodeTypeKernel ? 0 : t == hipGraphNodeTypeMemcpy ? 1 : t == hipGraphNodeTypeMemset ? 2 : 3]++;
   }
   return 0;
+}
+
+extern "C" int jzt_packed_get(jzt_stream_t stream, const void* data, int64_t nrows, const void* ispl, int ispl64,
+                              int level, const void* fill, int elem_bytes, int row_elems, void* out, int64_t size) {
+  using namespace jzt;
+  if (size <= 0) return 0;
+  if ((elem_bytes != 4 && elem_bytes != 8) || row_elems < 1) return fail("PackedGet: elements of 4 or 8 bytes");
+  if (!data || !ispl || !fill || !out || level < 0) return fail("PackedGet: null argument");
+  const int ew = elem_bytes / 4, rw = ew * row_elems;
+  const int64_t nt = size * rw;
+  hipLaunchKernelGGL(k_packed_get, dim3((unsigned)div_up(nt, 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const uint32_t*)data, nrows, ispl, ispl64, level, (const uint32_t*)fill, ew, rw, (uint32_t*)out,
+                     size);
+  return check_launch("PackedGet");
 }

@@ -137,6 +137,9 @@ class PackedArray:
     def get(self, level, size=None, fill_value=None):
         if size is None:
             size = self.size()
+        fast = self._get_device(level, size, fill_value)
+        if fast is not None:
+            return fast
         idx = torch.arange(size, dtype=torch.int64, device=self.data.device) + self.ispl[level]
         valid = idx < self.ispl[level + 1]
         vals = self.data[torch.clamp(idx, max=len(self.data) - 1)]
@@ -148,6 +151,36 @@ class PackedArray:
         if self.data.ndim > 1:
             valid = valid.reshape((-1,) + (1,) * (self.data.ndim - 1))
         return torch.where(valid, vals, fill)
+
+    _GET_DTYPES = {torch.int32: 4, torch.float32: 4, torch.int64: 8, torch.float64: 8}
+
+    def _get_device(self, level, size, fill_value):
+        """get() as one kernel (jzt_packed_get) for device data of 4- or 8-byte elements and a
+        scalar fill; None where that does not apply (the caller then uses the torch ops)."""
+        d = self.data
+        esize = self._GET_DTYPES.get(d.dtype)
+        if not isinstance(level, int) or not isinstance(size, int) or not 0 <= level < len(self.ispl) - 1:
+            return None
+        if not d.is_cuda or esize is None or self.ispl.dtype not in (torch.int32, torch.int64) or not self.ispl.is_cuda:
+            return None
+        if fill_value is None:
+            if (self.fill_values is None or not self.fill_values.is_cuda or self.fill_values.dtype != d.dtype
+                    or level >= len(self.fill_values)):
+                return None
+            fill = self.fill_values.contiguous()[level:level + 1]
+        elif torch.is_tensor(fill_value):
+            if fill_value.numel() != 1:
+                return None
+            fill = fill_value.to(device=d.device, dtype=d.dtype).reshape(1).contiguous()
+        else:
+            fill = torch.full((1,), fill_value, dtype=d.dtype, device=d.device)
+        from . import _lib as L
+        d = d.contiguous()
+        row = int(np.prod(d.shape[1:])) if d.ndim > 1 else 1
+        out = torch.empty((size,) + tuple(d.shape[1:]), dtype=d.dtype, device=d.device)
+        L.call("jzt_packed_get", L.stream(), L.ptr(d), d.shape[0], L.ptr(self.ispl.contiguous()),
+               int(self.ispl.dtype == torch.int64), int(level), L.ptr(fill), esize, max(row, 1), L.ptr(out), int(size))
+        return out
 
     def set(self, level, values, num=None, fill_value=None):
         """data.py:300-307 (values written at ispl[level], later levels start after them)."""

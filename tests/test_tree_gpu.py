@@ -103,3 +103,27 @@ def test_leaf_allocation_overflow_matches_oracle(dev, oracle):
         jz.tree.build_tree_hierarchy(_t(pz, dev), KNNConfig().tree)
         jz.checks.flush()
     assert str(e_ref.value).split(".")[0] in str(e_gpu.value)
+
+
+@pytest.mark.parametrize("dtype,row", [(torch.int32, ()), (torch.float32, (3,)), (torch.float64, (3,)),
+                                       (torch.int64, ()), (torch.float32, ())])
+@pytest.mark.parametrize("ispl_dtype", [torch.int32, torch.int64])
+def test_packed_get_kernel_matches_torch_ops(dev, dtype, row, ispl_dtype):
+    """PackedArray.get on the device (one jzt_packed_get launch) equals the index + select torch
+    ops that the same call runs on CPU tensors, for every plane, sizes past the plane's end, and
+    fills from the array, a scalar and a one-element tensor."""
+    from jztree_amd.data import PackedArray
+    g = torch.Generator().manual_seed(3)
+    n = 1000
+    data = (torch.randn((n,) + row, generator=g) * 100).to(dtype)
+    ispl = torch.tensor([0, 10, 10, 400, 997], dtype=ispl_dtype)
+    fills = torch.tensor([-1, 7, -2000, 5], dtype=torch.float64).to(dtype)
+    cpu = PackedArray(data, ispl, fills)
+    gpu = PackedArray(data.to(dev), ispl.to(dev), fills.to(dev))
+    for level in range(4):
+        for size in (0, 1, 5, 390, 1000, 1500):
+            for fv in (None, 3, torch.tensor(9, dtype=torch.int32)):
+                a = cpu.get(level, size, fv)
+                b = gpu.get(level, size, fv)
+                assert b.dtype == a.dtype and b.shape == a.shape
+                assert torch.equal(b.cpu(), a), (level, size, fv)
