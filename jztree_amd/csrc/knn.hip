@@ -113,11 +113,18 @@ template <int KB> struct RmaxBins {
   }
 };
 
+#ifndef JZT_RMAX_GROUP
+#define JZT_RMAX_GROUP 16
+#endif
+constexpr int kRmaxGroup = JZT_RMAX_GROUP;  // children per lane group of k_find_rmax_groups (8, 16 or 32)
+static_assert(kRmaxGroup == 8 || kRmaxGroup == 16 || kRmaxGroup == 32, "lane group size");
+
 template <int KB, int D, typename T>
 __global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pil_spl, const int32_t* __restrict__ pil_src,
                                                   const float* __restrict__ pil_r2, const int32_t* __restrict__ parent_spl,
                                                   const T* __restrict__ nodes, const int32_t* __restrict__ npart,
-                                                  float* __restrict__ rmax_out, int k, T boxsize, int64_t pil_size, int64_t nitems) {
+                                                  float* __restrict__ rmax_out, int k, T boxsize, int64_t pil_size, int64_t nitems,
+                                                  int skip_le) {
   __shared__ Vec<D, T> s_x[64];
   __shared__ Vec<D, T> s_e[64];
   __shared__ int s_n[64];
@@ -125,6 +132,7 @@ __global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pi
   const int64_t nact_ = active_items(parent_spl, nitems);
   for (int pq = blockIdx.x; pq < nact_; pq += gridDim.x) {
     const int q0 = parent_spl[pq], q1 = parent_spl[pq + 1];
+    if (q1 - q0 <= skip_le) continue;  // done by k_find_rmax_groups
     const int e0 = pil_spl[pq], e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
     for (int qoff = q0; qoff < q1; qoff += 64) {
       const int iq = min(qoff + lane, q1 - 1);
@@ -158,6 +166,89 @@ __global__ __launch_bounds__(64) void k_find_rmax(const int32_t* __restrict__ pi
       if (qoff + lane < q1) rmax_out[iq] = rmax2;
     }
   
+  }
+}
+
+// FindRmax for parents with at most G children, 64 / G of them per wave: lane group g (G lanes)
+// takes parent base + g, lane j of the group its child j.  Each group does exactly what
+// k_find_rmax does for such a parent -- the same entries in list order, the same break test (no
+// child of the parent reaches the next entry), the same targets in order -- so every child's bins
+// see the same sequence and the radii are identical; only the lanes are shared.  The loop is
+// wave-uniform: each step every unfinished group stages (up to G targets of) its current entry
+// in its own LDS slice, and its lanes consider them.
+template <int KB, int D, typename T, int G>
+__global__ __launch_bounds__(64) void k_find_rmax_groups(const int32_t* __restrict__ pil_spl,
+                                                         const int32_t* __restrict__ pil_src,
+                                                         const float* __restrict__ pil_r2,
+                                                         const int32_t* __restrict__ parent_spl,
+                                                         const T* __restrict__ nodes, const int32_t* __restrict__ npart,
+                                                         float* __restrict__ rmax_out, int k, T boxsize,
+                                                         int64_t pil_size, int64_t nitems) {
+  constexpr int NG = 64 / G;
+  __shared__ Vec<D, T> s_x[64];
+  __shared__ Vec<D, T> s_e[64];
+  __shared__ int s_n[64];
+  const int lane = threadIdx.x, g = lane / G, j = lane - g * G;
+  const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (g * G);
+  const int64_t nact_ = active_items(parent_spl, nitems);
+  const int64_t nwaves = (nact_ + NG - 1) / NG;
+  for (int64_t wv = blockIdx.x; wv < nwaves; wv += gridDim.x) {
+    const int64_t pq = wv * NG + g;
+    int q0 = 0, nc = 0, e = 0, e1 = 0;
+    if (pq < nact_) {
+      q0 = parent_spl[pq];
+      nc = parent_spl[pq + 1] - q0;
+      e = pil_spl[pq];
+      e1 = (int)min((int64_t)pil_spl[pq + 1], pil_size);
+    }
+    // a group with more than G children (k_find_rmax's) or none has nothing to do
+    bool gdone = nc <= 0 || nc > G;
+    const bool valid = !gdone && j < nc;
+    const int iq = q0 + min(j, max(nc - 1, 0));
+    NodeT<D, T> nq = load_node<D, T>(nodes, valid ? iq : 0);
+    const Vec<D, T> eq = lvl_half_ext<D, T>(nq.level);
+    RmaxBins<KB> bins(k);
+    float rmax2 = INFINITY;
+    int toff = 0, t1 = 0;  // current entry's target range; toff >= t1: fetch the next entry
+    while (__ballot(!gdone) != 0ull) {
+      if (!gdone && toff >= t1) {
+        bool go = e < e1;
+        if (go) {
+          const float r2t = pil_r2[e];
+          go = (__ballot(valid && r2t <= rmax2) & gmask) != 0ull;
+        }
+        if (go) {
+          const int pt = pil_src[e];
+          toff = parent_spl[pt];
+          t1 = parent_spl[pt + 1];
+          e++;
+        } else {
+          gdone = true;
+        }
+      }
+      const int it = toff + j;
+      if (!gdone && it < t1 && j < G) {
+        NodeT<D, T> nt = load_node<D, T>(nodes, it);
+        s_x[lane] = nt.cent;
+        s_e[lane] = lvl_half_ext<D, T>(nt.level);
+        s_n[lane] = npart[it];
+      }
+      __syncthreads();
+      if (!gdone) {
+        const int nl = min(t1 - toff, G);
+        if (valid) {
+          for (int jj = 0; jj < nl; jj++) {
+            const int sl = g * G + jj;
+            float r2 = (float)maxdist2<D, T>(s_x[sl], nq.cent, add<D, T>(eq, s_e[sl]), boxsize);
+            bins.consider(r2, s_n[sl]);
+          }
+        }
+        toff += G;
+        if (toff >= t1) rmax2 = bins.max_r2(k) * kRmaxSlack;  // the entry is done
+      }
+      __syncthreads();
+    }
+    if (valid) rmax_out[iq] = rmax2;
   }
 }
 
@@ -637,9 +728,22 @@ int node2node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src
     if (size_nodes > 0)
       hipLaunchKernelGGL(k_scale_rmax2, dim3((unsigned)div_up(size_nodes, 256)), dim3(256), 0, s, rmax2_in, rmax2,
                          size_nodes);
-  } else if (k <= 16) hipLaunchKernelGGL((k_find_rmax<8, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
-  else if (k <= 32) hipLaunchKernelGGL((k_find_rmax<16, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
-  else hipLaunchKernelGGL((k_find_rmax<32, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, k, (T)boxsize, pil_size, size_parents);
+  } else {
+    // parents with at most kRmaxGroup children: several per wave; the others: one wave each
+    constexpr int G = kRmaxGroup;
+    dim3 gg(grid_for(div_up(size_parents, 64 / G)));
+#define JZT_FIND_RMAX(KB)                                                                                           \
+  do {                                                                                                              \
+    hipLaunchKernelGGL((k_find_rmax_groups<KB, D, T, G>), gg, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd,   \
+                       npart, rmax2, k, (T)boxsize, pil_size, size_parents);                                        \
+    hipLaunchKernelGGL((k_find_rmax<KB, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, npart, rmax2, \
+                       k, (T)boxsize, pil_size, size_parents, G);                                                   \
+  } while (0)
+    if (k <= 16) JZT_FIND_RMAX(8);
+    else if (k <= 32) JZT_FIND_RMAX(16);
+    else JZT_FIND_RMAX(32);
+#undef JZT_FIND_RMAX
+  }
   if (check_launch("KnnNode2Node FindRmax")) return 1;
   hipLaunchKernelGGL((k_count_insert<0, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, parent_spl, nd, rmax2, il_spl + 1,
                      (const int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr, il_size, (T)boxsize, pil_size, size_parents);
