@@ -37,18 +37,29 @@ __device__ __forceinline__ int uf_find(const int32_t* p, int x) {
   }
 }
 
-// Joins the sets of a and b; returns the root the two were joined under (a root at the time of the
-// join: later links elsewhere may hook it further).
+// Roots of the sets of a and b, following both parent chains in lock step so that their loads
+// overlap (each step waits for one memory round trip, not two).
+__device__ __forceinline__ void uf_find2(const int32_t* p, int& a, int& b) {
+  while (true) {
+    const int pa = uf_parent(p, a), pb = uf_parent(p, b);
+    if (pa == a && pb == b) return;
+    a = pa;
+    b = pb;
+  }
+}
+
+// Joins the sets of a and b (any members of them); returns the root the two were joined under (a
+// root at the time of the join: later links elsewhere may hook it further).
 __device__ __forceinline__ int uf_link(int32_t* p, int a, int b) {
-  a = uf_find(p, a);
-  b = uf_find(p, b);
+  uf_find2(p, a, b);
   while (a != b) {
     int hi = a > b ? a : b, lo = a > b ? b : a;
     int old = atomicMin(p + hi, lo);
     if (old == hi || old == lo) return lo;  // hooked (or already joined to lo)
     // hi had been hooked to `old` meanwhile; join old's tree with lo and go on
-    a = uf_find(p, old);
-    b = uf_find(p, lo);
+    a = old;
+    b = lo;
+    uf_find2(p, a, b);
   }
   return a;
 }
@@ -278,16 +289,32 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
               const int gj = s_r[j].g;
               // labels are snapshots of set members; a label equal to q's own, or to one q was
               // already joined with, belongs to q's set by now
-              if (gj == gq || gj == glast || gj == glast2) continue;
-              Vec<D, T> xp;
+              bool want = false;
+              if (!(gj == gq || gj == glast || gj == glast2)) {
+                Vec<D, T> xp;
   #pragma unroll
-              for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
-              const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
-              if (r2 <= (T)r2link) {
-                const int root = uf_link(ig, iq, toff + j);
-                glast2 = glast;
-                glast = gj;
-                gq = root;
+                for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
+                const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
+                want = r2 <= (T)r2link;
+              }
+              // Lanes whose q holds the same label are in one set: a single link of that set with
+              // p's joins them all, so one lane links and the others take its root.
+              uint64_t wm = __ballot(want);
+              while (wm) {
+                const int L = __builtin_ctzll(wm);
+                const bool mine = want && gq == __builtin_amdgcn_readlane(gq, L);
+                const uint64_t mm = __ballot(mine);
+                int root = 0;
+                // from q's label and p's (members of the two sets): one parent step shorter each
+                if (lane == L) root = uf_link(ig, gq, gj);
+                root = __builtin_amdgcn_readlane(root, L);
+                if (mine) {
+                  glast2 = glast;
+                  glast = gj;
+                  gq = root;
+                  want = false;
+                }
+                wm &= ~mm;
               }
             }
           }
