@@ -255,9 +255,78 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
       // the last two labels it was joined with
       int gq = ig[iq];
       int glast = -1, glast2 = -1;
+      auto own_leaf_links = [&](const Vec<D, T>& x, bool ok, int n, const Vec<D, T>& lo, const Vec<D, T>& hi, int& g,
+                                int& gl, int& gl2) {
+        bool need_wrap = false;
+        if (ok) {
+#pragma unroll
+          for (int d = 0; d < D; d++)
+            if (PER) need_wrap |= !(lo[d] - x[d] >= -half) || !(hi[d] - x[d] < half);
+          FofRec<D, T> r;
+#pragma unroll
+          for (int d = 0; d < D; d++) r.p[d] = x[d];
+          r.g = g;
+          s_r[lane] = r;
+        }
+        const bool wrap = PER && wave_any(need_wrap);
+        __syncthreads();
+        // neighbours within rlink (the same canonical r2 and test as for any other source chunk)
+        uint64_t nb = 0ull;
+        if (ok) {
+          for (int j = 0; j < n; j++) {
+            Vec<D, T> xp;
+#pragma unroll
+            for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
+            const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
+            if (j != lane && r2 <= (T)r2link) nb |= 1ull << j;
+          }
+        }
+        __syncthreads();
+        // component of each particle: closure of the neighbour sets (c grows by the sets of its
+        // members, so the reach doubles per round)
+        uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
+        bool changed = true;
+        while (wave_any(changed)) {
+          uint64_t cn = c;
+          for (int j = 0; j < n; j++) {
+            const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+            if ((c >> j) & 1ull) cn |= cj;
+          }
+          changed = cn != c;
+          c = cn;
+        }
+        const int lroot = ok ? __builtin_ctzll(c) : lane;
+        const int groot = __shfl(g, lroot, 64);  // label of the component's first particle
+        bool want = ok && lroot != lane && g != groot;
+        uint64_t wm = __ballot(want);
+        while (wm) {
+          const int L = __builtin_ctzll(wm);
+          const int gL = __builtin_amdgcn_readlane(g, L), rL = __builtin_amdgcn_readlane(groot, L);
+          const bool mine = want && g == gL && groot == rL;
+          const uint64_t mm = __ballot(mine);
+          int root = 0;
+          if (lane == L) root = uf_link(ig, gL, rL);
+          root = __builtin_amdgcn_readlane(root, L);
+          if (mine) {
+            gl2 = gl;
+            gl = groot;
+            g = root;
+            want = false;
+          }
+          wm &= ~mm;
+        }
+      };
       for (int e = e0; e < e1; e++) {
         const int lt = il[e];
         if (lt < lq) continue;  // each leaf pair once
+        if (lt == lq && q1 - q0 <= 64) {
+          // The leaf with itself, in one chunk: its particles' links are resolved in the wave
+          // (components of the within-rlink graph of the <= 64 particles), and only one member
+          // per (label, component label) pair is linked in global memory.
+          own_leaf_links(xq, valid, q1 - q0, qlo, qhi, gq, glast, glast2);
+          continue;
+        }
         const int t0 = spl[lt], t1 = spl[lt + 1];
         for (int toff = t0; toff < t1; toff += 64) {
           const int it = toff + lane;
