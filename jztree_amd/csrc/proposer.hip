@@ -9,10 +9,11 @@
 // bf16 (the trainable fp32 master copy is rounded to nearest-even after every Adam step), all
 // arithmetic and accumulation in fp32.
 //
-// Work layout of the MLP kernels: one wave per tile of 64 samples.  Activations of a tile live in
-// LDS transposed (row = neuron, column = sample, row stride 65 floats so that a wave writing one
+// Work layout of the training kernel: one wave per tile of 64 samples.  Activations of a tile live
+// in LDS transposed (row = neuron, column = sample, row stride 65 floats so that a wave writing one
 // neuron row per lane hits 64 distinct banks); a layer is computed with lane = output neuron,
 // reading each input row by broadcast and keeping the 64 per-sample accumulators in registers.
+// Inference has no backward pass to feed and runs one sample per lane instead (k_mlp_infer).
 #include "util.h"
 #include "../../include/jztree_amd.h"
 
@@ -124,49 +125,73 @@ __device__ __forceinline__ void layer_acc(float (&acc)[64], const float* __restr
 // ------------------------------------------------------------------------------------------------
 
 // norm = {feature mean[12], feature std[12], target mean, target std}
-__global__ __launch_bounds__(64) void k_mlp_infer(const uint16_t* __restrict__ wq, const float* __restrict__ norm,
-                                                  const float* __restrict__ feat, const int32_t* __restrict__ npart0,
-                                                  int64_t n, float margin, float* __restrict__ out_r2,
-                                                  float* __restrict__ out_log2r) {
-  __shared__ float s_a[kNH * kLd];
-  __shared__ float s_b[kNH * kLd];
-  __shared__ uint16_t s_w[kNParam];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < kNParam; i += 64) s_w[i] = wq[i];
-  const float ym = norm[2 * kNF], ys = norm[2 * kNF + 1];
-  __syncthreads();
-  float acc[64];
-  for (int64_t tile = blockIdx.x; tile * 64 < n; tile += gridDim.x) {
-    const int64_t s = tile * 64 + lane;
+//
+// Inference runs one sample per lane, so all lanes do useful work.  The weights, converted to fp32
+// once per block, sit in LDS and every lane reads the same address (a broadcast); the lane's input
+// activations sit in its own LDS column (written and read by that lane only) and the 64
+// accumulators stay in registers.  Each output neuron accumulates b[k] + sum_j in[j] * W[j][k] in
+// ascending j with fmaf, the order of layer_acc, so the results are the training forward pass's,
+// bit for bit.
+constexpr int kInferThreads = 128;
+template <int NIN>
+__device__ __forceinline__ void layer_lane(float (&out)[kNH], const float* __restrict__ col, const float* __restrict__ w,
+                                           const float* __restrict__ b) {
 #pragma unroll
-    for (int c = 0; c < kNF; c++)
-      s_a[c * kLd + lane] = s < n ? (feat[s * kNF + c] - norm[c]) / norm[kNF + c] : 0.f;
-    __syncthreads();
-    layer_acc<kNF>(acc, s_a, s_w + kOffW1, s_w + kOffB1, lane);
+  for (int k = 0; k < kNH; k++) out[k] = b[k];
+#pragma unroll 1
+  for (int j = 0; j < NIN; j++) {
+    const float x = col[j * kInferThreads];
+    const float4* wr = reinterpret_cast<const float4*>(w + j * kNH);
 #pragma unroll
-    for (int t = 0; t < 64; t++) s_b[lane * kLd + t] = gelu(acc[t]);
-    __syncthreads();
-    layer_acc<kNH>(acc, s_b, s_w + kOffW2, s_w + kOffB2, lane);
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 64; t++) s_a[lane * kLd + t] = gelu(acc[t]);
-    __syncthreads();
-    layer_acc<kNH>(acc, s_a, s_w + kOffW3, s_w + kOffB3, lane);
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 64; t++) s_b[lane * kLd + t] = gelu(acc[t]);
-    __syncthreads();
-    float y = bf16_to_f32(s_w[kOffB4]);
-    for (int j = 0; j < kNH; j++) y = fmaf(s_b[j * kLd + lane], bf16_to_f32(s_w[kOffW4 + j]), y);
-    if (s < n) {
-      const float lr = fmaf(y, ys, ym);
-      if (out_log2r) out_log2r[s] = lr;
-      if (out_r2) {
-        const float r = exp2f(lr + margin);
-        out_r2[s] = npart0 && npart0[s] <= 0 ? 0.f : r * r;
-      }
+    for (int k4 = 0; k4 < kNH / 4; k4++) {
+      const float4 w4 = wr[k4];
+      out[4 * k4 + 0] = fmaf(x, w4.x, out[4 * k4 + 0]);
+      out[4 * k4 + 1] = fmaf(x, w4.y, out[4 * k4 + 1]);
+      out[4 * k4 + 2] = fmaf(x, w4.z, out[4 * k4 + 2]);
+      out[4 * k4 + 3] = fmaf(x, w4.w, out[4 * k4 + 3]);
     }
-    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kInferThreads) void k_mlp_infer(const uint16_t* __restrict__ wq,
+                                                             const float* __restrict__ norm,
+                                                             const float* __restrict__ feat,
+                                                             const int32_t* __restrict__ npart0, int64_t n,
+                                                             float margin, float* __restrict__ out_r2,
+                                                             float* __restrict__ out_log2r) {
+  __shared__ __align__(16) float s_w[(kNParam + 3) / 4 * 4];
+  __shared__ float s_act[kNH * kInferThreads];  // column t: thread t's activations
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kNParam; i += kInferThreads) s_w[i] = bf16_to_f32(wq[i]);
+  __syncthreads();
+  const int64_t s = (int64_t)blockIdx.x * kInferThreads + tid;
+  const bool live = s < n && !(npart0 && out_r2 && !out_log2r && npart0[s] <= 0);
+  if (!__ballot(live)) {  // a wave of empty leaf slots only: their bound is 0
+    if (s < n && out_r2) out_r2[s] = 0.f;
+    return;
+  }
+  float* col = s_act + tid;
+  const float ym = norm[2 * kNF], ys = norm[2 * kNF + 1];
+#pragma unroll
+  for (int c = 0; c < kNF; c++) col[c * kInferThreads] = s < n ? (feat[s * kNF + c] - norm[c]) / norm[kNF + c] : 0.f;
+  float h[kNH];
+  layer_lane<kNF>(h, col, s_w + kOffW1, s_w + kOffB1);
+#pragma unroll
+  for (int k = 0; k < kNH; k++) col[k * kInferThreads] = gelu(h[k]);
+  layer_lane<kNH>(h, col, s_w + kOffW2, s_w + kOffB2);
+#pragma unroll
+  for (int k = 0; k < kNH; k++) col[k * kInferThreads] = gelu(h[k]);
+  layer_lane<kNH>(h, col, s_w + kOffW3, s_w + kOffB3);
+  float y = s_w[kOffB4];
+#pragma unroll
+  for (int j = 0; j < kNH; j++) y = fmaf(gelu(h[j]), s_w[kOffW4 + j], y);
+  if (s < n) {
+    const float lr = fmaf(y, ys, ym);
+    if (out_log2r) out_log2r[s] = lr;
+    if (out_r2) {
+      const float r = exp2f(lr + margin);
+      out_r2[s] = npart0 && npart0[s] <= 0 ? 0.f : r * r;
+    }
   }
 }
 
@@ -329,10 +354,8 @@ extern "C" int jzt_leaf_features(jzt_stream_t stream, const int32_t* npart0, con
 extern "C" int jzt_mlp_infer(jzt_stream_t stream, const uint16_t* weights, const float* norm, const float* feat,
                              const int32_t* npart, int64_t n, float margin, float* out_r2, float* out_log2r) {
   if (n <= 0) return 0;
-  const int64_t tiles = div_up(n, 64);
-  const unsigned grid = (unsigned)(tiles < 4096 ? tiles : 4096);
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64), 0, (hipStream_t)stream, weights, norm, feat, npart, n, margin,
-                     out_r2, out_log2r);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)div_up(n, kInferThreads)), dim3(kInferThreads), 0,
+                     (hipStream_t)stream, weights, norm, feat, npart, n, margin, out_r2, out_log2r);
   return check_launch("MlpInfer");
 }
 
