@@ -215,6 +215,15 @@ int jzt_knn_group(jzt_stream_t stream, const int32_t* parent_ilist_spl, const in
 int jzt_packed_get(jzt_stream_t stream, const void* data, int64_t nrows, const void* ispl, int ispl64, int level,
                    const void* fill, int elem_bytes, int row_elems, void* out, int64_t size);
 
+/* Guaranteed k-neighbour radius^2 of the nodes of one tree plane (extension, the start bound of
+ * the bounded kNN search), planes taken from the top down: cap[c] = squared diagonal (float64) of
+ * node c's cell if it holds >= k particles (npart = spl[n2l[c+1]] - spl[n2l[c]], indices clamped
+ * to nspl - 1), else cap_up of its parent in the plane above (the last P with n2c_up[P] <= c,
+ * clamped into [0, size)); +inf at the top plane (n2c_up = cap_up = NULL).  n2l has size+1
+ * entries, n2c_up size+1 (non-decreasing). */
+int jzt_plane_cap(jzt_stream_t stream, const int32_t* n2l, const int32_t* spl, int64_t nspl, const int32_t* lvl,
+                  const int32_t* n2c_up, const double* cap_up, int64_t size, int k, int dim, double* cap);
+
 /* Segment maximum (extension, used to pass search radii up the tree planes): out[i] = max of
  * vals over [spl[i], spl[i+1]) clamped to [0, nvals), or `empty` for an empty segment.  spl has
  * nseg+1 entries. */

@@ -704,6 +704,39 @@ __global__ void k_scale_rmax2(const float* __restrict__ in, float* __restrict__ 
   if (i < n) out[i] = in[i] * kRmaxSlack;
 }
 
+// Guaranteed k-neighbour radius^2 of the nodes of one plane, planes taken from the top down:
+// cap[c] = squared cell diagonal of node c if it holds >= k particles, else its parent's cap (the
+// parent: the last node P of the plane above with n2c_up[P] <= c, clamped into [0, size)), +inf
+// at the top plane.  npart[c] = spl[n2l[c+1]] - spl[n2l[c]] (indices clamped to the spl array).
+// The diagonal^2 sums 4^l_i over the per-dim levels of lvl_vec, in float64 (exact).
+__global__ void k_plane_cap(const int32_t* __restrict__ n2l, const int32_t* __restrict__ spl, int64_t nspl,
+                            const int32_t* __restrict__ lvl, const int32_t* __restrict__ n2c_up,
+                            const double* __restrict__ cap_up, int64_t size, int k, int dim, double* __restrict__ cap) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= size) return;
+  const int64_t a = min((int64_t)n2l[c], nspl - 1), b = min((int64_t)n2l[c + 1], nspl - 1);
+  const int64_t np = (int64_t)spl[b] - (int64_t)spl[a];
+  double r;
+  if (np >= k) {
+    const int o = (lvl[c] + 2000 * dim) / dim - 2000;
+    const int rem = lvl[c] - o * dim;
+    r = 0.0;
+    for (int i = 0; i < dim; i++) r += ldexp(1.0, 2 * (o + (rem >= dim - i ? 1 : 0)));
+  } else if (!n2c_up) {
+    r = INFINITY;
+  } else {
+    int64_t lo = 0, hi = size + 1;  // count of n2c_up entries <= c
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (n2c_up[mid] <= c) lo = mid + 1;
+      else hi = mid;
+    }
+    const int64_t parent = min(max(lo - 1, (int64_t)0), size - 1);
+    r = cap_up[parent];
+  }
+  cap[c] = r;
+}
+
 // out[i] = max of v over the segment [spl[i], spl[i+1]) (clamped to [0, nv)), `empty` if it is empty
 __global__ void k_segment_max(const int32_t* __restrict__ spl, const float* __restrict__ v, int64_t nv,
                               float* __restrict__ out, int64_t nseg, float empty) {
@@ -850,6 +883,17 @@ extern "C" int jzt_segment_max_f32(jzt_stream_t stream, const int32_t* spl, cons
   hipLaunchKernelGGL(k_segment_max, dim3((unsigned)div_up(nseg, 256)), dim3(256), 0, (hipStream_t)stream, spl, vals,
                      nvals, out, nseg, empty);
   return check_launch("SegmentMax");
+}
+
+extern "C" int jzt_plane_cap(jzt_stream_t stream, const int32_t* n2l, const int32_t* spl, int64_t nspl,
+                             const int32_t* lvl, const int32_t* n2c_up, const double* cap_up, int64_t size, int k,
+                             int dim, double* cap) {
+  if (size <= 0) return 0;
+  if (nspl < 1 || dim < 1) return fail("PlaneCap: empty splits or dim < 1");
+  if ((n2c_up == nullptr) != (cap_up == nullptr)) return fail("PlaneCap: n2c_up and cap_up go together");
+  hipLaunchKernelGGL(k_plane_cap, dim3((unsigned)div_up(size, 256)), dim3(256), 0, (hipStream_t)stream, n2l, spl, nspl,
+                     lvl, n2c_up, cap_up, size, k, dim, cap);
+  return check_launch("PlaneCap");
 }
 
 extern "C" size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype) {

@@ -251,37 +251,26 @@ def _knn_group(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     return rnn, inn
 
 
-def _cell_diag2(lvl: torch.Tensor, dim: int) -> torch.Tensor:
-    """Squared diagonal of the Morton cell of each level (float64): per-dim sides 2^l_i with the
-    level split as in lvl_vec (math.cuh:524-541)."""
-    lv = lvl.to(torch.int64)
-    o = torch.div(lv + 2000 * dim, dim, rounding_mode="floor") - 2000
-    r = lv - o * dim
-    d2 = torch.zeros(lv.shape, dtype=torch.float64, device=lv.device)
-    for i in range(dim):
-        li = o + (r >= dim - i).to(torch.int64)
-        d2 = d2 + torch.pow(4.0, li.to(torch.float64))
-    return d2
-
-
 def _guaranteed_leaf_r2(th: TreeHierarchy, k: int) -> torch.Tensor:
     """Per leaf slot, a radius^2 within which every query of the leaf has its k nearest (itself
     included): the squared cell diagonal of the smallest enclosing node (the leaf or an ancestor)
     that holds >= k particles (all of them lie in that cell, and the periodic wrap only shortens
-    differences); +inf when no node holds k particles."""
+    differences); +inf when no node holds k particles.  One jzt_plane_cap launch per plane, from
+    the top plane down (a node takes its parent's value unless it holds k particles itself)."""
     size = th.size()
     dim = th.geom_cent.data.shape[-1]
     dev = th.lvl.data.device
-    cap = torch.full((size,), math.inf, dtype=torch.float64, device=dev)
-    idx = torch.arange(size, device=dev)
-    for p in range(th.num_planes()):
-        npart = th.npart(p, size=size)
-        d2 = _cell_diag2(th.lvl.get(p, size), dim)
-        take = (npart[idx] >= k) & torch.isinf(cap)
-        cap = torch.where(take, d2[idx], cap)
-        if p + 1 < th.num_planes():
-            idx = _leaf_parent(th.ispl_n2n.get(p + 1, size + 1), size).long()[idx]
-    return cap
+    spl = th.splits_leaf_to_part().to(torch.int32).contiguous()
+    cap_up = n2c_up = None
+    for p in reversed(range(th.num_planes())):
+        n2l = th.ispl_n2l.get(p, size + 1).to(torch.int32).contiguous()
+        lvl = th.lvl.get(p, size).to(torch.int32).contiguous()
+        cap = torch.empty(size, dtype=torch.float64, device=dev)
+        L.call("jzt_plane_cap", L.stream(), L.ptr(n2l), L.ptr(spl), spl.numel(), L.ptr(lvl), L.ptr(n2c_up),
+               L.ptr(cap_up), size, int(k), int(dim), L.ptr(cap))
+        cap_up = cap
+        n2c_up = th.ispl_n2n.get(p, size + 1).to(torch.int32).contiguous() if p > 0 else None
+    return cap_up
 
 
 def _plane_radii(th: TreeHierarchy, leaf_r2: torch.Tensor, upper_r2: torch.Tensor | None = None) -> list:

@@ -1,5 +1,7 @@
 """Exact kNN on the GPU vs the oracle: neighbour ids bit-exact (ties by index), r2 bit-exact,
 radii = correctly rounded sqrt of r2 and within 2 ulp of the float64 distance."""
+import dataclasses
+import math
 import numpy as np
 import pytest
 import torch
@@ -241,3 +243,40 @@ def test_start_bound_fallback(dev, k, boxsize):
         r1, i1 = jz.knn._knn_leaf_start_bound(il, spl, partz, k, boxsize, leaf_r2 * fac, originz.idx)
         torch.cuda.synchronize()
         assert torch.equal(i1, iz) and torch.equal(r1, rz), f"factor {fac}"
+
+
+def _cell_diag2_ref(lvl, dim):
+    """Squared cell diagonal per level: sum of 4^l_i over the per-dim levels (python floor
+    division as lvl_vec), exact powers of two from numpy.ldexp (a device pow is not exact)."""
+    lv = lvl.cpu().numpy().astype(np.int64)
+    o = (lv + 2000 * dim) // dim - 2000
+    r = lv - o * dim
+    d2 = np.zeros(lv.shape, dtype=np.float64)
+    for i in range(dim):
+        d2 = d2 + np.ldexp(1.0, 2 * (o + (r >= dim - i)))
+    return torch.from_numpy(d2).to(lvl.device)
+
+
+@pytest.mark.parametrize("n,leaf,k", [(1 << 14, 32, 32), (1 << 17, 64, 32), (1 << 17, 16, 40)])
+def test_guaranteed_leaf_r2_matches_ancestor_walk(dev, n, leaf, k):
+    """The per-plane kernel (jzt_plane_cap, top-down) equals walking each leaf up its ancestors
+    to the first node with >= k particles and taking that cell's squared diagonal."""
+    import jztree_amd as jz
+    from jztree_amd.knn import _guaranteed_leaf_r2, _leaf_parent
+    x = torch.from_numpy(clustered(n, nhalos=64, seed=11)).to(dev)
+    cfg = jz.config.KNNConfig()
+    tcfg = dataclasses.replace(cfg.tree, max_leaf_size=leaf, alloc_fac_nodes=2.0)
+    _, th = jz.tree.zsort_and_tree(x, tcfg)
+    size = th.size()
+    ref = torch.full((size,), math.inf, dtype=torch.float64, device=dev)
+    idx = torch.arange(size, device=dev)
+    for p in range(th.num_planes()):
+        npart = th.npart(p, size=size)
+        d2 = _cell_diag2_ref(th.lvl.get(p, size), 3)
+        take = (npart[idx] >= k) & torch.isinf(ref)
+        ref = torch.where(take, d2[idx], ref)
+        if p + 1 < th.num_planes():
+            idx = _leaf_parent(th.ispl_n2n.get(p + 1, size + 1), size).long()[idx]
+    got = _guaranteed_leaf_r2(th, k)
+    nl = int(th.num(0))
+    assert torch.equal(got[:nl], ref[:nl])
