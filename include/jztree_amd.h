@@ -274,16 +274,19 @@ int jzt_knn_fused(jzt_stream_t stream, const int32_t* parent_ilist_spl, const in
                   void* knn_rad, int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize,
                   int dim, int dtype);
 
-/* KnnFused retry pass (extension): recomputes, without a bound, exactly the queries flagged in
- * `retry` (size_part int32, input) and writes their rows; other rows are left untouched.  Same
- * arguments as jzt_knn_fused.  This is the exact fallback after any bounded pass (the fused one or
+/* KnnFused retry pass (extension): recomputes exactly the queries flagged in `retry` (size_part
+ * int32, input) and writes their rows; other rows are left untouched.  Same arguments as
+ * jzt_knn_fused, plus bound_r2 (per leaf, may be NULL): a radius^2 within which every flagged
+ * query of the leaf is known to have k neighbours (e.g. the guaranteed cell radius); the search
+ * starts there, and a chunk whose flagged queries come up short after all is redone unbounded.
+ * NULL = unbounded.  This is the exact fallback after any bounded pass (the fused one or
  * jzt_knn_leaf2leaf_ex with init_r2). */
 int jzt_knn_fused_retry(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
                         const float* parent_ilist_r2, int64_t parent_ilist_size, const int32_t* parent_child_spl,
                         const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* leaf_spl, const void* xz,
-                        const int32_t* src_ids, const int32_t* dst_rows, const int32_t* retry, void* knn_rad,
-                        int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize, int dim,
-                        int dtype);
+                        const int32_t* src_ids, const int32_t* dst_rows, const float* bound_r2, const int32_t* retry,
+                        void* knn_rad, int32_t* knn_id, int64_t size_leaves, int64_t size_part, int k, float boxsize,
+                        int dim, int dtype);
 
 /* ------------------------------------------------------------------------------------------ */
 /* learned radius proposer (no reference counterpart)                                          */

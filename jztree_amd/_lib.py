@@ -61,7 +61,7 @@ _SIGNATURES = {
                               _i64, _i32, _f32, _i32, _i32], _i32),
     "jzt_knn_fused": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32,
                        _i32, _i32], _i32),
-    "jzt_knn_fused_retry": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32,
+    "jzt_knn_fused_retry": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32,
                              _f32, _i32, _i32], _i32),
     "jzt_proposer_num_params": ([], _i32),
     "jzt_leaf_features": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p], _i32),

@@ -139,8 +139,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     int64_t pil_size, const int32_t* __restrict__ n2c, const int32_t* __restrict__ leaf_parent,
     const T* __restrict__ leaf_nodes, const int32_t* __restrict__ l2p, const T* __restrict__ x,
     const int32_t* __restrict__ src_ids, const int32_t* __restrict__ dst_rows, const float* __restrict__ init_r2,
-    int32_t* __restrict__ retry, int only_retry, T* __restrict__ rad_out, int32_t* __restrict__ id_out, int ktot,
-    T boxsize, int64_t nitems) {
+    int32_t* __restrict__ retry, int only_retry, const float* __restrict__ retry_r2, T* __restrict__ rad_out,
+    int32_t* __restrict__ id_out, int ktot, T boxsize, int64_t nitems) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_list[kListCap];
   __shared__ uint64_t s_buf[kBufWords];
@@ -153,8 +153,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     const int par = leaf_parent[lq];
     const int e0 = pil_spl[par], e1 = (int)min((int64_t)pil_spl[par + 1], pil_size);
     const T half = (T)(float)(T(0.5) * boxsize);
-    const float r2_init = (init_r2 && !only_retry) ? init_r2[lq] : INFINITY;
-    const uint64_t empty = ((uint64_t)__float_as_uint(r2_init) << 32) | 0xffffffffull;
+    // the bounded pass starts at init_r2; the retry pass at retry_r2 (a bound every flagged query
+    // is known to have k neighbours within), else unbounded
+    const float r2_init = only_retry ? (retry_r2 ? retry_r2[lq] : INFINITY) : (init_r2 ? init_r2[lq] : INFINITY);
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 
     for (int qoff = q0; qoff < q1; qoff += 64) {
@@ -162,6 +163,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       const bool valid = qoff + lane < q1;
       if (only_retry && !wave_any(valid && retry[iq] != 0)) continue;
       const Vec<D, T> q = load_vec<D, T>(x, iq);
+      // a retry pass whose bound leaves some flagged query short (rounding at the very edge of
+      // the bound) redoes the chunk unbounded
+      for (float r2_try = r2_init;; r2_try = INFINITY) {
+      const uint64_t empty = ((uint64_t)__float_as_uint(r2_try) << 32) | 0xffffffffull;
       uint64_t best[K];
   #pragma unroll
       for (int i = 0; i < K; i++) best[i] = empty;
@@ -242,11 +247,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       }
 
       if (wave_any(bcnt > 0)) topk_flush<K>(best, row, bcnt);
-      if (valid && (!only_retry || retry[iq] != 0)) {
-        uint64_t kth_key = best[0];
+      const bool mine = valid && (!only_retry || retry[iq] != 0);
+      uint64_t kth_key = best[0];
   #pragma unroll
-        for (int i = 0; i < K; i++) kth_key = (i == ktot - 1) ? best[i] : kth_key;
-        const bool short_list = kth_key == empty && r2_init != INFINITY;
+      for (int i = 0; i < K; i++) kth_key = (i == ktot - 1) ? best[i] : kth_key;
+      const bool short_list = kth_key == empty && r2_try != INFINITY;
+      if (only_retry && wave_any(mine && short_list)) continue;
+      if (mine) {
         if (retry && !only_retry) retry[iq] = short_list ? 1 : 0;
         if (!short_list) {
           const int64_t row = (int64_t)(dst_rows ? dst_rows[iq] : iq) * ktot;
@@ -259,18 +266,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
           }
         }
       }
+      break;
+      }
     }
   
   }
 }
 
 // passes: bit 0 = bounded pass (flags short queries when init_r2 is given), bit 1 = retry pass
-// (recomputes the queries flagged in `retry` without a bound)
+// (recomputes the queries flagged in `retry`, from retry_r2 when given, else without a bound)
 template <int D, typename T>
 int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, const float* pil_r2, int64_t pil_size,
                const int32_t* n2c, const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* l2p,
                const void* x, const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2, int32_t* retry,
-               void* rad, int32_t* ids, int64_t nleaves, int64_t npart, int k, float boxsize, int passes) {
+               const float* retry_r2, void* rad, int32_t* ids, int64_t nleaves, int64_t npart, int k, float boxsize,
+               int passes) {
   if (nleaves <= 0) return 0;
   if (init_r2 && !retry) return fail("KnnFused: a bound needs the retry buffer");
   dim3 g(grid_for(nleaves)), b(64);
@@ -279,8 +289,8 @@ int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, co
     const int only = pass;
 #define JZT_FUSED(KK, PP)                                                                                        \
   hipLaunchKernelGGL((k_knn_fused<KK, PP, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, pil_size, n2c, leaf_parent, \
-                     (const T*)leaf_nodes, l2p, (const T*)x, src_ids, dst_rows, init_r2, retry, only, (T*)rad, ids, k,    \
-                     (T)boxsize, nleaves)
+                     (const T*)leaf_nodes, l2p, (const T*)x, src_ids, dst_rows, init_r2, retry, only, retry_r2, (T*)rad,  \
+                     ids, k, (T)boxsize, nleaves)
 #define JZT_FUSED_K(PP)              \
   do {                               \
     if (k <= 8) JZT_FUSED(8, PP);    \
@@ -318,7 +328,8 @@ extern "C" int jzt_knn_fused(jzt_stream_t stream, const int32_t* parent_ilist_sp
   if (k < 1 || k > 64) return fail("KnnFused: k must be in [1, 64]");
   return fused_dispatch(dim, dtype, (hipStream_t)stream, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2,
                         parent_ilist_size, parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows,
-                        init_r2, retry, knn_rad, knn_id, size_leaves, size_part, k, boxsize, init_r2 ? 3 : 1);
+                        init_r2, retry, (const float*)nullptr, knn_rad, knn_id, size_leaves, size_part, k, boxsize,
+                        init_r2 ? 3 : 1);
 }
 
 extern "C" int jzt_knn_fused_retry(jzt_stream_t stream, const int32_t* parent_ilist_spl,
@@ -326,12 +337,12 @@ extern "C" int jzt_knn_fused_retry(jzt_stream_t stream, const int32_t* parent_il
                                    int64_t parent_ilist_size, const int32_t* parent_child_spl,
                                    const int32_t* leaf_parent, const void* leaf_nodes, const int32_t* leaf_spl,
                                    const void* xz, const int32_t* src_ids, const int32_t* dst_rows,
-                                   const int32_t* retry, void* knn_rad, int32_t* knn_id, int64_t size_leaves,
-                                   int64_t size_part, int k, float boxsize, int dim, int dtype) {
+                                   const float* bound_r2, const int32_t* retry, void* knn_rad, int32_t* knn_id,
+                                   int64_t size_leaves, int64_t size_part, int k, float boxsize, int dim, int dtype) {
   if (k < 1 || k > 64) return fail("KnnFused: k must be in [1, 64]");
   if (!retry) return fail("KnnFused: retry flags required");
   return fused_dispatch(dim, dtype, (hipStream_t)stream, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2,
                         parent_ilist_size, parent_child_spl, leaf_parent, leaf_nodes, leaf_spl, xz, src_ids, dst_rows,
-                        (const float*)nullptr, const_cast<int32_t*>(retry), knn_rad, knn_id, size_leaves, size_part,
+                        (const float*)nullptr, const_cast<int32_t*>(retry), bound_r2, knn_rad, knn_id, size_leaves, size_part,
                         k, boxsize, 2);
 }

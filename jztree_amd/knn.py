@@ -412,10 +412,12 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
+    # the flagged queries start from the guaranteed leaf radius (every query has its k nearest
+    # within it), not from an unbounded search
     L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
            ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
-           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, n,
-           int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
+           L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(cap.contiguous()), L.ptr(retry), L.ptr(rnn),
+           L.ptr(inn), size, n, int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
     return rnn, inn, first
 
 
