@@ -140,28 +140,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
     const T* __restrict__ leaf_nodes, const int32_t* __restrict__ l2p, const T* __restrict__ x,
     const int32_t* __restrict__ src_ids, const int32_t* __restrict__ dst_rows, const float* __restrict__ init_r2,
     int32_t* __restrict__ retry, int only_retry, const float* __restrict__ retry_r2, T* __restrict__ rad_out,
-    int32_t* __restrict__ id_out, int ktot, T boxsize, int64_t nitems) {
+    int32_t* __restrict__ id_out, int ktot, T boxsize, int64_t nitems, int64_t npart) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_list[kListCap];
   __shared__ uint64_t s_buf[kBufWords];
   uint64_t* const row = s_buf + threadIdx.x * kBufStride;
   const int lane = threadIdx.x;
   const int64_t nact_ = active_items(l2p, nitems);
-  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
-    const int q0 = l2p[lq], q1 = l2p[lq + 1];
-    if (q1 <= q0) continue;
+  const T half = (T)(float)(T(0.5) * boxsize);
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  // the query chunk [qoff, min(qoff + 64, q1)) of query leaf lq
+  auto run_chunk = [&](int lq, int qoff) {
+    const int q1 = l2p[lq + 1];
     const int par = leaf_parent[lq];
     const int e0 = pil_spl[par], e1 = (int)min((int64_t)pil_spl[par + 1], pil_size);
-    const T half = (T)(float)(T(0.5) * boxsize);
     // the bounded pass starts at init_r2; the retry pass at retry_r2 (a bound every flagged query
     // is known to have k neighbours within), else unbounded
     const float r2_init = only_retry ? (retry_r2 ? retry_r2[lq] : INFINITY) : (init_r2 ? init_r2[lq] : INFINITY);
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-
-    for (int qoff = q0; qoff < q1; qoff += 64) {
+    {
       const int iq = min(qoff + lane, q1 - 1);
       const bool valid = qoff + lane < q1;
-      if (only_retry && !wave_any(valid && retry[iq] != 0)) continue;
+      if (only_retry && !wave_any(valid && retry[iq] != 0)) return;
       const Vec<D, T> q = load_vec<D, T>(x, iq);
       // a retry pass whose bound leaves some flagged query short (rounding at the very edge of
       // the bound) redoes the chunk unbounded
@@ -269,7 +268,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       break;
       }
     }
-  
+  };
+  if (!only_retry) {
+    for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+      const int q0 = l2p[lq], q1 = l2p[lq + 1];
+      for (int qoff = q0; qoff < q1; qoff += 64) run_chunk(lq, qoff);
+    }
+    return;
+  }
+  // Retry pass: the flags are scanned 64 queries at a time (coalesced), and only the leaf chunks
+  // holding a flagged query run.  A chunk that straddles two scanned blocks may run twice; it then
+  // writes the same rows twice.
+  for (int64_t qb = (int64_t)blockIdx.x * 64; qb < npart; qb += (int64_t)gridDim.x * 64) {
+    uint64_t m = __ballot(qb + lane < npart && retry[qb + lane] != 0);
+    while (m) {
+      const int64_t qf = qb + __builtin_ctzll(m);
+      int64_t lo = 0, hi = nact_;  // the leaf of qf: the last lq < nact_ with l2p[lq] <= qf
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (l2p[mid] <= qf) lo = mid;
+        else hi = mid;
+      }
+      const int lq = (int)lo, q0 = l2p[lq], q1 = l2p[lq + 1];
+      const int qoff = q0 + (int)((qf - q0) / 64) * 64;
+      run_chunk(lq, qoff);
+      // the chunk's queries: [qoff, min(qoff + 64, q1)) -- the leaf may end inside the block
+      const int64_t b0 = max((int64_t)qoff - qb, (int64_t)0);
+      const int64_t b1 = min((int64_t)min(qoff + 64, q1) - qb, (int64_t)64);
+      const uint64_t done = (b1 >= 64 ? ~0ull : ((1ull << b1) - 1ull)) & ~((1ull << b0) - 1ull);
+      m &= ~done;
+    }
   }
 }
 
@@ -287,10 +315,11 @@ int fused_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil_src, co
   for (int pass = 0; pass < 2; pass++) {
     if (!(passes & (1 << pass))) continue;
     const int only = pass;
+    const dim3 gp = pass == 0 ? g : dim3(grid_for(div_up(npart, 64)));  // the retry pass scans the queries
 #define JZT_FUSED(KK, PP)                                                                                        \
-  hipLaunchKernelGGL((k_knn_fused<KK, PP, D, T>), g, b, 0, s, pil_spl, pil_src, pil_r2, pil_size, n2c, leaf_parent, \
+  hipLaunchKernelGGL((k_knn_fused<KK, PP, D, T>), gp, b, 0, s, pil_spl, pil_src, pil_r2, pil_size, n2c, leaf_parent, \
                      (const T*)leaf_nodes, l2p, (const T*)x, src_ids, dst_rows, init_r2, retry, only, retry_r2, (T*)rad,  \
-                     ids, k, (T)boxsize, nleaves)
+                     ids, k, (T)boxsize, nleaves, npart)
 #define JZT_FUSED_K(PP)              \
   do {                               \
     if (k <= 8) JZT_FUSED(8, PP);    \
