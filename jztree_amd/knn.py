@@ -20,7 +20,7 @@ from .checks import raise_if
 from .config import KNNConfig
 from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
 from .tools import inverse_of_splits, masked_inverse
-from .tree import _tmp, top_dense_interaction_list, zsort_and_tree
+from .tree import _tmp, top_dense_interaction_list, zsort_and_tree, zsort_and_tree_origin
 
 
 @dataclass
@@ -463,14 +463,13 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
     dev = pos.device
     size = len(pos)
     try:
-        origin = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
         num_origin = get_num(part, default_to_length=True)
 
         if th is not None:
             partz = part
-            originz = origin
+            originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
         else:
-            partz, originz, th = zsort_and_tree(part, cfg.tree, data=origin)
+            partz, originz, th = zsort_and_tree_origin(part, cfg.tree)
         if output_order == "z":
             originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
             num_origin = len(get_pos(partz))

@@ -211,12 +211,11 @@ class RadiusProposer:
 def training_data(points: list, k: int, cfg: KNNConfig = KNNConfig(), boxsize: float | None = None):
     """(features, log2 radius targets) of the leaves of each point set, from the plain kNN."""
     from .knn import _knn_dual_walk, _knn_leaf2leaf
-    from .tree import zsort_and_tree
+    from .tree import zsort_and_tree_origin
     fs, ts = [], []
     for x in points:
         dev = x.device
-        ridx = RankIdx(rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev))
-        partz, originz, th = zsort_and_tree(x, cfg.tree, data=ridx)
+        partz, originz, th = zsort_and_tree_origin(x, cfg.tree)
         il = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
         rz, _ = _knn_leaf2leaf(il, th.splits_leaf_to_part(), get_pos(partz), k=k, boxsize=boxsize or 0.,
                                src_ids=originz.idx)
@@ -233,15 +232,14 @@ def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = 
     knn(part, k, boxsize, output_order=...).  With return_retry, also the per-query flags of the
     queries that the first (proposed-radius) round could not finish (z order)."""
     from .knn import _knn_bounded
-    from .tree import zsort_and_tree
+    from .tree import zsort_and_tree_origin
     assert output_order in ("z", "input")
     assert k == proposer.k, "the proposer was trained for another k"
     pos = get_pos(part)
     dev = pos.device
     size = len(pos)
     try:
-        origin = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
-        partz, originz, th = zsort_and_tree(part, cfg.tree, data=origin)
+        partz, originz, th = zsort_and_tree_origin(part, cfg.tree)
         bound = proposer.bound_r2(th)
         if output_order == "input":  # rows and reported ids in input order, as knn()
             sid, dst = originz.idx, originz.idx

@@ -16,7 +16,7 @@ import torch
 from . import _lib as L
 from .checks import raise_if
 from .config import TreeConfig, RegularizationConfig
-from .data import (InteractionList, LevelInfo, PackedArray, Pos, TreeHierarchy, get_num, get_num_total, get_pos,
+from .data import (InteractionList, RankIdx, LevelInfo, PackedArray, Pos, TreeHierarchy, get_num, get_num_total, get_pos,
                    same_width_int, tree_map_by_len, verify_ilist)
 from .tools import cumsum_starting_with_zero, div_ceil
 
@@ -435,6 +435,15 @@ def zsort_and_tree(part, cfg_tree: TreeConfig = TreeConfig(), data: Any | None =
 
 
 zsort_and_tree.jit = zsort_and_tree
+
+
+def zsort_and_tree_origin(part, cfg_tree: TreeConfig = TreeConfig()):
+    """zsort_and_tree(part, cfg_tree, data=RankIdx(rank=None, idx=arange(n) int32)), without the
+    gather of the arange: its z-ordered copy is the sort permutation itself.
+    Returns (partz, originz, th)."""
+    partz, isort = zsort(part)
+    th = build_tree_hierarchy(partz, cfg_tree)
+    return partz, RankIdx(rank=None, idx=isort.to(torch.int32)), th
 
 
 # ------------------------------------------------------------------------------------------------
