@@ -165,7 +165,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       // a retry pass whose bound leaves some flagged query short (rounding at the very edge of
       // the bound) redoes the chunk unbounded
       for (float r2_try = r2_init;; r2_try = INFINITY) {
-      const uint64_t empty = ((uint64_t)__float_as_uint(r2_try) << 32) | 0xffffffffull;
+      // In the retry pass only the flagged queries search: the others start full at key 0 (they
+      // accept nothing and do not widen the wave's search radius) and write nothing.
+      const bool searching = !only_retry || (valid && retry[iq] != 0);
+      const uint64_t empty = searching ? (((uint64_t)__float_as_uint(r2_try) << 32) | 0xffffffffull) : 0ull;
       uint64_t best[K];
   #pragma unroll
       for (int i = 0; i < K; i++) best[i] = empty;
