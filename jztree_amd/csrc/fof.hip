@@ -48,10 +48,13 @@ __device__ __forceinline__ void uf_find2(const int32_t* p, int& a, int& b) {
   }
 }
 
-// Joins the sets of a and b (any members of them); returns the root the two were joined under (a
-// root at the time of the join: later links elsewhere may hook it further).
+// Joins the sets of a and b (any members of them); returns a member of the joined set (the
+// smaller index at the successful hook).  Parents always have smaller indices than their children
+// (min-hooking), so the larger of a and b is hooked under the smaller at once, without finding
+// roots first: if it was a root, that is the whole link (one atomic); if not, its old parent's set
+// still has to be joined with the smaller one's, which the loop does from their roots.  Every set
+// that was connected stays connected, so the components are those of the finds-first version.
 __device__ __forceinline__ int uf_link(int32_t* p, int a, int b) {
-  uf_find2(p, a, b);
   while (a != b) {
     int hi = a > b ? a : b, lo = a > b ? b : a;
     int old = atomicMin(p + hi, lo);
