@@ -355,40 +355,61 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
           s_r[lane] = r;
           __syncthreads();
           if (valid) {
-            while (m) {
-              const int j = __builtin_ctzll(m);
-              m &= m - 1ull;
-              const int gj = s_r[j].g;
-              // labels are snapshots of set members; a label equal to q's own, or to one q was
-              // already joined with, belongs to q's set by now
-              bool want = false;
-              if (!(gj == gq || gj == glast || gj == glast2)) {
+            // Sources go two at a time, their distances and tests without branches (a label test
+            // that would skip a source only saves arithmetic): the loop's scalar control and exec
+            // mask work is paid once per pair.  The periodic and plain variants are separate loops.
+            auto sources = [&](auto wrap_c) {
+              constexpr bool W = decltype(wrap_c)::value;
+              auto test = [&](int j, int& gj) {
+                const FofRec<D, T> rj = s_r[j];
+                gj = rj.g;
                 Vec<D, T> xp;
   #pragma unroll
-                for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
-                const T r2 = wrap ? dist2P<true, D, T>(xq, xp, boxsize, half) : dist2P<false, D, T>(xq, xp, boxsize, half);
-                want = r2 <= (T)r2link;
-              }
+                for (int d = 0; d < D; d++) xp[d] = rj.p[d];
+                const T r2 = dist2P<W, D, T>(xq, xp, boxsize, half);
+                // labels are snapshots of set members; a label equal to q's own, or to one q was
+                // already joined with, belongs to q's set by now
+                return (gj != gq) & (gj != glast) & (gj != glast2) & (r2 <= (T)r2link);
+              };
               // Lanes whose q holds the same label are in one set: a single link of that set with
               // p's joins them all, so one lane links and the others take its root.
-              uint64_t wm = __ballot(want);
-              while (wm) {
-                const int L = __builtin_ctzll(wm);
-                const bool mine = want && gq == __builtin_amdgcn_readlane(gq, L);
-                const uint64_t mm = __ballot(mine);
-                int root = 0;
-                // from q's label and p's (members of the two sets): one parent step shorter each
-                if (lane == L) root = uf_link(ig, gq, gj);
-                root = __builtin_amdgcn_readlane(root, L);
-                if (mine) {
-                  glast2 = glast;
-                  glast = gj;
-                  gq = root;
-                  want = false;
+              auto link = [&](bool want, int gj) {
+                uint64_t wm = __ballot(want);
+                while (wm) {
+                  const int L = __builtin_ctzll(wm);
+                  const bool mine = want && gq == __builtin_amdgcn_readlane(gq, L);
+                  const uint64_t mm = __ballot(mine);
+                  int root = 0;
+                  if (lane == L) root = uf_link(ig, gq, gj);
+                  root = __builtin_amdgcn_readlane(root, L);
+                  if (mine) {
+                    glast2 = glast;
+                    glast = gj;
+                    gq = root;
+                    want = false;
+                  }
+                  wm &= ~mm;
                 }
-                wm &= ~mm;
+              };
+              while (m) {
+                const int j1 = __builtin_ctzll(m);
+                m &= m - 1ull;
+                if (m) {
+                  const int j2 = __builtin_ctzll(m);
+                  m &= m - 1ull;
+                  int g1, g2;
+                  const bool w1 = test(j1, g1), w2 = test(j2, g2);
+                  link(w1, g1);
+                  link(w2 && (g2 != gq) && (g2 != glast) && (g2 != glast2), g2);
+                } else {
+                  int g1;
+                  const bool w1 = test(j1, g1);
+                  link(w1, g1);
+                }
               }
-            }
+            };
+            if (wrap) sources(std::integral_constant<bool, true>());
+            else sources(std::integral_constant<bool, false>());
           }
           __syncthreads();
         }
