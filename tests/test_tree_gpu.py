@@ -127,3 +127,18 @@ def test_packed_get_kernel_matches_torch_ops(dev, dtype, row, ispl_dtype):
                 b = gpu.get(level, size, fv)
                 assert b.dtype == a.dtype and b.shape == a.shape
                 assert torch.equal(b.cpu(), a), (level, size, fv)
+
+
+def test_zsort_and_tree_origin_equals_gathered_arange(dev):
+    """The z-ordered origin indices taken from the sort permutation equal the gathered arange of
+    zsort_and_tree(data=...), and the trees are the same."""
+    import dataclasses
+    import jztree_amd as jz
+    x = torch.from_numpy(clustered(1 << 15, nhalos=32, seed=4)).to(dev)
+    cfg = dataclasses.replace(jz.config.KNNConfig().tree, alloc_fac_nodes=2.0)
+    ridx = jz.data.RankIdx(rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev))
+    pz1, o1, th1 = jz.tree.zsort_and_tree(x, cfg, data=ridx)
+    pz2, o2, th2 = jz.tree.zsort_and_tree_origin(x, cfg)
+    assert o2.idx.dtype == o1.idx.dtype and torch.equal(o1.idx, o2.idx)
+    assert torch.equal(pz1, pz2)
+    assert torch.equal(th1.ispl_n2n.data, th2.ispl_n2n.data) and torch.equal(th1.lvl.data, th2.lvl.data)
