@@ -208,6 +208,12 @@ int jzt_knn_group(jzt_stream_t stream, const int32_t* parent_ilist_spl, const in
                   const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* knn_rad, int32_t* knn_id,
                   int64_t ngroups, int64_t out_rows, int k, float boxsize, int dim, int dtype);
 
+/* Regularization's extra leaf splits (tree.py:236-243 of the reference, one kernel here):
+ * flags[i] = flags[i] | (lvl[i] >= *lvl_max && i <= *npart) as 0/1 bytes, i < n; lvl_max and npart
+ * are device scalars. */
+int jzt_flags_or_level(jzt_stream_t stream, int8_t* flags, const int32_t* lvl, int64_t n, const int32_t* lvl_max,
+                       const int32_t* npart);
+
 /* One plane of a per-plane packed array (extension; the data.py:251-345 PackedArray.get of the
  * reference, which jax evaluates as an index + select): out row i = data row ispl[level] + i
  * while that is below ispl[level + 1], else every element of the row = *fill.  Rows are

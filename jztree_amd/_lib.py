@@ -50,6 +50,7 @@ _SIGNATURES = {
     "jzt_knn_node2node_ex": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i32,
                               _f32, _i32, _i32], _i32),
     "jzt_segment_max_f32": ([_p, _p, _p, _i64, _p, _i64, _f32], _i32),
+    "jzt_flags_or_level": ([_p, _p, _p, _i64, _p, _p], _i32),
     "jzt_plane_cap": ([_p, _p, _p, _i64, _p, _p, _p, _i64, _i32, _i32, _p], _i32),
     "jzt_packed_get": ([_p, _p, _i64, _p, _i32, _i32, _p, _i32, _i32, _p, _i64], _i32),
     "jzt_knn_group": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32, _i32,

@@ -259,6 +259,15 @@ __global__ void k_packed_get(const uint32_t* __restrict__ data, int64_t nrows, c
   out[t] = (r < end && r >= 0 && r < nrows) ? data[r * row_words + w] : fill[w % elem_words];
 }
 
+// flags[i] |= (lvl[i] >= *lvl_max && i <= *npart), as 0/1 bytes (regularization's extra splits)
+__global__ void k_flags_or_level(int8_t* __restrict__ flags, const int32_t* __restrict__ lvl, int64_t n,
+                                 const int32_t* __restrict__ lvl_max, const int32_t* __restrict__ npart) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool extra = lvl[i] >= lvl_max[0] && i < (int64_t)npart[0] + 1;
+  flags[i] = (flags[i] != 0 || extra) ? 1 : 0;
+}
+
 static int g_tune[kTuneCount] = {4, 12};  // leaf kNN waves per EU (K <= 32), onesweep keys per thread
 int tune_get(int knob) { return (knob >= 0 && knob < kTuneCount) ? __atomic_load_n(&g_tune[knob], __ATOMIC_RELAXED) : 0; }
 
@@ -341,4 +350,14 @@ extern "C" int jzt_packed_get(jzt_stream_t stream, const void* data, int64_t nro
                      (const uint32_t*)data, nrows, ispl, ispl64, level, (const uint32_t*)fill, ew, rw, (uint32_t*)out,
                      size);
   return check_launch("PackedGet");
+}
+
+extern "C" int jzt_flags_or_level(jzt_stream_t stream, int8_t* flags, const int32_t* lvl, int64_t n,
+                                  const int32_t* lvl_max, const int32_t* npart) {
+  using namespace jzt;
+  if (n <= 0) return 0;
+  if (!flags || !lvl || !lvl_max || !npart) return fail("FlagsOrLevel: null argument");
+  hipLaunchKernelGGL(k_flags_or_level, dim3((unsigned)div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, flags, lvl,
+                     n, lvl_max, npart);
+  return check_launch("FlagsOrLevel");
 }

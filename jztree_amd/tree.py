@@ -190,8 +190,8 @@ def detect_leaf_boundaries(posz: torch.Tensor, ptype: torch.Tensor | None = None
         lvl_max = find_regularization_level(leaf_lvl, linfo, weights=leaf_npart,
                                             max_vol_fac=cfg_reg.max_volume_fac,
                                             min_percentile=cfg_reg.regularize_percentile)
-        ar = torch.arange(n + 1, device=dev)
-        flags = (flags.bool() | ((lvl >= lvl_max) & (ar < npart_t + 1))).to(torch.int8)
+        L.call("jzt_flags_or_level", L.stream(), L.ptr(flags), L.ptr(lvl), n + 1,
+               L.ptr(lvl_max.to(torch.int32).reshape(1).contiguous()), L.ptr(npart_t.reshape(1).contiguous()))
 
     splits, nfilled = _where_flags(flags, alloc_size, npart_t)
     raise_if(nfilled > alloc_size,
