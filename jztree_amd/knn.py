@@ -372,7 +372,8 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     # with direct_retry the planes above the leaves search the guaranteed radii: their plane-1 lists
     # then serve the fused retry pass directly (no second walk)
     ovf1 = run(r2, cap if direct_retry else None)
-    first = retry.clone()
+    # the fused retry pass only reads the flags; later rounds (direct_retry off) rewrite them
+    first = retry if direct_retry else retry.clone()
     # one read-back per round: (lists overflowed, queries left)
     h = torch.stack([ovf1.to(torch.int64), first.sum().to(torch.int64)]).cpu()
     st["short_round1"] = int(h[1])
