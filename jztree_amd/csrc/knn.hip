@@ -672,14 +672,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
       if (qoff + lane < q1) {
         const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
         if (orow >= 0 && orow < out_rows) write_row<K>(best, rad_out, id_out, orow * ktot + koff, ktot - koff);
-        if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
-        if (resume) {
-          const int last = min(K, ktot - koff) - 1;
-          uint64_t lk = best[0];
+        // the last slot this pass reports (k need not be a power of two: slots past it may
+        // keep the start key even when the query has all its neighbours)
+        const int last = min(K, ktot - koff) - 1;
+        uint64_t lk = best[0];
   #pragma unroll
-          for (int i = 0; i < K; i++) lk = (i == last) ? best[i] : lk;
-          resume[iq] = lk;
-        }
+        for (int i = 0; i < K; i++) lk = (i == last) ? best[i] : lk;
+        if (retry) retry[iq] = (lk == start_key && init_r2) ? 1 : 0;
+        if (resume) resume[iq] = lk;
       }
     }
   

@@ -162,8 +162,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 4 
       const bool valid = qoff + lane < q1;
       if (only_retry && !wave_any(valid && retry[iq] != 0)) return;
       const Vec<D, T> q = load_vec<D, T>(x, iq);
-      // a retry pass whose bound leaves some flagged query short (rounding at the very edge of
-      // the bound) redoes the chunk unbounded
+      // A retry pass whose bound leaves some flagged query short redoes the chunk with no bound.
+      // That second try is exact only if the parent lists hold every leaf it can need: true for
+      // lists from the guaranteed-radius walk, and for lists built for retry_r2 itself as long as
+      // retry_r2 is a true k-neighbour bound, so that the first try never comes up short.  The
+      // caller's retry_r2 (the squared diagonal of the smallest enclosing cell with >= k
+      // particles, widened by 2^-10) is such a bound: all k lie in that cell with the query.
       for (float r2_try = r2_init;; r2_try = INFINITY) {
       // In the retry pass only the flagged queries search: the others start full at key 0 (they
       // accept nothing and do not widen the wave's search radius) and write nothing.

@@ -143,7 +143,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
       if (qoff + lane < p1) {
         const int64_t orow = dst_rows ? (int64_t)dst_rows[iq] : (int64_t)iq;
         if (orow >= 0 && orow < out_rows) write_row<K>(best, rad_out, id_out, orow * ktot, ktot);
-        if (retry) retry[iq] = (best[K - 1] == start_key && init_r2) ? 1 : 0;
+        if (retry) {  // the k-th slot, not the template's last (k need not be a power of two)
+          uint64_t lk = best[0];
+  #pragma unroll
+          for (int i = 0; i < K; i++) lk = (i == ktot - 1) ? best[i] : lk;
+          retry[iq] = (lk == start_key && init_r2) ? 1 : 0;
+        }
       }
     }
   }

@@ -355,7 +355,10 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     # never search beyond the guaranteed radius (a wild proposal would only inflate the lists); the
     # 2^-10 widening keeps rounding of the canonical r2 from causing avoidable retries
     cap = (_guaranteed_leaf_r2(th, k) * (1.0 + 2.0 ** -10)).to(torch.float32)
-    r2 = torch.minimum(leaf_r2.to(torch.float32), cap).contiguous()
+    # a NaN or negative proposal (a diverged network) would make the leaf kernel skip the leaf
+    # (that is how later rounds leave leaves untouched): such leaves search the guaranteed radius
+    lr2 = leaf_r2.to(torch.float32)
+    r2 = torch.where(torch.isnan(lr2) | (lr2 < 0), cap, torch.minimum(lr2, cap)).contiguous()
 
     plane1 = []
 

@@ -735,6 +735,182 @@ void orc_fof_f32(const float* x, int64_t n, int dim, float r2link, float boxsize
     free(mn); free(par);
 }
 
+/*
+ * FoF on fine cells, the same groups as orc_fof_f32 (predicate canonical r2 <= r2link, labels =
+ * min key per group) at a cost that does not blow up in dense clumps.
+ *
+ * Cells have side h <= sqrt(r2link/dim) * (1 - 2^-12).  Two points of one cell are then at true
+ * squared distance <= r2link * (1 - 2^-11), and the canonical FP32 r2 of the pair (at most a few
+ * roundings of relative size 2^-24 above the true value) stays below float32 r2link: every cell
+ * is one group from the start.  Two cells within reach (index distance <= ceil(rlink/h) in every
+ * dim) belong to one group as soon as one cross pair satisfies the predicate, so a cell pair is
+ * scanned only until its first link.  The scan of cell pairs runs in parallel and records links;
+ * a sequential union-find over cells joins them.  Points must lie in [0, boxsize) when periodic;
+ * otherwise (or when the cells would be too many to index) the plain cell list is used.
+ */
+typedef struct { int64_t key; int64_t val; } kv64_t;
+
+static int cmp_kv64(const void* a, const void* b) {
+    const kv64_t* x = (const kv64_t*)a; const kv64_t* y = (const kv64_t*)b;
+    return (x->key > y->key) - (x->key < y->key);
+}
+
+static inline uint64_t mix64(uint64_t z) {  /* splitmix64 finaliser (hash of a cell id) */
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+void orc_fof_cells_f32(const float* x, int64_t n, int dim, float r2link, float boxsize,
+                       const int64_t* key, int64_t* label_out) {
+    const int periodic = boxsize > 0.f;
+    double rl = sqrt((double)r2link);
+    double hmax = rl / sqrt((double)dim) * (1.0 - 1.0 / 4096.0);
+    double lo[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    int64_t nc[3] = {1, 1, 1};
+    int ok = n > 0 && hmax > 0.0 && dim <= 3;
+    for (int d = 0; d < dim && ok; d++) {
+        double a = INFINITY, b = -INFINITY;
+        for (int64_t i = 0; i < n; i++) { double v = x[i * dim + d]; a = v < a ? v : a; b = v > b ? v : b; }
+        if (!(a >= -1e300 && b <= 1e300)) { ok = 0; break; }         /* NaN / inf */
+        if (periodic) {
+            if (a < 0.0 || b >= (double)boxsize) { ok = 0; break; }
+            nc[d] = (int64_t)ceil((double)boxsize / hmax);
+            if (nc[d] < 1) nc[d] = 1;
+            lo[d] = 0.0; h[d] = (double)boxsize / (double)nc[d];
+        } else {
+            nc[d] = (int64_t)floor((b - a) / hmax) + 1;
+            lo[d] = a; h[d] = hmax;
+        }
+        if (nc[d] > (1 << 20)) ok = 0;
+    }
+    if (!ok) { orc_fof_f32(x, n, dim, r2link, boxsize, key, label_out); return; }
+    int R = 1;
+    for (int d = 0; d < dim; d++) { int r = (int)ceil(rl / h[d]); R = r > R ? r : R; }
+
+    /* points sorted by cell */
+    kv64_t* pc = (kv64_t*)malloc(sizeof(kv64_t) * (size_t)n);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t lin = 0;
+        for (int d = dim - 1; d >= 0; d--) {
+            int64_t c = (int64_t)floor(((double)x[i * dim + d] - lo[d]) / h[d]);
+            c = c < 0 ? 0 : (c >= nc[d] ? nc[d] - 1 : c);
+            lin = lin * nc[d] + c;
+        }
+        pc[i].key = lin; pc[i].val = i;
+    }
+    qsort(pc, (size_t)n, sizeof(kv64_t), cmp_kv64);
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++) if (i == 0 || pc[i].key != pc[i - 1].key) m++;
+    int64_t* cstart = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m + 1));
+    int64_t* cid = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    m = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (i == 0 || pc[i].key != pc[i - 1].key) { cid[m] = pc[i].key; cstart[m++] = i; }
+    cstart[m] = n;
+    /* open-addressing hash: cell id -> cell index */
+    int64_t hs = 1;
+    while (hs < 2 * m) hs <<= 1;
+    int64_t* ht = (int64_t*)malloc(sizeof(int64_t) * (size_t)hs);
+    for (int64_t i = 0; i < hs; i++) ht[i] = -1;
+    for (int64_t c = 0; c < m; c++) {
+        uint64_t s = mix64((uint64_t)cid[c]) & (uint64_t)(hs - 1);
+        while (ht[s] >= 0) s = (s + 1) & (uint64_t)(hs - 1);
+        ht[s] = c;
+    }
+    /* half stencil: offsets lexicographically after 0 */
+    int noff = 0, offs[343][3];
+    int span = 2 * R + 1, tot = 1;
+    for (int d = 0; d < dim; d++) tot *= span;
+    for (int o = 0; o < tot && noff < 343; o++) {
+        int v[3] = {0, 0, 0}, oo = o, first = 0;
+        for (int d = 0; d < dim; d++) { v[d] = oo % span - R; oo /= span; }
+        for (int d = dim - 1; d >= 0; d--) if (v[d] != 0) { first = v[d]; break; }
+        if (first > 0) { offs[noff][0] = v[0]; offs[noff][1] = v[1]; offs[noff][2] = v[2]; noff++; }
+    }
+    if (R > 3) { orc_fof_f32(x, n, dim, r2link, boxsize, key, label_out);
+                 free(pc); free(cstart); free(cid); free(ht); return; }
+
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    int64_t** lk = (int64_t**)calloc((size_t)nth, sizeof(int64_t*));
+    int64_t* lkn = (int64_t*)calloc((size_t)nth, sizeof(int64_t));
+    int64_t* lkc = (int64_t*)calloc((size_t)nth, sizeof(int64_t));
+    #pragma omp parallel
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t a = 0; a < m; a++) {
+            int64_t ca[3] = {0, 0, 0}, rem = cid[a];
+            for (int d = 0; d < dim; d++) { ca[d] = rem % nc[d]; rem /= nc[d]; }
+            int64_t seen[343];
+            int nseen = 0;
+            for (int o = 0; o < noff; o++) {
+                int64_t lin = 0;
+                int skip = 0;
+                for (int d = dim - 1; d >= 0; d--) {
+                    int64_t c = ca[d] + offs[o][d];
+                    if (periodic) c = ((c % nc[d]) + nc[d]) % nc[d];
+                    else if (c < 0 || c >= nc[d]) { skip = 1; break; }
+                    lin = lin * nc[d] + c;
+                }
+                if (skip || lin == cid[a]) continue;
+                int dup = 0;
+                for (int s = 0; s < nseen; s++) dup |= seen[s] == lin;
+                if (dup) continue;
+                seen[nseen++] = lin;
+                uint64_t s = mix64((uint64_t)lin) & (uint64_t)(hs - 1);
+                int64_t b = -1;
+                while (ht[s] >= 0) { if (cid[ht[s]] == lin) { b = ht[s]; break; } s = (s + 1) & (uint64_t)(hs - 1); }
+                if (b < 0) continue;
+                /* (with few cells per dim a wrapped pair can be met from both cells: a redundant
+                 * link, harmless to the union-find) */
+                int linked = 0;
+                for (int64_t i = cstart[a]; i < cstart[a + 1] && !linked; i++) {
+                    const float* xi = x + pc[i].val * dim;
+                    for (int64_t j = cstart[b]; j < cstart[b + 1]; j++) {
+                        if (r2_f32(xi, x + pc[j].val * dim, dim, boxsize) <= r2link) { linked = 1; break; }
+                    }
+                }
+                if (linked) {
+                    if (lkn[t] + 2 > lkc[t]) {
+                        lkc[t] = lkc[t] ? 2 * lkc[t] : 1024;
+                        lk[t] = (int64_t*)realloc(lk[t], sizeof(int64_t) * (size_t)lkc[t]);
+                    }
+                    lk[t][lkn[t]++] = a; lk[t][lkn[t]++] = b;
+                }
+            }
+        }
+    }
+    int64_t* par = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    for (int64_t c = 0; c < m; c++) par[c] = c;
+    for (int t = 0; t < nth; t++) {
+        for (int64_t i = 0; i < lkn[t]; i += 2) uf_union(par, lk[t][i], lk[t][i + 1]);
+        free(lk[t]);
+    }
+    free(lk); free(lkn); free(lkc);
+    int64_t* mn = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    for (int64_t c = 0; c < m; c++) mn[c] = INT64_MAX;
+    for (int64_t c = 0; c < m; c++) {
+        int64_t r = uf_find(par, c);
+        for (int64_t i = cstart[c]; i < cstart[c + 1]; i++) {
+            int64_t kk = key ? key[pc[i].val] : pc[i].val;
+            if (kk < mn[r]) mn[r] = kk;
+        }
+    }
+    for (int64_t c = 0; c < m; c++) {
+        int64_t l = mn[uf_find(par, c)];
+        for (int64_t i = cstart[c]; i < cstart[c + 1]; i++) label_out[pc[i].val] = l;
+    }
+    free(mn); free(par); free(pc); free(cstart); free(cid); free(ht);
+}
+
 /* brute force FoF (tiny n only): same output convention as orc_fof_f32 */
 void orc_fof_brute_f32(const float* x, int64_t n, int dim, float r2link, float boxsize,
                        const int64_t* key, int64_t* label_out) {

@@ -75,6 +75,7 @@ def lib():
         fof_args = [_f32p, _c.c_int64, _c.c_int, _c.c_float, _c.c_float, _c.c_void_p, _i64p]
         L.orc_fof_f32.argtypes = fof_args
         L.orc_fof_brute_f32.argtypes = fof_args
+        L.orc_fof_cells_f32.argtypes = fof_args
         L.orc_num_threads.restype = _c.c_int
         _LIB = L
     return _LIB
@@ -429,6 +430,8 @@ def knn(xs, k, xq=None, boxsize=0.0, ids=None, method="kdtree"):
 def fof(x, rlink, boxsize=0.0, key=None, method="grid"):
     """FoF labels (min key per group) for the predicate canonical r2 <= float32(rlink*rlink).
 
+    method: "grid" (cell list with cells >= rlink), "cells" (fine cells of side <= rlink/sqrt(dim),
+    each one group from the start: fast on dense clumps), "brute" (all pairs, tiny n).
     r2link is computed as in fof.py:50/:165: np.float32(rlink*rlink) with rlink*rlink in f64."""
     x = np.ascontiguousarray(x, np.float32)
     n, dim = x.shape
@@ -438,7 +441,7 @@ def fof(x, rlink, boxsize=0.0, key=None, method="grid"):
     if key is not None:
         key = np.ascontiguousarray(key, np.int64)
         kp = key.ctypes.data
-    fn = lib().orc_fof_brute_f32 if method == "brute" else lib().orc_fof_f32
+    fn = {"brute": lib().orc_fof_brute_f32, "grid": lib().orc_fof_f32, "cells": lib().orc_fof_cells_f32}[method]
     fn(x, n, dim, r2link, float(boxsize), kp, out)
     return out
 

@@ -183,10 +183,12 @@ def _leaf_max_r2k(jz, th, rz, k):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k,boxsize", [(16, 1.0), (32, 0.0), (32, 1.0)])
+@pytest.mark.parametrize("k,boxsize", [(16, 1.0), (32, 0.0), (32, 1.0), (12, 1.0), (40, 0.0)])
 def test_bounded_matches_plain(dev, oracle, k, boxsize):
     """Bounded search from a proposed radius per leaf (lists + leaf kernel + retry) equals the plain
-    search for bounds too small (retries), about right, and too large."""
+    search for bounds too small (retries), about right, too large, and for proposals with NaN and
+    negative entries (a diverged network).  With k not a power of two (12, 40) the kernel's top-k
+    is wider than k: a bound just above every leaf's exact radius must still flag no query."""
     import jztree_amd as jz
     x = clustered(1 << 16, nhalos=64, seed=21)
     t = torch.from_numpy(x).to(dev)
@@ -198,10 +200,15 @@ def test_bounded_matches_plain(dev, oracle, k, boxsize):
     exact = _leaf_max_r2k(jz, th, rz, k)
     mixed = exact.clone()
     mixed[::3] = 0.0
+    wild = exact.clone()
+    wild[::5] = float("nan")
+    wild[1::5] = -1.0
+    wild[2::5] = -float("inf")
     for fac, bound, grow, direct in [(0.25, exact * 0.25, None, True), (0.25, exact * 0.25, None, False),
                                      (0.25, exact * 0.25, 16.0, False), (1.0, exact * (1 + 1e-6), None, True),
                                      (3.0, exact * 3.0, None, True), ("mixed", mixed, None, True),
-                                     ("mixed", mixed, None, False)]:
+                                     ("mixed", mixed, None, False), ("nan_neg", wild, None, True),
+                                     ("nan_neg", wild, None, False)]:
         st = {}
         r1, i1, retry = jz.knn._knn_bounded(th, partz, k, bound, boxsize=boxsize, grow=grow, direct_retry=direct,
                                             alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=originz.idx, stats=st)
@@ -255,6 +262,28 @@ def _cell_diag2_ref(lvl, dim):
     for i in range(dim):
         d2 = d2 + np.ldexp(1.0, 2 * (o + (r >= dim - i)))
     return torch.from_numpy(d2).to(lvl.device)
+
+
+@pytest.mark.parametrize("k,boxsize", [(32, 1.0), (40, 0.0), (8, 1.0)])
+def test_guaranteed_leaf_r2_bounds_true_kth(dev, oracle, k, boxsize):
+    """The guaranteed leaf radius^2 (widened by 2^-10, as the bounded search and its retry pass use
+    it) is at least the true k-th neighbour r2 of every query in the leaf: the oracle's canonical
+    FP32 r2 of all 2^16 queries, clustered points (dense halos: small cells)."""
+    import jztree_amd as jz
+    from jztree_amd.knn import _guaranteed_leaf_r2
+    from jztree_amd.tools import inverse_of_splits
+    x = clustered(1 << 16, nhalos=64, seed=29)
+    t = torch.from_numpy(x).to(dev)
+    cfg = _clustered_cfg()
+    partz, th = jz.tree.zsort_and_tree(t, cfg.tree)
+    cap = (_guaranteed_leaf_r2(th, k) * (1.0 + 2.0 ** -10)).to(torch.float32)
+    nl = int(th.num(0))
+    leafof = inverse_of_splits(th.splits_leaf_to_part()[: nl + 1].contiguous(), len(x)).long()
+    r2o, _ = oracle.knn(partz.cpu().numpy(), k, boxsize=boxsize)
+    r2k = torch.from_numpy(r2o[:, k - 1]).to(dev)
+    bad = int((r2k > cap[leafof]).sum())
+    assert bad == 0, f"{bad} queries have their k-th neighbour beyond the guaranteed radius"
+    assert bool(torch.isfinite(cap[:nl]).all())
 
 
 @pytest.mark.parametrize("n,leaf,k", [(1 << 14, 32, 32), (1 << 17, 64, 32), (1 << 17, 16, 40)])

@@ -230,6 +230,39 @@ def test_fof_grid_equals_brute(oracle, periodic):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("kind", ["clustered", "uniform", "duplicates", "small_box"])
+def test_fof_cells_equals_brute(oracle, periodic, kind):
+    """The fine-cell FoF (every cell one group from the start, a cell pair scanned until its
+    first link) equals brute force on all pairs, in dense clumps, exact duplicates, sparse sets
+    and boxes with only a few cells per dim (wrapped stencils meet a cell twice)."""
+    box = 1.0 if periodic else 0.0
+    if kind == "clustered":
+        x, rl = clustered(3000, nhalos=12, seed=5, a=0.01), 0.25 * 3000 ** (-1 / 3)
+    elif kind == "uniform":
+        x, rl = uniform(3000, seed=6), 0.7 * 3000 ** (-1 / 3)
+    elif kind == "duplicates":
+        x = uniform(2000, seed=7)
+        x = np.concatenate([x, x[:500]])
+        rl = 0.5 * 2500 ** (-1 / 3)
+    else:
+        x, rl = uniform(1500, seed=8), 0.3
+    a = oracle.fof(x, rl, boxsize=box, method="cells")
+    b = oracle.fof(x, rl, boxsize=box, method="brute")
+    assert np.array_equal(a, b)
+    key = np.random.default_rng(1).permutation(len(x))
+    assert np.array_equal(oracle.fof(x, rl, boxsize=box, key=key, method="cells"),
+                          oracle.fof(x, rl, boxsize=box, key=key, method="brute"))
+
+
+def test_fof_cells_equals_grid_dense_halos(oracle):
+    """At the 2^24 configuration's density and linking length (rlink = 0.2 x mean spacing of
+    2^24 points, Plummer a = 0.002), in a 1/16-volume subsample: cells equal the plain cell list."""
+    x = clustered(1 << 18, nhalos=64, seed=9)
+    rl = 0.2 * (1 << 18) ** (-1 / 3)
+    assert np.array_equal(oracle.fof(x, rl, boxsize=1.0, method="cells"), oracle.fof(x, rl, boxsize=1.0))
+
+
 def test_fof_vs_float64_components(oracle):
     """Groups equal the connected components of the float64 pair graph (no pair within 1e-6 of
     the threshold in this sample), checked with fof_is_superset both ways (checks/tests/test_fof.py)."""
