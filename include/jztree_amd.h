@@ -3,9 +3,9 @@
  *
  * One entry point per XLA-FFI target that jztree binds for this path.  Each declaration cites
  * the reference interface it replaces (paths relative to the jztree repository):
- *   handler signature  src/jztree_cuda/generated/ffi_*.cu
- *   host function      src/jztree_cuda/*.cuh
- *   Python call site   src/jztree/*.py
+ *   handler signature  src/jztree_cuda/generated/ffi_<name>.cu
+ *   host function      src/jztree_cuda/<name>.cuh
+ *   Python call site   src/jztree/<name>.py
  *
  * Conventions (mirroring the reference's FFI contract, see ffi_knn.cu:159-168):
  *   - buffer sizes the FFI handlers infer from buffer dimensions are explicit arguments here;
@@ -49,9 +49,14 @@ int jzt_abi_version(void);
  *   JZT_TUNE_LEAF_WAVES  waves per EU the leaf kNN kernel is compiled for when k <= 32 (1, 2, 4;
  *                        default 4): the register budget per lane against occupancy
  *   JZT_TUNE_SORT_KPT    keys per thread of the onesweep radix passes (8, 12, 16; default 12):
- *                        tile = 256 * kpt keys */
+ *                        tile = 256 * kpt keys
+ *   JZT_TUNE_XCD         order of the query leaves in the kNN and FoF leaf kernels (0, 1; default
+ *                        1): 1 gives each of the 8 XCDs a contiguous eighth of the z-ordered
+ *                        leaves, so that neighbouring leaves share an L2; 0 deals consecutive
+ *                        leaves round-robin to the XCDs */
 #define JZT_TUNE_LEAF_WAVES 0
 #define JZT_TUNE_SORT_KPT 1
+#define JZT_TUNE_XCD 2
 int jzt_set_tuning(int knob, int value);
 int jzt_get_tuning(int knob);
 
@@ -126,6 +131,16 @@ int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* l
                           int64_t size_nodes, int64_t size_part, int dim, int dtype, int lvl_invalid,
                           uint32_t mode_flags, int upper_extent);
 
+/* jzt_geometry_to_f32 -- no counterpart in the reference (extension for float64 positions with
+ * float32 distance kernels).  Per node of a float64 tree (centres cent (n, dim), levels lvl): the
+ * smallest enclosing Morton cell whose boundaries and centre are float32 numbers (per-dim side at
+ * least 2^(E-23) for coordinates below 2^E in magnitude), as float32 centres and levels.  It
+ * contains the float32-rounded points of the node (rounding is monotone and fixes those
+ * boundaries), so the float32 node kernels prune exactly for the rounded points.  Invalid slots
+ * (lvl_invalid or non-finite centre) are copied. */
+int jzt_geometry_to_f32(jzt_stream_t stream, const double* cent, const int32_t* lvl, int64_t n, int dim,
+                        int lvl_invalid, float* cent_out, int32_t* lvl_out);
+
 /* The JAX glue of the tree build, as device ops (no reference FFI counterpart: the reference runs
  * these steps as XLA ops inside tree.py).
  *
@@ -135,6 +150,11 @@ int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* l
 size_t jzt_where_tmp_bytes(int64_t n);
 int jzt_where_flags(jzt_stream_t stream, const int8_t* flags, int64_t n, int32_t* out, int64_t size,
                     int32_t fill, const int32_t* fill_dev, int32_t* count, void* tmp, size_t tmp_bytes);
+
+/* jzt_count_not_nan: data.py:82-90 get_num's default `jnp.sum(~jnp.isnan(pos[..., 0]))`, the
+ * filled particle count, as one device op: *out (device int32) = number of rows whose first
+ * coordinate is not NaN (a kernel zero-fills it first; no memset, so it can be graph-captured). */
+int jzt_count_not_nan(jzt_stream_t stream, const void* pos, int64_t n, int dim, int dtype, int32_t* out);
 
 /* jzt_regularization_level: tree.py:55-68 + :160-187 find_regularization_level.  Histogram of
  * `lvl` weighted by `weights` (NULL = 1) over [min_lvl, max_lvl], then the first level whose
@@ -192,21 +212,6 @@ int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_ilist_spl, c
                          int64_t size_parents, int64_t size_nodes, int64_t parent_ilist_size,
                          int64_t node_ilist_size, int k, float boxsize, int dim, int dtype);
 
-/* jzt_knn_group (extension; k <= 64, float32): the leaf plane of the kNN walk over query groups.
- * A group is a plane-1 node: its queries (the particles of its child leaves, ranges from
- * parent_to_leaf and leaf_to_part) go 64 at a time through one wave, which walks the node's
- * r2-sorted plane-1 list (parent_ilist_*, as KnnNode2Node writes it for plane 1) and evaluates
- * the listed nodes' child leaves (cells in leaf_nodes: centre + level bits, dim+1 floats per
- * leaf) on the fly.  Replaces the last KnnNode2Node + KnnLeaf2Leaf (knn.cuh:602-694, :309-356)
- * with the same result; lanes stay busy across leaf boundaries.  init_r2 (per group, NULL = none;
- * negative = skip the group) starts every query's top-k at that bound; queries left with fewer
- * than k neighbours inside it are flagged in retry (required with init_r2).  Output rows as
- * jzt_knn_leaf2leaf_ex (dst_rows, out_rows). */
-int jzt_knn_group(jzt_stream_t stream, const int32_t* parent_ilist_spl, const int32_t* parent_ilist_isrc,
-                  const float* parent_ilist_r2, int64_t parent_ilist_size, const int32_t* parent_to_leaf,
-                  const void* leaf_nodes, const int32_t* leaf_to_part, const void* pos, const int32_t* src_ids,
-                  const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* knn_rad, int32_t* knn_id,
-                  int64_t ngroups, int64_t out_rows, int k, float boxsize, int dim, int dtype);
 
 /* Regularization's extra leaf splits (tree.py:236-243 of the reference, one kernel here):
  * flags[i] = flags[i] | (lvl[i] >= *lvl_max && i <= *npart) as 0/1 bytes, i < n; lvl_max and npart
@@ -377,6 +382,36 @@ int jzt_fof_leaf2leaf_sets(jzt_stream_t stream, const int32_t* ilist_spl, const 
                            const int32_t* spl, const float* leaf_r2link, const void* pos,
                            const int32_t* part_igroup_in, int32_t* part_igroup, int64_t size_leaves,
                            int64_t size_part, float boxsize, int dim, int dtype);
+
+/* ------------------------------------------------------------------------------------------ */
+/* FoF catalogue                                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+/* jzt_fof_order -- the group order of the reference's _fof_order (fof.py:510-519), which runs as
+ * XLA ops (argsort + scatter-add) with no FFI target.  igroup: labels of the z-ordered particles
+ * (each the index of the group's first member); npart (device int32, NULL = n): the filled count.
+ * isort = stable argsort of (igroup[i] for i < npart, n otherwise); counts[i] = number of filled
+ * particles labelled isort[i] (the group size at each group's first member, 0 elsewhere). */
+size_t jzt_fof_order_tmp_bytes(int64_t n);
+int jzt_fof_order(jzt_stream_t stream, const int32_t* igroup, const int32_t* npart, int64_t n, int32_t* isort,
+                  int32_t* counts, void* tmp, size_t tmp_bytes);
+
+/* jzt_fof_catalogue -- the reductions of _fof_catalogue_from_groups (fof.py:650-766, one GPU),
+ * XLA ops in the reference.  Particles in group order (AoS pos/vel of `dtype`, optional per-particle
+ * mass, v_rad, zplus1; mass == NULL means every particle has mass mass_scalar) with counts[i] the
+ * group size at each group's first member (jzt_fof_order).  Groups with >= npart_min members are
+ * catalogued in particle order: *ngroups (device), count, offset (first particle; size_part past
+ * ngroups), mass (mass array: sum; else count * mass_scalar), com_pos (centre of mass, periodic
+ * differences to the first member when boxsize > 0, wrapped into the box), com_inertia_radius,
+ * com_vel, v_rad, scale_factor (mass-weighted mean of 1/zplus1).  Every output but ngroups, count
+ * and offset may be NULL (not computed; the inertia radius needs com_pos internally only).  Slots
+ * past ngroups: count 0, offset size_part, mass 0, NaN elsewhere.  Sums run in float64. */
+size_t jzt_fof_catalogue_tmp_bytes(int64_t size_part, int64_t size_cata, int dim, int dtype);
+int jzt_fof_catalogue(jzt_stream_t stream, const void* pos, const void* mass, double mass_scalar, const void* vel,
+                      const void* v_rad, const void* zplus1, const int32_t* counts, int64_t size_part,
+                      int64_t size_cata, int npart_min, double boxsize, int dim, int dtype, int32_t* ngroups,
+                      int32_t* count, int32_t* offset, void* mass_out, void* com_pos, void* com_inertia_radius,
+                      void* com_vel, void* v_rad_out, void* scale_factor, void* tmp, size_t tmp_bytes);
 
 /* jzt_graph_count_nodes: diagnostic, no counterpart in the reference.  counts[0..3] = kernel,
  * memcpy, memset and other nodes of a HIP graph (hipGraph_t passed as void*).  Lets a caller that

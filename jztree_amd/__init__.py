@@ -11,6 +11,6 @@ there is no CPU fallback.
 """
 from . import config, data, tools  # noqa: F401
 from . import _lib  # noqa: F401  (loads libjztree_amd.so, raises if missing)
-from . import tree, knn, fof, proposer, batch, autotune  # noqa: F401
+from . import tree, knn, fof, proposer, batch, autotune, graph  # noqa: F401
 
-__all__ = ["config", "data", "tools", "tree", "knn", "fof", "proposer", "batch", "autotune"]
+__all__ = ["config", "data", "tools", "tree", "knn", "fof", "proposer", "batch", "autotune", "graph"]

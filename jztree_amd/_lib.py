@@ -19,6 +19,7 @@ _p = _c.c_void_p
 _i64 = _c.c_int64
 _i32 = _c.c_int
 _f32 = _c.c_float
+_f64 = _c.c_double
 _u32 = _c.c_uint32
 _sz = _c.c_size_t
 
@@ -37,8 +38,10 @@ _SIGNATURES = {
     "jzt_flag_leaf_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
     "jzt_find_node_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32], _i32),
     "jzt_get_node_geometry": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32, _i32, _u32, _i32], _i32),
+    "jzt_geometry_to_f32": ([_p, _p, _p, _i64, _i32, _i32, _p, _p], _i32),
     "jzt_where_tmp_bytes": ([_i64], _sz),
     "jzt_where_flags": ([_p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _sz], _i32),
+    "jzt_count_not_nan": ([_p, _p, _i64, _i32, _i32, _p], _i32),
     "jzt_dense_ilist": ([_p, _p, _i32, _p, _p, _i64, _p, _p, _i64, _p], _i32),
     "jzt_dense_ilist_sets_tmp_bytes": ([_i64], _sz),
     "jzt_graph_count_nodes": ([_p, _p], _i32),
@@ -53,8 +56,6 @@ _SIGNATURES = {
     "jzt_flags_or_level": ([_p, _p, _p, _i64, _p, _p], _i32),
     "jzt_plane_cap": ([_p, _p, _p, _i64, _p, _p, _p, _i64, _i32, _i32, _p], _i32),
     "jzt_packed_get": ([_p, _p, _i64, _p, _i32, _i32, _p, _i32, _i32, _p, _i64], _i32),
-    "jzt_knn_group": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _f32, _i32,
-                       _i32], _i32),
     "jzt_knn_leaf2leaf_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_knn_leaf2leaf": ([_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _i64, _i64, _i32, _f32,
                            _i32, _i32], _i32),
@@ -77,6 +78,11 @@ _SIGNATURES = {
     "jzt_fof_node2node_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _i32, _i32],
                                _i32),
     "jzt_fof_leaf2leaf_sets": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
+    "jzt_fof_order_tmp_bytes": ([_i64], _sz),
+    "jzt_fof_order": ([_p, _p, _p, _i64, _p, _p, _p, _sz], _i32),
+    "jzt_fof_catalogue_tmp_bytes": ([_i64, _i64, _i32, _i32], _sz),
+    "jzt_fof_catalogue": ([_p, _p, _p, _f64, _p, _p, _p, _p, _i64, _i64, _i32, _f64, _i32, _i32, _p, _p, _p, _p, _p,
+                           _p, _p, _p, _p, _p, _sz], _i32),
 }
 
 

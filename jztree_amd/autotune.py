@@ -31,12 +31,12 @@ import torch
 from . import _lib as L
 from .config import KNNConfig
 
-KNOBS = {"leaf_waves": 0, "sort_kpt": 1}  # JZT_TUNE_LEAF_WAVES, JZT_TUNE_SORT_KPT
-ALLOWED = {"leaf_waves": (1, 2, 4), "sort_kpt": (8, 12, 16)}
+KNOBS = {"leaf_waves": 0, "sort_kpt": 1, "xcd": 2}  # JZT_TUNE_LEAF_WAVES, JZT_TUNE_SORT_KPT, JZT_TUNE_XCD
+ALLOWED = {"leaf_waves": (1, 2, 4), "sort_kpt": (8, 12, 16), "xcd": (0, 1)}
 
 
 def set_knobs(**values):
-    """Set library tuning knobs by name (leaf_waves, sort_kpt)."""
+    """Set library tuning knobs by name (leaf_waves, sort_kpt, xcd)."""
     for name, v in values.items():
         if name not in KNOBS:
             raise ValueError(f"unknown knob {name!r}; known: {sorted(KNOBS)}")

@@ -31,6 +31,7 @@ from . import checks
 from .config import FofConfig, KNNConfig
 from .data import Pos
 from .fof import _fof_dual_walk, _fof_leaf2leaf
+from .graph import Captured
 from .knn import _knn_dual_walk, _knn_leaf2leaf
 from .tree import _tmp, build_tree_hierarchy
 
@@ -211,27 +212,10 @@ class CapturedSets:
         cfg_tree = cfg_tree if cfg_tree is not None else KNNConfig().tree
         args = (k, self.r2set, boxsize, cfg_tree, alloc_fac_ilist_knn, alloc_fac_ilist_fof)
         self._step = lambda: _pipeline_core(self.pos, self.off, *args)
-        # eager warm-up on a side stream (first-call setup stays out of the capture), checked
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            try:
-                self._step()
-                checks.flush()
-            except BaseException:
-                checks.clear()
-                raise
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph(keep_graph=True)
-        checks.clear()
-        with torch.cuda.graph(self.graph):
-            self.out = self._step()
-        self._conds = checks.take()
-        counts = torch.zeros(4, dtype=torch.int64)
-        L.call("jzt_graph_count_nodes", self.graph.raw_cuda_graph(), counts.data_ptr())
-        self.node_counts = dict(zip(("kernel", "memcpy", "memset", "other"), counts.tolist()))
-        self.graph.instantiate()
+        self._g = Captured(self._step)
+        self.graph = self._g.graph
+        self.out = self._g.out
+        self.node_counts = self._g.node_counts
 
     def memset_nodes(self) -> int:
         return self.node_counts["memset"]
@@ -245,4 +229,4 @@ class CapturedSets:
 
     def check(self):
         """Raise if the last replay overflowed an allocation (synchronises)."""
-        checks.evaluate(self._conds)
+        self._g.check()

@@ -7,6 +7,7 @@ recorded conditions are read back with one copy when the public entry point retu
 """
 from __future__ import annotations
 
+import contextlib
 import threading
 
 import torch
@@ -37,8 +38,29 @@ def take() -> list:
     return items
 
 
+@contextlib.contextmanager
+def deferred():
+    """Within this block nothing reads device data on the host, so the calls can be captured into
+    a HIP graph: flush() keeps the recorded conditions for take() (to be evaluated after each
+    replay), and passes whose launch depends on device data (retries of short queries) are
+    launched unconditionally; they do nothing when there is nothing to redo."""
+    prev = getattr(_tls, "deferred", False)
+    _tls.deferred = True
+    try:
+        yield
+    finally:
+        _tls.deferred = prev
+
+
+def is_deferred() -> bool:
+    return getattr(_tls, "deferred", False)
+
+
 def flush():
-    """Raise RuntimeError for the first recorded condition that holds; then forget all."""
+    """Raise RuntimeError for the first recorded condition that holds; then forget all.  Inside
+    deferred() the conditions are kept instead."""
+    if is_deferred():
+        return
     evaluate(take())
 
 

@@ -234,11 +234,11 @@ template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
                                                  float r2link_all, const float* __restrict__ r2v, T boxsize,
-                                                 int64_t il_size, int64_t nitems) {
+                                                 int64_t il_size, int64_t nitems, int xcd) {
   __shared__ FofRec<D, T> s_r[64];
   const int lane = threadIdx.x;
-  const int64_t nact_ = active_items(spl, nitems);
-  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+  const ItemRange items = xcd_items(active_items(spl, nitems), xcd != 0);
+  for (int64_t lq = items.first; lq < items.end; lq += items.step) {
     const int q0 = spl[lq], q1 = spl[lq + 1];
     const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
     const T half = (T)(float)(T(0.5) * boxsize);
@@ -464,12 +464,13 @@ int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64
   if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_part * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return fail("FofLeaf2Leaf: copy failed");
   if (size_leaves > 0) {
+    const int xcd = tune_get(kTuneXcd);
     if (boxsize > 0.f)
-      hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves);
+      hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd);
     else
-      hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3(grid_for(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves);
+      hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd);
   }
   if (size_part > 0)
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);

@@ -559,12 +559,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
                                                  const float* __restrict__ init_r2, int32_t* __restrict__ retry,
                                                  float* __restrict__ rad_out, int32_t* __restrict__ id_out,
                                                  uint64_t* __restrict__ resume, int koff, int ktot, float boxsize,
-                                                 int64_t il_size, int64_t out_rows, int64_t nitems) {
+                                                 int64_t il_size, int64_t out_rows, int64_t nitems, int xcd) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_buf[kBufWords];
   const int lane = threadIdx.x;
-  const int64_t nact_ = active_items(splQ, nitems);
-  for (int lq = blockIdx.x; lq < nact_; lq += gridDim.x) {
+  const ItemRange items = xcd_items(active_items(splQ, nitems), xcd != 0);
+  for (int64_t lq = items.first; lq < items.end; lq += items.step) {
     uint64_t* const row = s_buf + lane * kBufStride;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int q0 = splQ[lq], q1 = splQ[lq + 1];
@@ -798,16 +798,17 @@ int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, 
   if (nleaves_q <= 0) return 0;
   if (init_r2 && k > 64) return fail("KnnLeaf2Leaf: a start bound needs k <= 64");
   uint64_t* resume = k > 64 ? (uint64_t*)tmp : nullptr;
-  dim3 g(grid_for(nleaves_q)), b(64);
+  dim3 g(grid_for8(nleaves_q)), b(64);
   const float* xt = (const float*)xT;
   const float* xq = (const float*)xQ;
   const int waves = tune_get(kTuneLeafWaves);
+  const int xcd = tune_get(kTuneXcd);
   for (int koff = 0; koff < k; koff += 64) {
     const int kn = min(k - koff, 64);
     const bool floor = resume && koff > 0;
 #define JZT_KNN_LAUNCH(KK, PER, FLOOR, W)                                                                              \
   hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D, W>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
-                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q)
+                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q, xcd)
   // waves per EU (the register budget of the kernel): tunable for K <= 32; the multi-pass tail
   // (FLOOR) and K = 64 keep their fixed choice
 #define JZT_KNN_LAUNCH_W(KK, PER)                                                  \

@@ -486,8 +486,6 @@ struct SortLayout {
   int64_t ntiles;
 };
 
-inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 SortLayout sort_layout(int64_t n, int dim, int dtype) {
   SortLayout L;
   size_t o = 0;
@@ -823,10 +821,133 @@ int zsort_seg_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>
   return check_launch("zsort_segmented gather");
 }
 
+// ---------------------------------------------------------------------------------------------
+// 8. group order (FoF catalogue): stable sort of the particles by their group label
+// ---------------------------------------------------------------------------------------------
+// _fof_order (fof.py:510-519): igr[i] = igroup[i] for the filled particles (i < npart), n for the
+// rest; isort = stable argsort of igr; counts[i] = number of filled particles labelled isort[i]
+// (the group size at each group's first member, whose index is the label; 0 elsewhere; labels
+// out of [0, n) are not counted, as the reference's scatter-add drops them).  Labels have
+// ceil(bits(n) / 8) radix digits, sorted by that many of the onesweep passes of the z-order sort;
+// the label counts come from wave-aggregated atomics (equal labels in consecutive lanes are added
+// once).
+
+__global__ __launch_bounds__(256) void k_label_keys(const int32_t* __restrict__ igroup,
+                                                    const int32_t* __restrict__ npart_dev, int64_t n, int npass,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ hist, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t sh[4][kRadix];
+  for (int i = threadIdx.x; i < 4 * kRadix; i += blockDim.x) (&sh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t np = npart_dev ? min(max((int64_t)npart_dev[0], (int64_t)0), n) : n;
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // whole waves step together (the aggregation below needs every lane)
+  const int64_t nw = (n + 63) & ~(int64_t)63;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i - lane < nw; i += stride) {
+    const bool in = i < n;
+    const int64_t lab64 = (in && i < np) ? (int64_t)igroup[i] : n;
+    const bool counted = in && i < np && lab64 >= 0 && lab64 < n;
+    const uint32_t key = in ? (uint32_t)(lab64 >= 0 && lab64 <= n ? lab64 : n) : 0u;
+    if (in) {
+      keys[i] = key;
+      vals[i] = (uint32_t)i;
+      for (int p = 0; p < npass; p++) atomicAdd(&sh[p][(key >> (8 * p)) & 255u], 1u);
+    }
+    // runs of equal counted labels in consecutive lanes: the first lane of a run adds its length
+    const int prev = __shfl_up((int)key, 1, 64);
+    const bool prev_counted = __shfl_up((int)counted, 1, 64) != 0;
+    const bool head = counted && !(lane > 0 && prev_counted && (uint32_t)prev == key);
+    const uint64_t heads = __ballot(head), cmask = __ballot(counted);
+    if (head) {
+      const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
+      const uint64_t stop = (heads | ~cmask) & above;
+      const int end = stop ? __builtin_ctzll(stop) : 64;
+      atomicAdd(&cnt[key], (uint32_t)(end - lane));
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < npass * kRadix; j += blockDim.x) {
+    const uint32_t c = (&sh[0][0])[j];
+    if (c) atomicAdd(&hist[j], c);
+  }
+}
+
+__global__ void k_label_counts(const uint32_t* __restrict__ isort, const uint32_t* __restrict__ cnt, int64_t n,
+                               int32_t* __restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counts[i] = (int32_t)cnt[isort[i] < (uint64_t)n ? isort[i] : 0];
+}
+
+struct OrderLayout {
+  size_t keys0, keys1, vals0, vals1, cnt, hist, meta, lookback, total;
+  int64_t ntiles;
+};
+
+OrderLayout order_layout(int64_t n) {
+  OrderLayout L;
+  size_t o = 0;
+  L.ntiles = div_up(n > 0 ? n : 1, kSortThreads * kKPTMin);
+  L.keys0 = o; o = align256(o + 8 * (size_t)n);
+  L.keys1 = o; o = align256(o + 8 * (size_t)n);
+  L.vals0 = o; o = align256(o + 4 * (size_t)n);
+  L.vals1 = o; o = align256(o + 4 * (size_t)n);
+  // cnt, hist, meta and the look-back state are zeroed together
+  L.cnt = o; o += 4 * (size_t)(n + 1);
+  L.hist = o; o += 4 * (size_t)kPasses * kRadix;
+  L.meta = o; o += 4 * 16;
+  L.lookback = o; o = align256(o + 4 * (size_t)4 * L.ntiles * kRadix);
+  L.total = o;
+  return L;
+}
+
+int fof_order_impl(hipStream_t s, const int32_t* igroup, const int32_t* npart, int64_t n, int32_t* isort,
+                   int32_t* counts, char* tmp) {
+  const OrderLayout L = order_layout(n);
+  uint64_t* k0 = (uint64_t*)(tmp + L.keys0);
+  uint64_t* k1 = (uint64_t*)(tmp + L.keys1);
+  uint32_t* v0 = (uint32_t*)(tmp + L.vals0);
+  uint32_t* v1 = (uint32_t*)(tmp + L.vals1);
+  uint32_t* cnt = (uint32_t*)(tmp + L.cnt);
+  uint32_t* hist = (uint32_t*)(tmp + L.hist);
+  uint32_t* meta = (uint32_t*)(tmp + L.meta);
+  uint32_t* lookback = (uint32_t*)(tmp + L.lookback);
+  if (fill_u32(s, cnt, 0u, (int64_t)((L.total - L.cnt) / 4))) return 1;
+  int bits = 0;
+  while (bits < 32 && ((uint64_t)n >> bits) != 0) bits++;  // keys are <= n
+  const int npass = std::max(1, (bits + 7) / 8);
+  const unsigned gb = (unsigned)std::min(div_up(n, 256), (int64_t)2048);
+  hipLaunchKernelGGL(k_label_keys, dim3(gb), dim3(256), 0, s, igroup, npart, n, npass, k0, v0, hist, cnt);
+  hipLaunchKernelGGL(k_hist_scan, dim3(1), dim3(256), 0, s, hist);
+  if (check_launch("FofOrder keys")) return 1;
+  for (int p = 0; p < npass; p++) {
+    const uint64_t* ki = (p & 1) ? k1 : k0;
+    const uint32_t* vi = (p & 1) ? v1 : v0;
+    uint64_t* ko = (p & 1) ? k0 : k1;
+    // the last pass writes the permutation straight into isort
+    uint32_t* vo = (p == npass - 1) ? (uint32_t*)isort : ((p & 1) ? v0 : v1);
+    onesweep(s, ki, vi, ko, vo, n, 8 * p, hist + p * kRadix, lookback + (int64_t)p * L.ntiles * kRadix, meta + 8 + p);
+  }
+  hipLaunchKernelGGL(k_label_counts, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, (const uint32_t*)isort, cnt, n,
+                     counts);
+  return check_launch("FofOrder");
+}
+
 }  // namespace
 }  // namespace jzt
 
 using namespace jzt;
+
+extern "C" size_t jzt_fof_order_tmp_bytes(int64_t n) { return order_layout(n).total; }
+
+extern "C" int jzt_fof_order(jzt_stream_t stream, const int32_t* igroup, const int32_t* npart, int64_t n,
+                             int32_t* isort, int32_t* counts, void* tmp, size_t tmp_bytes) {
+  if (n < 0) return fail("jzt_fof_order: negative n");
+  if (n == 0) return 0;
+  if (n >= (int64_t)kCountMask) return fail("jzt_fof_order: n must be < 2^30");
+  if (tmp_bytes < order_layout(n).total) return fail("jzt_fof_order: tmp buffer too small");
+  return fof_order_impl((hipStream_t)stream, igroup, npart, n, isort, counts, (char*)tmp);
+}
 
 extern "C" size_t jzt_zsort_tmp_bytes(int64_t n, int dim, int dtype) { return sort_layout(n, dim, dtype).total; }
 

@@ -338,10 +338,76 @@ __global__ __launch_bounds__(kRegThreads) void k_reg_level(const unsigned long l
   if (tid == 0) out[0] = s_first[1] == INT_MAX ? max_lvl : s_first[1] + min_lvl;
 }
 
+// Node geometry of a float64 tree for float32 distance kernels on the float32-rounded points.
+// The node's Morton cell (c, level) holds its float64 points.  It is replaced by the smallest
+// enclosing Morton cell whose per-dim sides are at least 2^(E_i - 23), where every coordinate of
+// the cell has magnitude <= 2^E_i: the boundaries of that cell (multiples of its side) and its
+// centre (multiples of half a side) are then float32 numbers, so rounding to float32, which is
+// monotone and leaves those boundaries fixed, keeps every point inside it.  The float32 cell is
+// therefore a conservative bound for the rounded points, as the float64 cell is for the originals.
+// Levels past the float32 range (a node spanning signs) become the float32 "whole space" level.
+template <int D>
+__global__ void k_geometry_to_f32(const double* __restrict__ cent, const int32_t* __restrict__ lvl, int64_t n,
+                                  int lvl_invalid, float* __restrict__ cent_out, int32_t* __restrict__ lvl_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int level = lvl[i];
+  double c[D];
+  bool finite = true;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    c[d] = cent[i * D + d];
+    finite &= __builtin_isfinite(c[d]);
+  }
+  const int top32 = 128 * D;  // float32: cells spanning signs or infinities
+  if (level == lvl_invalid || !finite || level >= top32) {
+#pragma unroll
+    for (int d = 0; d < D; d++) cent_out[i * D + d] = (float)c[d];
+    lvl_out[i] = (level == lvl_invalid || !finite) ? level : top32;
+    return;
+  }
+  int L = level;
+  for (int guard = 0; guard < 64 * D; guard++) {
+    int l[D];
+    lvl_vec<D>(L, l);
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      // |x| <= |centre| + side for x in the cell; E = exponent with that bound <= 2^E
+      const double m = fabs(round_pow2_cent(c[d], l[d])) + ldexp(1.0, l[d]);
+      int e;
+      frexp(m, &e);  // m < 2^e
+      ok &= l[d] >= max(e - 23, -149);
+    }
+    if (ok) break;
+    L++;
+  }
+  if (L >= top32) L = top32;
+  int l[D];
+  lvl_vec<D>(L, l);
+#pragma unroll
+  for (int d = 0; d < D; d++) cent_out[i * D + d] = (float)round_pow2_cent(c[d], l[d]);
+  lvl_out[i] = L;
+}
+
 }  // namespace
 }  // namespace jzt
 
 using namespace jzt;
+
+extern "C" int jzt_geometry_to_f32(jzt_stream_t stream, const double* cent, const int32_t* lvl, int64_t n, int dim,
+                                   int lvl_invalid, float* cent_out, int32_t* lvl_out) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n <= 0) return 0;
+  const unsigned g = (unsigned)div_up(n, 256);
+  if (dim == 3)
+    hipLaunchKernelGGL(k_geometry_to_f32<3>, dim3(g), dim3(256), 0, s, cent, lvl, n, lvl_invalid, cent_out, lvl_out);
+  else if (dim == 2)
+    hipLaunchKernelGGL(k_geometry_to_f32<2>, dim3(g), dim3(256), 0, s, cent, lvl, n, lvl_invalid, cent_out, lvl_out);
+  else
+    return fail("GeometryToF32: unsupported dim");
+  return check_launch("GeometryToF32");
+}
 
 extern "C" size_t jzt_regularization_tmp_bytes(int min_lvl, int max_lvl) {
   return (size_t)(max_lvl - min_lvl + 1) * 8;

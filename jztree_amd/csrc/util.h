@@ -19,8 +19,11 @@ int scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, bool inc
 
 // process-wide tuning knobs (set through jzt_set_tuning; the values the kernels were written for
 // are the defaults)
-enum TuneKnob { kTuneLeafWaves = 0, kTuneSortKpt = 1, kTuneCount = 2 };
+enum TuneKnob { kTuneLeafWaves = 0, kTuneSortKpt = 1, kTuneXcd = 2, kTuneCount = 3 };
 int tune_get(int knob);
+
+// byte offsets of the regions inside one scratch buffer are kept 256-byte aligned
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // fill n 32-bit words with v
 int fill_u32(hipStream_t s, void* p, uint32_t v, int64_t n);
@@ -77,6 +80,27 @@ __device__ __forceinline__ int64_t active_items(const int32_t* __restrict__ spl,
     else hi = mid;
   }
   return lo;
+}
+
+// XCD-aware order of a grid-stride loop over n items.  The chip deals workgroups to its 8 XCDs
+// round-robin (workgroup b runs on XCD b % 8) and every XCD has its own L2, so the plain order
+// (block b takes items b, b + grid, ...) spreads neighbouring items over all eight L2s.  With `on`
+// (and a grid that is a multiple of 8), XCD x takes the x-th eighth of the items and its blocks
+// stride through that range: the items an XCD has in flight are neighbours, which in z-order share
+// most of their sources.  The item set is the same either way.
+struct ItemRange {
+  int64_t first, end, step;
+};
+__device__ __forceinline__ ItemRange xcd_items(int64_t n, bool on) {
+  const int64_t g = gridDim.x;
+  if (!on || (g & 7) != 0) return {(int64_t)blockIdx.x, n, g};
+  const int64_t x = blockIdx.x & 7, per = (n + 7) >> 3;
+  return {x * per + (int64_t)(blockIdx.x >> 3), n < (x + 1) * per ? n : (x + 1) * per, g >> 3};
+}
+// grid for xcd_items: at most kMaxGrid blocks, a multiple of 8
+inline unsigned grid_for8(int64_t n) {
+  int64_t g = n <= 0 ? 8 : (n < kMaxGrid ? n : kMaxGrid);
+  return (unsigned)((g + 7) & ~(int64_t)7);
 }
 
 }  // namespace jzt

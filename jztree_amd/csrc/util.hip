@@ -268,16 +268,55 @@ __global__ void k_flags_or_level(int8_t* __restrict__ flags, const int32_t* __re
   flags[i] = (flags[i] != 0 || extra) ? 1 : 0;
 }
 
-static int g_tune[kTuneCount] = {4, 12};  // leaf kNN waves per EU (K <= 32), onesweep keys per thread
+// leaf kNN waves per EU (K <= 32), onesweep keys per thread, XCD-aware order of the leaf kernels
+static int g_tune[kTuneCount] = {4, 12, 1};
 int tune_get(int knob) { return (knob >= 0 && knob < kTuneCount) ? __atomic_load_n(&g_tune[knob], __ATOMIC_RELAXED) : 0; }
 
 }  // namespace jzt
+
+namespace jzt {
+namespace {
+// number of rows whose first coordinate is not NaN: per-block counts, one atomic per block
+template <typename T>
+__global__ __launch_bounds__(256) void k_count_not_nan(const T* __restrict__ pos, int64_t n, int dim,
+                                                       uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_c;
+  if (threadIdx.x == 0) s_c = 0;
+  __syncthreads();
+  uint32_t c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += __builtin_isnan(pos[i * dim]) ? 0u : 1u;
+  for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_c, c);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_c) atomicAdd(out, s_c);
+}
+}  // namespace
+}  // namespace jzt
+
+extern "C" int jzt_count_not_nan(jzt_stream_t stream, const void* pos, int64_t n, int dim, int dtype, int32_t* out) {
+  using namespace jzt;
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0 || n >= (int64_t)INT32_MAX) return fail("jzt_count_not_nan: n must be in [0, 2^31)");
+  if (dim < 1) return fail("jzt_count_not_nan: dim < 1");
+  if (fill_u32(s, out, 0u, 1)) return 1;
+  if (n == 0) return 0;
+  const unsigned g = (unsigned)std::min(div_up(n, 256), (int64_t)1024);
+  if (dtype == JZT_F32)
+    hipLaunchKernelGGL(k_count_not_nan<float>, dim3(g), dim3(256), 0, s, (const float*)pos, n, dim, (uint32_t*)out);
+  else if (dtype == JZT_F64)
+    hipLaunchKernelGGL(k_count_not_nan<double>, dim3(g), dim3(256), 0, s, (const double*)pos, n, dim, (uint32_t*)out);
+  else
+    return fail("jzt_count_not_nan: unsupported dtype");
+  return check_launch("jzt_count_not_nan");
+}
 
 extern "C" int jzt_set_tuning(int knob, int value) {
   using namespace jzt;
   bool ok = false;
   if (knob == kTuneLeafWaves) ok = value == 1 || value == 2 || value == 4;
   else if (knob == kTuneSortKpt) ok = value == 8 || value == 12 || value == 16;
+  else if (knob == kTuneXcd) ok = value == 0 || value == 1;
   else return fail("jzt_set_tuning: unknown knob " + std::to_string(knob));
   if (!ok) return fail("jzt_set_tuning: value " + std::to_string(value) + " not allowed for knob " + std::to_string(knob));
   __atomic_store_n(&g_tune[knob], value, __ATOMIC_RELAXED);

@@ -78,10 +78,22 @@ def get_num(part, default_to_length=False, default_to_nancount=True):
         if default_to_length:
             return len(get_pos(part))
         if default_to_nancount:
-            return torch.sum(~torch.isnan(get_pos(part)[:, 0])).to(torch.int32)
+            return count_not_nan(get_pos(part))
         raise ValueError("Need .num attribute in particle structure for distributed mode")
     num = part.num
     return torch.sum(num) if torch.is_tensor(num) else num
+
+
+def count_not_nan(pos: torch.Tensor) -> torch.Tensor:
+    """Number of rows whose first coordinate is not NaN, as a device int32 scalar: one library
+    kernel (a torch reduction over many blocks would clear its scratch with a memset, which a
+    captured HIP graph should not hold)."""
+    from . import _lib as L
+    pos = pos.contiguous()
+    out = torch.empty((), dtype=torch.int32, device=pos.device)
+    L.call("jzt_count_not_nan", L.stream(), L.ptr(pos), pos.shape[0], pos.shape[-1], L.dtype_code(pos.dtype),
+           L.ptr(out))
+    return out
 
 
 def get_num_total(part, default_to_length=False):
@@ -332,3 +344,61 @@ def verify_ilist(ilist: InteractionList) -> InteractionList:
         raise ValueError("Intercation list allocation too large and may produce integer overflows",
                          "Hint: Use more GPUs or decrease alloc_fac_ilist (if possible)")
     return ilist
+
+
+# ------------------------------------------------------------------------------------------------
+# FoF catalogue (data.py:573-680 of the reference)
+# ------------------------------------------------------------------------------------------------
+
+@dataclass
+class FofCatalogue:
+    """Per-group properties of a FoF catalogue, arrays of the catalogue's allocated size with
+    `ngroups` (device int32 scalar) filled; optional properties are None when the particles lack
+    the data.  count/offset: members and first member in the group-ordered particle array;
+    com_pos/com_vel: centre of mass and mean velocity; com_inertia_radius: sqrt of the mean squared
+    distance to the centre of mass; scale_factor/v_rad: mass-weighted means for light-cone data;
+    offset_rank: only set on catalogues squeezed from several devices."""
+    ngroups: torch.Tensor
+    mass: torch.Tensor | None = None
+    count: torch.Tensor | None = None
+    offset: torch.Tensor | None = None
+    com_pos: torch.Tensor | None = None
+    com_vel: torch.Tensor | None = None
+    com_inertia_radius: torch.Tensor | None = None
+    scale_factor: torch.Tensor | None = None
+    v_rad: torch.Tensor | None = None
+    offset_rank: torch.Tensor | None = None
+
+
+def squeeze_catalogue(cata: FofCatalogue, size_out: int | None = None, offset_mode: str = "rank",
+                      nparts=None) -> FofCatalogue:
+    """The filled part of a catalogue (data.py:619-663).  On one GPU this only drops the slots past
+    ngroups (size_out defaults to ngroups, read back from the device)."""
+    assert offset_mode in ("rank", "global")
+    if cata.count.ndim != 1:
+        raise NotImplementedError("catalogues of several devices are not part of this one-GPU build")
+    if size_out is None:
+        size_out = int(cata.ngroups)
+    out = tree_map_by_len(lambda x: x[:size_out], cata, cata.count.shape[0])
+    return out
+
+
+def sort_catalogue(cata: FofCatalogue, by: str = "count", descending: bool = True) -> FofCatalogue:
+    """Catalogue entries ordered by one property (data.py:665-673), stable: equal keys keep their
+    order (as jnp.argsort's stable descending sort)."""
+    key = getattr(cata, by, None)
+    assert key is not None, f"catalogue doesn't have {by}"
+    assert cata.count.ndim == 1, "Please squeeze catalogue before sorting"
+    isort = torch.argsort(key, descending=descending, stable=True)
+    return tree_map_by_len(lambda x: x[isort], cata, len(isort))
+
+
+def catalogues_equal(c1: FofCatalogue, c2: FofCatalogue) -> bool:
+    """Every property equal (data.py:675-679)."""
+    for f in dataclasses.fields(FofCatalogue):
+        a, b = getattr(c1, f.name), getattr(c2, f.name)
+        if (a is None) != (b is None):
+            return False
+        if a is not None and not bool(torch.equal(a, b)):
+            return False
+    return True

@@ -1,8 +1,10 @@
-"""Friends-of-friends labels (the API of src/jztree/fof.py, single GPU).
+"""Friends-of-friends labels and catalogues (the API of src/jztree/fof.py, single GPU).
 
 Pipeline (fof.py:772-816): z-sort + tree -> dual walk over the planes (NodeToChildLabel,
 FofNode2Node: guaranteed node links + lists of uncertain node pairs) -> particle level
 (NodeToChildLabel, FofLeaf2Leaf).  Labels in z-order are the minimum z-index of each group.
+Catalogue (fof.py:510-519, :650-766, :872-907): particles brought into group order (jzt_fof_order),
+then per-group sums (jzt_fof_catalogue).
 """
 from __future__ import annotations
 
@@ -15,9 +17,10 @@ import torch
 from . import _lib as L
 from . import checks
 from .checks import raise_if
-from .config import FofConfig
-from .data import InteractionList, PackedArray, TreeHierarchy, get_pos, verify_ilist
-from .tree import build_tree_hierarchy, top_dense_interaction_list, zsort
+from .config import FofCatalogueConfig, FofConfig
+from .data import (FofCatalogue, InteractionList, PackedArray, TreeHierarchy, count_not_nan, get_pos,
+                   tree_map_by_len, verify_ilist)
+from .tree import _tmp, build_tree_hierarchy, top_dense_interaction_list, zsort
 
 
 @dataclass
@@ -184,20 +187,24 @@ def fof_is_superset(igroup_sup, igroup, mask=None):
 
 
 def fof_labels(part, rlink: float, boxsize: float = 0., cfg: FofConfig = FofConfig(), th: TreeHierarchy | None = None,
-               output_order: str = "z"):
+               output_order: str = "z", dist_dtype: torch.dtype | None = None):
     """Friends-of-friends groups on one GPU (fof.py:772-816).
 
     Returns (part, igroup): with output_order="z" the z-ordered particles and labels pointing to
     the first z-order particle of each group; with "input" the input particles and labels in
-    input order (the input index of the group's first z-order member)."""
+    input order (the input index of the group's first z-order member).  dist_dtype (extension):
+    torch.float32 with float64 positions links the float32-rounded positions with the float32
+    kernels over the float64 tree (z-order from the float64 keys)."""
     try:
         if th is None:
             partz, idz = zsort(part)
             th = build_tree_hierarchy(partz, cfg_tree=cfg.tree)
         else:
             partz, idz = part, None
+        from .knn import _distance_view
+        th, posz = _distance_view(th, get_pos(partz), dist_dtype)
         node_data, ilist = _fof_dual_walk(th, rlink=rlink, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
-        igroupz = _fof_leaf2leaf(node_data, ilist, get_pos(partz), rlink=rlink, boxsize=boxsize)
+        igroupz = _fof_leaf2leaf(node_data, ilist, posz, rlink=rlink, boxsize=boxsize)
         checks.flush()
     except BaseException:
         checks.clear()
@@ -211,3 +218,97 @@ def fof_labels(part, rlink: float, boxsize: float = 0., cfg: FofConfig = FofConf
 
 
 fof_labels.jit = fof_labels
+
+
+# ------------------------------------------------------------------------------------------------
+# group order and catalogue (fof.py:510-519, :650-766, :872-907)
+# ------------------------------------------------------------------------------------------------
+
+def _fof_order(igroup: torch.Tensor, part, npart=None):
+    """fof.py:510-519: particles in group order -- every group a contiguous run starting at its
+    first z-order member, groups ordered by that member, members in z-order -- and the group size
+    at each group's first member (0 elsewhere).  Returns (partf, counts)."""
+    pos = get_pos(part)
+    n = len(igroup)
+    dev = igroup.device
+    if npart is None:
+        npart = count_not_nan(pos)
+    if not torch.is_tensor(npart):
+        npart = torch.full((), int(npart), dtype=torch.int32, device=dev)
+    npart = npart.to(torch.int32).reshape(()).contiguous()
+    isort = torch.empty(n, dtype=torch.int32, device=dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    nb = L.LIB.jzt_fof_order_tmp_bytes(n)
+    tmp = _tmp(nb, dev)
+    L.call("jzt_fof_order", L.stream(), L.ptr(igroup.to(torch.int32).contiguous()), L.ptr(npart), n, L.ptr(isort),
+           L.ptr(counts), L.ptr(tmp), nb)
+    il = isort.long()
+    partf = tree_map_by_len(lambda x: x[il], part, n)
+    return partf, counts
+
+
+def _fof_catalogue_from_groups(part, group_counts: torch.Tensor, cfg_cata: FofCatalogueConfig = FofCatalogueConfig(),
+                               boxsize: float = 0., size_cata: int | None = None) -> FofCatalogue:
+    """fof.py:650-766 on one GPU: the catalogue of the groups with at least cfg_cata.npart_min
+    members, from particles in group order (_fof_order).  Properties the particles lack (mass,
+    vel, v_rad, zplus1) are None; without masses every particle weighs 1."""
+    pos = get_pos(part).contiguous()
+    n = len(group_counts)
+    dev = pos.device
+    dtype = pos.dtype
+    dim = pos.shape[-1]
+    npart_min = int(cfg_cata.npart_min)
+    if size_cata is None:
+        size_cata = (n + npart_min - 1) // npart_min  # worst case
+    mass = getattr(part, "mass", None)
+    mass_arr, mass_scalar = None, 1.0
+    if mass is not None:
+        if torch.is_tensor(mass) and mass.numel() > 1:
+            mass_arr = mass.to(dtype).contiguous()
+        else:
+            mass_scalar = float(mass)
+    vel = getattr(part, "vel", None)
+    vrad = getattr(part, "v_rad", None)
+    zp1 = getattr(part, "zplus1", None)
+    prep = lambda t: None if t is None else t.to(dtype).contiguous()
+    vel, vrad, zp1 = prep(vel), prep(vrad), prep(zp1)
+    f = lambda *shape: torch.empty(shape, dtype=dtype, device=dev)
+    ngroups = torch.empty((), dtype=torch.int32, device=dev)
+    count = torch.empty(size_cata, dtype=torch.int32, device=dev)
+    offset = torch.empty(size_cata, dtype=torch.int32, device=dev)
+    mass_out = f(size_cata) if mass is not None else None
+    com_pos, radius = f(size_cata, dim), f(size_cata)
+    com_vel = f(size_cata, dim) if vel is not None else None
+    vrad_out = f(size_cata) if vrad is not None else None
+    a_out = f(size_cata) if zp1 is not None else None
+    nb = L.LIB.jzt_fof_catalogue_tmp_bytes(n, size_cata, dim, L.dtype_code(dtype))
+    tmp = _tmp(nb, dev)
+    L.call("jzt_fof_catalogue", L.stream(), L.ptr(pos), L.ptr(mass_arr), mass_scalar, L.ptr(vel), L.ptr(vrad),
+           L.ptr(zp1), L.ptr(group_counts.to(torch.int32).contiguous()), n, size_cata, npart_min,
+           float(boxsize or 0.), dim, L.dtype_code(dtype), L.ptr(ngroups), L.ptr(count), L.ptr(offset),
+           L.ptr(mass_out), L.ptr(com_pos), L.ptr(radius), L.ptr(com_vel), L.ptr(vrad_out), L.ptr(a_out),
+           L.ptr(tmp), nb)
+    return FofCatalogue(ngroups=ngroups, mass=mass_out, count=count, offset=offset, com_pos=com_pos,
+                        com_vel=com_vel, com_inertia_radius=radius, scale_factor=a_out, v_rad=vrad_out)
+
+
+def fof_and_catalogue(part, rlink: float, boxsize: float = 0., cfg: FofConfig = FofConfig(),
+                      th: TreeHierarchy | None = None):
+    """Particles in FoF group order and the group catalogue (fof.py:872-907, one GPU).
+
+    part: positions (N, dim) or a particle dataclass (pos and optionally mass -- an array or a
+    constant --, vel, v_rad, zplus1); every per-particle field is reordered with the positions.
+    Returns (partf, catalogue): each group a contiguous run of partf starting at offset, groups with
+    at least cfg.catalogue.npart_min members catalogued."""
+    try:
+        partz, igroup = fof_labels(part, rlink=rlink, th=th, boxsize=boxsize, cfg=cfg)
+        partf, counts = _fof_order(igroup, partz)
+        cata = _fof_catalogue_from_groups(partf, counts, cfg.catalogue, boxsize=boxsize)
+        checks.flush()
+    except BaseException:
+        checks.clear()
+        raise
+    return partf, cata
+
+
+fof_and_catalogue.jit = fof_and_catalogue

@@ -20,7 +20,7 @@ from .checks import raise_if
 from .config import KNNConfig
 from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
 from .tools import inverse_of_splits, masked_inverse
-from .tree import _tmp, top_dense_interaction_list, zsort_and_tree, zsort_and_tree_origin
+from .tree import _tmp, f32_view, top_dense_interaction_list, zsort_and_tree, zsort_and_tree_origin
 
 
 @dataclass
@@ -195,62 +195,6 @@ def _knn_fused(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, al
     return rnn, inn, retry
 
 
-def _knn_group(th: TreeHierarchy, posz, k: int, boxsize: float | None = None, alloc_fac_ilist: float = 256.,
-               src_ids=None, dst_rows=None, out_rows: int | None = None, walk=None, init_r2=None, out=None):
-    """Leaf plane over query groups (extension, k <= 64; see csrc/knn_group.hip): the queries of
-    each plane-1 node go 64 at a time through one wave, over the node's plane-1 list.
-
-    walk = (plane-1 lists, plane-1 radii, plane-1 -> leaf splits); default: the guaranteed-radius
-    walk down to plane 1.  init_r2 (per plane-1 node) is the start bound, default the node's
-    guaranteed radius widened by 2^-10.  Without an explicit init_r2, queries left short by
-    rounding are redone without a bound, so the result equals knn()'s.  With one, returns the retry
-    flags as well (rows of flagged queries are incomplete)."""
-    assert k <= 64, "query groups need k <= 64"
-    dev = posz.device
-    n = len(posz)
-    size = th.size()
-    boxsize = 0. if boxsize is None else boxsize
-    if walk is None:
-        walk = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
-                              return_parents=True)
-    ilist_p, rad_p, n2c = walk
-    explicit = init_r2 is not None
-    if not explicit and rad_p is not None:
-        init_r2 = (rad_p.to(torch.float64) * (1.0 + 2.0 ** -10)).to(torch.float32)
-    # a one-plane tree has no guaranteed radii (its groups are the top-level super-nodes): no bound
-    init_r2 = None if init_r2 is None else init_r2.to(torch.float32).contiguous()
-    leaf_nodes = PosLvlNum(pos=th.geom_cent.get(0, size), lvl=th.lvl.get(0, size), npart=th.lvl.get(0, size)).pos_lvl()
-    l2p = th.splits_leaf_to_part().contiguous()
-    n2c = n2c.to(torch.int32).contiguous()
-    rows = n if out_rows is None else int(out_rows)
-    if out is not None:
-        rnn, inn, retry = out
-    else:
-        rnn = torch.empty((rows, k), dtype=posz.dtype, device=dev)
-        inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
-        retry = torch.empty(n, dtype=torch.int32, device=dev)
-    sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
-    drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
-
-    def launch(r2):
-        L.call("jzt_knn_group", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
-               ilist_p.size(), L.ptr(n2c), L.ptr(leaf_nodes), L.ptr(l2p), L.ptr(posz.contiguous()), L.ptr(sid),
-               L.ptr(drow), L.ptr(r2), L.ptr(retry), L.ptr(rnn), L.ptr(inn), size, rows, int(k), float(boxsize),
-               posz.shape[-1], L.dtype_code(posz.dtype))
-
-    launch(init_r2)
-    if explicit:
-        return rnn, inn, retry
-    if init_r2 is not None and bool((retry != 0).any()):
-        from .tools import inverse_of_splits
-        gspl = l2p[n2c.long().clamp(0, len(l2p) - 1)].contiguous()
-        short = torch.zeros(size + 1, dtype=torch.bool, device=dev)
-        short[inverse_of_splits(gspl, n).long().clamp(0, size)[retry != 0]] = True
-        launch(torch.where(short[:size], torch.full_like(init_r2, float("inf")),
-                           torch.full_like(init_r2, -1.0)).contiguous())
-    return rnn, inn
-
-
 def _guaranteed_leaf_r2(th: TreeHierarchy, k: int) -> torch.Tensor:
     """Per leaf slot, a radius^2 within which every query of the leaf has its k nearest (itself
     included): the squared cell diagonal of the smallest enclosing node (the leaf or an ancestor)
@@ -377,8 +321,22 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     ovf1 = run(r2, cap if direct_retry else None)
     # the fused retry pass only reads the flags; later rounds (direct_retry off) rewrite them
     first = retry if direct_retry else retry.clone()
-    # one read-back per round: (lists overflowed, queries left)
-    h = torch.stack([ovf1.to(torch.int64), first.sum().to(torch.int64)]).cpu()
+    if checks.is_deferred():
+        # under graph capture nothing is read back: an overflow of round 1's lists is reported after
+        # the replay (as the reference reports list overflows), and the fused retry pass always runs
+        # (it returns at once when no query is flagged)
+        assert direct_retry, "graph capture of the bounded search needs direct_retry"
+        raise_if(ovf1, "Interaction list allocation too small for the proposed radii\n"
+                       "Hint: increase alloc_fac_ilist")
+        h = None
+    else:
+        # one read-back per round: (lists overflowed, queries left)
+        h = torch.stack([ovf1.to(torch.int64), first.sum().to(torch.int64)]).cpu()
+    if h is None:
+        st["fallback"] = "fused retry (unconditional)"
+        plane1_lists, n2c = plane1[0]
+        _fused_retry(th, plane1_lists, n2c, posz, k, boxsize, src_ids, dst_rows, cap, retry, rnn, inn)
+        return rnn, inn, first
     st["short_round1"] = int(h[1])
     if bool(h[0]):
         st["fallback"] = "plain (round 1 lists overflowed)"
@@ -413,16 +371,21 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     else:
         ilist_p, _, n2c = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=alloc_fac_ilist, stop_level=1,
                                          return_parents=True)
+    _fused_retry(th, ilist_p, n2c, posz, k, boxsize, src_ids, dst_rows, cap, retry, rnn, inn)
+    return rnn, inn, first
+
+
+def _fused_retry(th: TreeHierarchy, ilist_p: InteractionList, n2c, posz, k: int, boxsize: float, src_ids, dst_rows,
+                 cap, retry, rnn, inn):
+    """The flagged queries (retry != 0) redone over the plane-1 lists (jzt_knn_fused_retry), each
+    starting from its leaf's guaranteed radius (every query has its k nearest within it)."""
     leaf_parent, leaf_nodes, l2p = _leaf_plane_args(th, n2c)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
-    # the flagged queries start from the guaranteed leaf radius (every query has its k nearest
-    # within it), not from an unbounded search
     L.call("jzt_knn_fused_retry", L.stream(), L.ptr(ilist_p.ispl), L.ptr(ilist_p.isrc), L.ptr(ilist_p.rad2),
            ilist_p.size(), L.ptr(n2c.contiguous()), L.ptr(leaf_parent), L.ptr(leaf_nodes), L.ptr(l2p),
            L.ptr(posz.contiguous()), L.ptr(sid), L.ptr(drow), L.ptr(cap.contiguous()), L.ptr(retry), L.ptr(rnn),
-           L.ptr(inn), size, n, int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
-    return rnn, inn, first
+           L.ptr(inn), th.size(), len(posz), int(k), float(boxsize), posz.shape[-1], L.dtype_code(posz.dtype))
 
 
 def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, leaf_r2, src_ids=None,
@@ -438,27 +401,41 @@ def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, le
     init = (leaf_r2.to(torch.float64) * (1.0 + 2.0 ** -10)).to(torch.float32).contiguous()
     rnn, inn, retry = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
                                      dst_rows=dst_rows, init_r2=init)
-    if bool((retry != 0).any()):
-        short = torch.zeros(size + 1, dtype=torch.bool, device=posz.device)
-        short[inverse_of_splits(spl, n).long()[retry != 0]] = True
-        redo = torch.where(short[:size], torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
+    # eagerly the redo runs only when a query came up short (one read-back); under graph capture
+    # it always runs, and every leaf without a short query skips it (negative radius)
+    if checks.is_deferred() or bool((retry != 0).any()):
+        short = torch.zeros(size + 1, dtype=torch.int32, device=posz.device)
+        short.scatter_reduce_(0, inverse_of_splits(spl, n).long().clamp(0, size), retry, "amax")
+        redo = torch.where(short[:size] != 0, torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
         _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
-                       dst_rows=dst_rows, init_r2=redo, out=(rnn, inn, retry))
+                       dst_rows=dst_rows, init_r2=redo.contiguous(), out=(rnn, inn, retry))
     return rnn, inn
+
+
+def _distance_view(th: TreeHierarchy, posz: torch.Tensor, dist_dtype):
+    """(tree, z-ordered positions) for the distance kernels: unchanged, or for float32 distances on
+    float64 positions the float32 node geometry (tree.f32_view) and the rounded positions."""
+    if dist_dtype is None or dist_dtype == posz.dtype:
+        return th, posz
+    if not (dist_dtype == torch.float32 and posz.dtype == torch.float64):
+        raise ValueError("dist_dtype: only float32 distances on float64 positions are supported")
+    return f32_view(th), posz.to(torch.float32).contiguous()
 
 
 def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = None, part_query=None,
         result: str = "rad_globalidx", reduce_func: Callable | None = None, output_order: str = "input",
-        cfg: KNNConfig = KNNConfig(), fused: bool | str = False):
+        cfg: KNNConfig = KNNConfig(), fused: bool = False, dist_dtype: torch.dtype | None = None):
     """Exact k nearest neighbours (knn.py:198-347).
 
     Returns (rnn, inn) by default: radii and indices of the k nearest neighbours of every particle,
     ascending by (distance, index).  result: "_"-separated keys among rad, drad, rankidx,
     globalidx, part, reduce, or a particle attribute name.  output_order: "input" or "z".
     fused (extension, k <= 64, same results): True = the fused leaf-plane kernel (one wave per query
-    leaf over its parent's list), "group" = query groups (the queries of each plane-1 node 64 at a
-    time, csrc/knn_group.hip: faster for small leaves); False runs the reference's KnnNode2Node +
-    KnnLeaf2Leaf decomposition (the default: fastest with the leaf sizes the autotuner picks).
+    leaf over its parent's list); False runs the reference's KnnNode2Node + KnnLeaf2Leaf
+    decomposition (the default: fastest with the leaf sizes the autotuner picks).
+    dist_dtype (extension): torch.float32 with float64 positions sorts and builds the tree in
+    float64 (63-bit keys) and searches the float32-rounded positions with the float32 kernels; the
+    result is the exact kNN of the rounded points (radii float32).
     """
     assert output_order in ("z", "input")
     if part_query is not None:
@@ -480,14 +457,12 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
 
         spl = th.splits_leaf_to_part(ptype=0)
         posz = get_pos(partz)
+        th, posz = _distance_view(th, posz, dist_dtype)
         keys = result.split("_")
         # rows can be written straight to input order when only radii/indices are requested
         direct = (output_order == "input" and set(keys) <= {"rad", "globalidx"}
                   and isinstance(num_origin, int) and num_origin == size)
-        if fused == "group" and k <= 64:
-            rnnz, inn_g = _knn_group(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
-                                     src_ids=originz.idx, dst_rows=originz.idx if direct else None)
-        elif fused and k <= 64:
+        if fused and k <= 64:
             # neighbours are reported (and tie-broken) by their origin index directly
             rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                         src_ids=originz.idx, dst_rows=originz.idx if direct else None)

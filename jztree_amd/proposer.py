@@ -227,11 +227,13 @@ def training_data(points: list, k: int, cfg: KNNConfig = KNNConfig(), boxsize: f
 
 
 def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = None,
-                cfg: KNNConfig = KNNConfig(), output_order: str = "input", return_retry: bool = False):
+                cfg: KNNConfig = KNNConfig(), output_order: str = "input", return_retry: bool = False,
+                dist_dtype: torch.dtype | None = None):
     """Exact kNN (radii, global indices) with the proposer's starting bound; identical to
     knn(part, k, boxsize, output_order=...).  With return_retry, also the per-query flags of the
-    queries that the first (proposed-radius) round could not finish (z order)."""
-    from .knn import _knn_bounded
+    queries that the first (proposed-radius) round could not finish (z order).  dist_dtype as in
+    knn(): float32 distances on float64 positions."""
+    from .knn import _distance_view, _knn_bounded
     from .tree import zsort_and_tree_origin
     assert output_order in ("z", "input")
     assert k == proposer.k, "the proposer was trained for another k"
@@ -240,12 +242,13 @@ def knn_learned(part, k: int, proposer: RadiusProposer, boxsize: float | None = 
     size = len(pos)
     try:
         partz, originz, th = zsort_and_tree_origin(part, cfg.tree)
+        th, posz = _distance_view(th, get_pos(partz), dist_dtype)
         bound = proposer.bound_r2(th)
         if output_order == "input":  # rows and reported ids in input order, as knn()
             sid, dst = originz.idx, originz.idx
         else:  # z order: rows and ids are z-indices
             sid, dst = None, None
-        rnn, inn, retry = _knn_bounded(th, get_pos(partz), k, bound, boxsize=boxsize,
+        rnn, inn, retry = _knn_bounded(th, posz, k, bound, boxsize=boxsize,
                                        alloc_fac_ilist=cfg.alloc_fac_ilist, src_ids=sid, dst_rows=dst)
         checks.flush()
     except BaseException:

@@ -7,6 +7,7 @@ outputs follow the reference; each docstring cites the lines it mirrors.
 """
 from __future__ import annotations
 
+import dataclasses
 import math
 from typing import Any, Tuple
 
@@ -444,6 +445,27 @@ def zsort_and_tree_origin(part, cfg_tree: TreeConfig = TreeConfig()):
     partz, isort = zsort(part)
     th = build_tree_hierarchy(partz, cfg_tree)
     return partz, RankIdx(rank=None, idx=isort.to(torch.int32)), th
+
+
+def f32_view(th: TreeHierarchy) -> TreeHierarchy:
+    """The same hierarchy (splits, planes) with float32 node geometry for float32 distance kernels
+    on the float32-rounded particles (extension: float64 keys, float32 distances).  Each node's
+    Morton cell is widened to the smallest enclosing cell whose boundaries are float32 numbers
+    (jzt_geometry_to_f32), which contains the rounded particles of the node, so every pruning
+    decision of the float32 node kernels stays exact for the rounded points.  A float32 tree is
+    returned unchanged."""
+    if th.geom_cent.data.dtype == torch.float32:
+        return th
+    g = th.geom_cent.data.to(torch.float64).contiguous()
+    lv = th.lvl.data.to(torch.int32).contiguous()
+    n, dim = g.shape
+    cent = torch.empty((n, dim), dtype=torch.float32, device=g.device)
+    lvl = torch.empty(n, dtype=torch.int32, device=g.device)
+    L.call("jzt_geometry_to_f32", L.stream(), L.ptr(g), L.ptr(lv), n, dim, -2000, L.ptr(cent), L.ptr(lvl))
+    fill = th.geom_cent.fill_values
+    geom = dataclasses.replace(th.geom_cent, data=cent, fill_values=None if fill is None else fill.float())
+    lvlp = dataclasses.replace(th.lvl, data=lvl.to(th.lvl.data.dtype))
+    return dataclasses.replace(th, geom_cent=geom, lvl=lvlp)
 
 
 # ------------------------------------------------------------------------------------------------

@@ -465,3 +465,87 @@ def canonical_labels(labels):
     np.minimum.at(mins, inv, np.arange(n, dtype=np.int64))
     del mn
     return mins[inv]
+
+
+# ------------------------------------------------------------------------------------------------
+# FoF catalogue (fof.py:510-519 _fof_order, :650-766 _fof_catalogue_from_groups; one device)
+# ------------------------------------------------------------------------------------------------
+
+def fof_order(igroup, npart=None):
+    """fof.py:510-519: (isort, counts in group order).  igr = igroup for i < npart, else n; isort is
+    the stable argsort of igr; counts[i] = number of filled particles labelled isort[i]."""
+    igroup = np.asarray(igroup, np.int64)
+    n = len(igroup)
+    npart = n if npart is None else int(npart)
+    igr = np.where(np.arange(n) < npart, igroup, n)
+    counts = np.zeros(n + 1, np.int64)
+    np.add.at(counts, igr, 1)
+    counts = counts[:n]  # the scatter at index n is dropped, as jnp's .at[].add drops it
+    isort = np.argsort(igr, kind="stable")
+    return isort, counts[isort]
+
+
+def _py_mod(x, y):
+    """jnp.remainder: fmod, then + y where the remainder's sign differs from y's."""
+    r = np.fmod(x, y)
+    fix = (r != 0) & ((r < 0) != (y < 0))
+    return np.where(fix, r + y, r).astype(x.dtype)
+
+
+def fof_catalogue(pos, counts, npart_min, boxsize=0.0, mass=None, vel=None, v_rad=None, zplus1=None,
+                  size_cata=None):
+    """fof.py:650-766 for one device, on particles in group order: a dict of catalogue arrays.
+
+    Products are formed in the particle type as the reference forms them ((pos - pos0) wrapped,
+    times mass), sums in float64.  Groups: every i with counts[i] >= npart_min, in order.  A 2D
+    inertia radius counts the second component twice: the reference indexes dx[..., 2], which JAX
+    clamps to the last column."""
+    pos = np.asarray(pos)
+    T = pos.dtype.type
+    n, dim = pos.shape
+    counts = np.asarray(counts, np.int64)
+    if size_cata is None:
+        size_cata = (n + npart_min - 1) // npart_min
+    box = T(boxsize or 0.0)
+    wrap_dx = (lambda d: _py_mod(d + T(0.5) * box, box) - T(0.5) * box) if box > 0 else (lambda d: d)
+    wrap_pos = (lambda x: _py_mod(x, box)) if box > 0 else (lambda x: x)
+    starts = np.nonzero(counts >= npart_min)[0]
+    ng = len(starts)
+    k = min(ng, size_cata)
+    out = {"ngroups": ng}
+    count = np.zeros(size_cata, np.int64)
+    offset = np.full(size_cata, n, np.int64)
+    count[:k] = counts[starts[:k]]
+    offset[:k] = starts[:k]
+    out["count"], out["offset"] = count, offset
+    m_all = None if mass is None else np.broadcast_to(np.asarray(mass, pos.dtype), (n,))
+    mass_array = mass is not None and np.size(mass) > 1
+    nan = np.full(size_cata, np.nan, pos.dtype)
+    gm = np.zeros(size_cata, pos.dtype)
+    com = np.full((size_cata, dim), np.nan, pos.dtype)
+    cvel = np.full((size_cata, dim), np.nan, pos.dtype)
+    vr, sf, rad = nan.copy(), nan.copy(), nan.copy()
+    for g in range(k):
+        s, c = int(offset[g]), int(count[g])
+        sl = slice(s, s + c)
+        m = m_all[sl] if m_all is not None else np.ones(c, pos.dtype)
+        gm[g] = T(np.sum(m.astype(np.float64))) if mass_array else T(count[g]) * (T(np.asarray(mass).reshape(-1)[0])
+                                                                                    if mass is not None else T(1))
+        dx = wrap_dx(pos[sl] - pos[s])
+        com[g] = wrap_pos(np.sum((dx * m[:, None]).astype(np.float64), 0).astype(pos.dtype) / gm[g] + pos[s])
+        if vel is not None:
+            v = np.asarray(vel)[sl]
+            cvel[g] = np.sum(((v - v[0]) * m[:, None]).astype(np.float64), 0).astype(pos.dtype) / gm[g] + v[0]
+        if v_rad is not None:
+            vr[g] = T(np.sum((np.asarray(v_rad)[sl] * m).astype(np.float64))) / gm[g]
+        if zplus1 is not None:
+            sf[g] = T(np.sum(((T(1) / np.asarray(zplus1)[sl]) * m).astype(np.float64))) / gm[g]
+        d = wrap_dx(pos[sl] - com[g])
+        r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, min(2, dim - 1)] * d[:, min(2, dim - 1)]
+        rad[g] = np.sqrt(T(np.sum((r2 * m).astype(np.float64))) / gm[g])
+    out["mass"] = gm if mass is not None else None
+    out["com_pos"], out["com_inertia_radius"] = com, rad
+    out["com_vel"] = cvel if vel is not None else None
+    out["v_rad"] = vr if v_rad is not None else None
+    out["scale_factor"] = sf if zplus1 is not None else None
+    return out

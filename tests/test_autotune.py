@@ -12,17 +12,19 @@ from gen import uniform
 def test_knobs_roundtrip_and_validation():
     import jztree_amd as jz
     before = jz.autotune.get_knobs()
-    assert before["leaf_waves"] in (1, 2, 4) and before["sort_kpt"] in (8, 12, 16)
+    assert before["leaf_waves"] in (1, 2, 4) and before["sort_kpt"] in (8, 12, 16) and before["xcd"] in (0, 1)
     try:
-        jz.autotune.set_knobs(leaf_waves=2, sort_kpt=16)
-        assert jz.autotune.get_knobs() == {"leaf_waves": 2, "sort_kpt": 16}
+        jz.autotune.set_knobs(leaf_waves=2, sort_kpt=16, xcd=0)
+        assert jz.autotune.get_knobs() == {"leaf_waves": 2, "sort_kpt": 16, "xcd": 0}
         with pytest.raises(RuntimeError):
             jz.autotune.set_knobs(leaf_waves=3)
         with pytest.raises(RuntimeError):
             jz.autotune.set_knobs(sort_kpt=10)
+        with pytest.raises(RuntimeError):
+            jz.autotune.set_knobs(xcd=2)
         with pytest.raises(ValueError):
             jz.autotune.set_knobs(tile=4)
-        assert jz.autotune.get_knobs() == {"leaf_waves": 2, "sort_kpt": 16}
+        assert jz.autotune.get_knobs() == {"leaf_waves": 2, "sort_kpt": 16, "xcd": 0}
     finally:
         jz.autotune.set_knobs(**before)
 
@@ -72,7 +74,8 @@ def test_signature_and_cache(tmp_path):
     try:
         got = jz.autotune._tune("knn", (1 << 26) + 5, 32, torch.float32, 3, 1.0, 1, None, path, 1, (12,), (32,), (4,))
         assert got.cached and (got.leaf_size, got.leaf_waves, got.sort_kpt) == (48, 2, 8)
-        assert jz.autotune.get_knobs() == {"leaf_waves": 2, "sort_kpt": 8}
+        got_knobs = jz.autotune.get_knobs()
+        assert (got_knobs["leaf_waves"], got_knobs["sort_kpt"]) == (2, 8)
     finally:
         jz.autotune.set_knobs(**before)
 
@@ -88,7 +91,8 @@ def test_tune_knn_and_batch_on_gpu(dev, tmp_path):
                                         sort_kpts=(8, 16), reps=1, cache=str(tmp_path / "t.json"))
         print(f"[autotune] chose {res}", flush=True)
         assert cfg.tree.max_leaf_size == res.leaf_size and len(res.table) >= 5
-        assert jz.autotune.get_knobs() == {"leaf_waves": res.leaf_waves, "sort_kpt": res.sort_kpt}
+        kn = jz.autotune.get_knobs()
+        assert (kn["leaf_waves"], kn["sort_kpt"]) == (res.leaf_waves, res.sort_kpt)
         r1, i1 = jz.knn.knn(x, 16, boxsize=1.0, cfg=cfg)
         jz.autotune.set_knobs(**before)
         r0, i0 = jz.knn.knn(x, 16, boxsize=1.0)

@@ -118,35 +118,9 @@ def test_fused_equals_reference_decomposition(dev, oracle, case):
     cfg = _clustered_cfg()
     r1, i1 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=True)
     r2_, i2 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused=False)
-    r3, i3 = jz.knn.knn(t, k=k, boxsize=box, cfg=cfg, fused="group")
     torch.cuda.synchronize()
     assert torch.equal(i1, i2) and torch.equal(r1, r2_)
-    assert torch.equal(i3, i2) and torch.equal(r3, r2_), "query groups differ"
     _check(x, k, r1.cpu().numpy(), i1.cpu().numpy(), oracle, boxsize=box)
-
-
-def test_group_bound_and_retry(dev):
-    """Query groups with a start bound too small for some queries flag them; the default call redoes
-    them and equals the reference decomposition."""
-    import jztree_amd as jz
-    x = clustered(1 << 19, nhalos=256, seed=24)  # two planes: the plane-1 nodes have guaranteed radii
-    t = torch.from_numpy(x).to(dev)
-    cfg = _clustered_cfg()
-    partz, originz, th = jz.tree.zsort_and_tree(t, cfg.tree, data=jz.data.RankIdx(
-        rank=None, idx=torch.arange(len(x), dtype=torch.int32, device=dev)))
-    walk = jz.knn._knn_dual_walk(th, 16, boxsize=1.0, alloc_fac_ilist=cfg.alloc_fac_ilist, stop_level=1,
-                                 return_parents=True)
-    print("[group] bound x0.3", flush=True)
-    r1, i1, retry = jz.knn._knn_group(th, partz, 16, boxsize=1.0, src_ids=originz.idx, walk=walk,
-                                      init_r2=walk[1] * 0.3)
-    torch.cuda.synchronize()
-    assert int(retry.sum()) > 0
-    rz, iz = jz.knn._knn_leaf2leaf(jz.knn._knn_dual_walk(th, 16, boxsize=1.0, alloc_fac_ilist=cfg.alloc_fac_ilist),
-                                   th.splits_leaf_to_part(), partz, k=16, boxsize=1.0, src_ids=originz.idx)
-    ok = retry == 0
-    assert torch.equal(i1[ok], iz[ok]) and torch.equal(r1[ok], rz[ok]), "unflagged rows must be exact"
-    r2_, i2 = jz.knn._knn_group(th, partz, 16, boxsize=1.0, src_ids=originz.idx, walk=walk)
-    assert torch.equal(i2, iz) and torch.equal(r2_, rz)
 
 
 def test_fused_bound_and_retry(dev, oracle):
@@ -268,14 +242,17 @@ def _cell_diag2_ref(lvl, dim):
 def test_guaranteed_leaf_r2_bounds_true_kth(dev, oracle, k, boxsize):
     """The guaranteed leaf radius^2 (widened by 2^-10, as the bounded search and its retry pass use
     it) is at least the true k-th neighbour r2 of every query in the leaf: the oracle's canonical
-    FP32 r2 of all 2^16 queries, clustered points (dense halos: small cells)."""
+    FP32 r2 of all 2^19 queries, clustered points (dense halos: small cells).  A leaf whose top-plane
+    ancestor holds fewer than k particles has no bound from the tree (+inf); with 2^19 points the
+    tree has two planes and nearly every leaf gets a finite bound, so the check is not vacuous."""
     import jztree_amd as jz
     from jztree_amd.knn import _guaranteed_leaf_r2
     from jztree_amd.tools import inverse_of_splits
-    x = clustered(1 << 16, nhalos=64, seed=29)
+    x = clustered(1 << 19, nhalos=256, seed=29)
     t = torch.from_numpy(x).to(dev)
     cfg = _clustered_cfg()
     partz, th = jz.tree.zsort_and_tree(t, cfg.tree)
+    assert th.num_planes() >= 2
     cap = (_guaranteed_leaf_r2(th, k) * (1.0 + 2.0 ** -10)).to(torch.float32)
     nl = int(th.num(0))
     leafof = inverse_of_splits(th.splits_leaf_to_part()[: nl + 1].contiguous(), len(x)).long()
@@ -283,7 +260,9 @@ def test_guaranteed_leaf_r2_bounds_true_kth(dev, oracle, k, boxsize):
     r2k = torch.from_numpy(r2o[:, k - 1]).to(dev)
     bad = int((r2k > cap[leafof]).sum())
     assert bad == 0, f"{bad} queries have their k-th neighbour beyond the guaranteed radius"
-    assert bool(torch.isfinite(cap[:nl]).all())
+    finite = float(torch.isfinite(cap[:nl]).float().mean())
+    print(f"[cap] k={k} box={boxsize}: {finite:.4f} of {nl} leaves have a finite bound", flush=True)
+    assert finite > 0.9
 
 
 @pytest.mark.parametrize("n,leaf,k", [(1 << 14, 32, 32), (1 << 17, 64, 32), (1 << 17, 16, 40)])

@@ -128,6 +128,42 @@ def test_leaves_are_maximal_cells(oracle):
         assert hi - lo + 1 > leaf
 
 
+def test_nodes_are_maximal_cells(oracle):
+    """Every node of plane p holds <= leaf_size * coarse_fac^p particles, and a leaf boundary is a
+    node boundary of plane p exactly when the binary Morton cell it splits holds more than that
+    (cell counts by brute force from pairwise levels, independent of FindNodeBoundaries)."""
+    x = clustered(6000, nhalos=24, seed=13)
+    pz, _ = oracle.zsort(x)
+    th = oracle.build_tree_hierarchy(pz, 8, 4.0, 16, 2.0)
+    assert th.num_planes() >= 3
+    sizes = oracle.tree_level_node_sizes(len(pz), 8, 4.0, 16)
+    leaf_spl = th.ispl_l2p_per_type[0]
+    nleaves = int(np.argmax(th.ispl_n2l.get(0)))
+    leaf_spl = leaf_spl[:nleaves + 1]
+
+    def cell_count(b):
+        L = oracle.msb_diff_level(pz[b - 1], pz[b])
+        lo = b - 1
+        while lo - 1 >= 0 and oracle.msb_diff_level(pz[lo - 1], pz[b]) <= L:
+            lo -= 1
+        hi = b
+        while hi + 1 < len(pz) and oracle.msb_diff_level(pz[hi + 1], pz[b]) <= L:
+            hi += 1
+        return hi - lo + 1
+
+    counts = {int(b): cell_count(int(b)) for b in leaf_spl[1:-1]}
+    for p in range(1, th.num_planes()):
+        n2l = th.ispl_n2l.get(p)
+        nn = int(np.argmax(n2l))
+        node_leaf_spl = n2l[:nn + 1]
+        node_part_spl = leaf_spl[node_leaf_spl]
+        assert node_part_spl[0] == 0 and node_part_spl[-1] == len(pz)
+        assert np.all(np.diff(node_part_spl) <= sizes[p]), f"plane {p}: a node holds too many particles"
+        is_node_b = set(int(b) for b in node_part_spl[1:-1])
+        for b, c in counts.items():
+            assert (b in is_node_b) == (c > sizes[p]), f"plane {p}: boundary {b} (cell of {c})"
+
+
 def test_tree_hierarchy_invariants(oracle):
     """checks/tests/test_tree.py:11-29 and :31-50 on the oracle's hierarchy."""
     x = uniform(1 << 15, seed=4)
