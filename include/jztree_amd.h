@@ -72,6 +72,17 @@ size_t jzt_zsort_tmp_bytes(int64_t n, int dim, int dtype);
 int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out, void* tmp,
                         size_t tmp_bytes, int64_t n, int dim, int dtype);
 
+/* PosZorderSort without a workspace -- no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): same inputs, outputs and order as jzt_pos_zorder_sort, but the scratch is a
+ * fixed 64 MiB device array inside the library (one per device, held until the process ends), so
+ * nothing is allocated and no tmp buffer is passed.  n is limited to what fits that array
+ * (jzt_zsort_tmp_bytes(n, dim, dtype) <= 64 MiB: up to 1,427,456 points for f32 3D,
+ * 1,137,141 for f64 3D); larger n fails
+ * with a message.  Calls that overlap on one device share the array and race: serialise them. */
+int jzt_pos_zorder_sort_noalloc(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out, int64_t n,
+                                int dim, int dtype);
+
 /* Batched PosZorderSort -- no counterpart in the reference, whose batching is vmap with
  * vmap_method="sequential" (tree.py:88): one set at a time.  The n points hold nsets independent
  * sets, set s in rows [set_offsets[s], set_offsets[s+1]) (device int64, nsets+1 entries,

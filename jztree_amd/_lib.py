@@ -30,6 +30,7 @@ _SIGNATURES = {
     "jzt_get_tuning": ([_i32], _i32),
     "jzt_zsort_tmp_bytes": ([_i64, _i32, _i32], _sz),
     "jzt_pos_zorder_sort": ([_p, _p, _p, _p, _p, _sz, _i64, _i32, _i32], _i32),
+    "jzt_pos_zorder_sort_noalloc": ([_p, _p, _p, _p, _i64, _i32, _i32], _i32),
     "jzt_zsort_segmented_tmp_bytes": ([_i64, _i64, _i32, _i32], _sz),
     "jzt_pos_zorder_sort_segmented": ([_p, _p, _p, _p, _p, _i64, _p, _sz, _i64, _i32, _i32], _i32),
     "jzt_segment_sort": ([_p, _p, _p, _p, _p, _p, _i64, _i64], _i32),

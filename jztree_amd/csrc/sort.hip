@@ -22,6 +22,8 @@
 // of the reference removed (exactly equal coordinates get level -inf instead of e-24, and -0.0 is
 // treated as +0.0); on every pair where the reference's comparator is a strict weak order the two
 // orders agree.
+#include <mutex>
+
 #include "util.h"
 #include "../../include/jztree_amd.h"
 
@@ -970,6 +972,65 @@ extern "C" int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void
     if (dim == 2) return zsort_impl<2, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
   }
   return fail("jzt_pos_zorder_sort: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
+}
+
+// ---------------------------------------------------------------------------------------------
+// PosZorderSort without a caller workspace: the sort's scratch lives in a fixed device array.
+// ---------------------------------------------------------------------------------------------
+// Each device that the library runs on holds its own copy of `scratch` until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is only a stand-in for the
+// symbol); it uses the device address that hipGetSymbolAddress returns for the current device,
+// looked up once per device on its first call and kept.  Calls that overlap on one device share
+// the array and race; every region the sort needs zeroed is cleared by the sort itself.
+namespace jzt {
+namespace {
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+}  // namespace jzt
+
+static __device__ __align__(256) char scratch[jzt::kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[jzt::kMaxDevices] = {};
+
+// device address of this device's scratch array (nullptr after a failure, reported with fail())
+void* scratch_for_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= jzt::kMaxDevices) {
+    jzt::fail("jzt_pos_zorder_sort_noalloc: no current HIP device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      jzt::fail(std::string("jzt_pos_zorder_sort_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int jzt_pos_zorder_sort_noalloc(jzt_stream_t stream, const void* pos_in, void* pos_out, void* id_out,
+                                           int64_t n, int dim, int dtype) {
+  if (n < 0) return fail("jzt_pos_zorder_sort_noalloc: negative n");
+  if (n == 0) return 0;
+  if ((dim != 2 && dim != 3) || (dtype != JZT_F32 && dtype != JZT_F64))
+    return fail("jzt_pos_zorder_sort_noalloc: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
+  if (n >= (int64_t)kCountMask) return fail("jzt_pos_zorder_sort_noalloc: n must be < 2^30");
+  const size_t need = sort_layout(n, dim, dtype).total;
+  if (need > kNoAllocScratchBytes)
+    return fail("jzt_pos_zorder_sort_noalloc: " + std::to_string(n) + " points need " + std::to_string(need) +
+                " bytes of scratch, more than the fixed " + std::to_string(kNoAllocScratchBytes) +
+                " bytes (use jzt_pos_zorder_sort with a caller workspace)");
+  void* tmp = scratch_for_current_device();
+  if (!tmp) return 1;
+  return jzt_pos_zorder_sort(stream, pos_in, pos_out, id_out, tmp, need, n, dim, dtype);
 }
 
 extern "C" size_t jzt_zsort_segmented_tmp_bytes(int64_t n, int64_t nsets, int dim, int dtype) {
