@@ -15,7 +15,8 @@ plain kNN radii of 4 other seeds of the same configuration (generated on the hos
 their margins are calibrated on those seeds.
 
 Also reported: the same points with k=64 (learned and plain), 2^24 clustered points with 4096
-halos (FoF and plain kNN k=32), and the batched workload (256 uniform sets, kNN k=8 + FoF).
+halos (FoF and plain kNN k=32), 2^28 points with float64 positions searched in float32 (learned and
+plain kNN, FoF), and the batched workload (256 uniform sets, kNN k=8 + FoF).
 
 Prints one JSON line.  value = learned kNN queries/s (k=32); roofline = the leaf kernel of the
 learned search (the step's dominant kernel) timed with HIP events on the launch stream; fof_roofline
@@ -68,6 +69,8 @@ def parse():
     p.add_argument("--quantile", type=float, default=0.999)
     p.add_argument("--n-mid", type=int, default=1 << 24, help="points of the 4096-halo set (0 = skip)")
     p.add_argument("--halos-mid", type=int, default=4096)
+    p.add_argument("--c5-n", type=int, default=1 << 28,
+                   help="points of the float64-key / float32-distance leg (0 = skip)")
     p.add_argument("--cpu-side", type=float, default=0.5,
                    help="CPU baselines run on the points in [0, side)^3 of the benchmark set")
     p.add_argument("--no-cpu", action="store_true")
@@ -278,6 +281,66 @@ def bench_batch(jz, torch, dev, nsets, k, steps, warmup, timer):
     }
 
 
+def bench_c5(jz, torch, dev, args, timer, kcfg):
+    """2^28 clustered points (60% in Plummer halos, member weights on [32, 2^18]), float64
+    positions: z-order keys and tree in float64, kNN (k, learned) and FoF on the float32-rounded
+    positions.  Points drawn on the GPU; the proposer is trained on a 2^26 float32 set of the same
+    mix.  Learned and plain kNN are compared through exact checksums of their outputs."""
+    from gen import clustered_torch
+    n, k = args.c5_n, args.k
+    nh = int(0.6 * n / 421)
+    x = clustered_torch(n, nhalos=nh, halo_frac=0.6, mmax=2 ** 18, seed=11, device=dev)
+    xs = clustered_torch(1 << 26, nhalos=max(1, nh * (1 << 26) // n), halo_frac=0.6, mmax=2 ** 18, seed=12,
+                         device=dev, dtype=torch.float32)
+    ccfg = dataclasses.replace(kcfg, alloc_fac_ilist=256.0)
+    f, t = jz.proposer.training_data([xs], k, ccfg, boxsize=1.0)
+    del xs
+    prop = jz.proposer.RadiusProposer(k, device=dev, seed=0)
+    prop.train(f, t, steps=1000, batch=8192, lr=3e-3, lr_final=1e-4)
+    prop.calibrate(f, t, quantile=args.quantile)
+    del f, t
+    torch.cuda.empty_cache()
+    f32 = torch.float32
+    box = 1.0
+
+    def digest(out):
+        r, i = out[0], out[1]
+        ci = cr = 0
+        for s0 in range(0, n, 1 << 22):
+            w = (torch.arange(s0, min(n, s0 + (1 << 22)), device=dev, dtype=torch.int64) % 1009 + 1)[:, None]
+            ci += int((i[s0:s0 + (1 << 22)].to(torch.int64) * w).sum())
+            cr += int((r[s0:s0 + (1 << 22)].view(torch.int32).to(torch.int64) * w).sum())
+        return ci, cr
+
+    res = {}
+    for name, fn in (("learned", lambda: jz.proposer.knn_learned(x, k, prop, boxsize=box, cfg=ccfg, dist_dtype=f32)),
+                     ("plain", lambda: jz.knn.knn(x, k, boxsize=box, cfg=ccfg, dist_dtype=f32))):
+        g = jz.graph.Captured(fn)
+        ms = timer(g.run, max(2, args.steps // 2), 1)
+        g.check()
+        res[name] = (ms, digest(g.out), dict(g.node_counts))
+        del g
+        torch.cuda.empty_cache()
+        log(f"c5 {name} kNN: {ms:.1f} ms")
+    rlink = args.b * n ** (-1.0 / 3.0)
+    fcfg = jz.config.FofConfig()
+    fcfg = dataclasses.replace(fcfg, tree=dataclasses.replace(fcfg.tree, alloc_fac_nodes=2.0))
+    g = jz.graph.Captured(lambda: jz.fof.fof_labels(x, rlink, boxsize=box, cfg=fcfg, dist_dtype=f32))
+    t_fof = timer(g.run, max(2, args.steps // 2), 1)
+    g.check()
+    del g
+    torch.cuda.empty_cache()
+    t_l, t_p = res["learned"][0], res["plain"][0]
+    return {"workload": f"clustered {n} points ({nh} halos, 60% in halos, member weights on [32, 2^18]), periodic "
+                        f"unit box, float64 positions; float64 keys and tree, float32 distances; kNN k={k} learned "
+                        f"and plain, FoF b={args.b}",
+            "n_points": n, "k": k, "knn_learned_ms": t_l, "knn_plain_ms": t_p, "learned_speedup": t_p / t_l,
+            "learned_queries_per_s": n / (t_l * 1e-3), "learned_identical": res["learned"][1] == res["plain"][1],
+            "fof_ms": t_fof, "fof_mparticles_per_s": n / (t_fof * 1e-3) / 1e6,
+            "fof_roofline_frac": n * (FOF_BYTES_END_TO_END + 12) / (t_fof * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "graph_nodes_learned": res["learned"][2]}
+
+
 def cpu_baselines(xh, k, b, side):
     """The points of the benchmark set inside [0, side)^3 (same density and clustering, non-periodic),
     timed on this host's cores: scipy cKDTree build + k-nearest query of every one of them
@@ -440,6 +503,13 @@ def main():
         torch.cuda.empty_cache()
         log(f"mid: {mid}")
 
+    c5 = None
+    if args.c5_n:
+        del x
+        torch.cuda.empty_cache()
+        c5 = bench_c5(jz, torch, dev, args, timer, kcfg)
+        log(f"c5: {c5}")
+
     batched = None
     if args.batch_sets > 0:
         batched = bench_batch(jz, torch, dev, args.batch_sets, args.batch_k, args.steps, args.warmup, timer)
@@ -495,6 +565,7 @@ def main():
         "cpu_baseline": cpu,
         "fof_cpu_baseline": fof_cpu,
         "mid": mid,
+        "c5": c5,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -94,6 +94,13 @@ int jzt_pos_zorder_sort_segmented(jzt_stream_t stream, const void* pos_in, void*
                                   const int64_t* set_offsets, int64_t nsets, void* tmp, size_t tmp_bytes,
                                   int64_t n, int dim, int dtype);
 
+/* SearchSortedZ -- replaces ffi target "SearchSortedZ" (kernel sort.cuh:82-132, caller tree.py:143-158).
+ * indices[i] = left insertion point of posz_query[i] into the z-sorted posz_have (have[idx-1] <
+ * x <= have[idx] in the z-order of jzt_pos_zorder_sort).  leaf_search != 0: posz_have holds one
+ * point per leaf; the index of the neighbouring leaf with the smaller level difference to x. */
+int jzt_search_sorted_z(jzt_stream_t stream, const void* posz_have, const void* posz_query, int64_t n_have,
+                        int64_t n_query, int dim, int dtype, int leaf_search, int32_t* indices);
+
 /* SegmentSort -- replaces ffi target "SegmentSort" (generated/ffi_knn.cu:412-423, knn.cuh:700-722,
  * caller knn.py:112-127).  Sorts (key, val) within segments spl[i]..spl[i+1] by (key, val). */
 int jzt_segment_sort(jzt_stream_t stream, const int32_t* spl, const float* key, const int32_t* val,

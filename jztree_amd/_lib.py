@@ -34,6 +34,7 @@ _SIGNATURES = {
     "jzt_zsort_segmented_tmp_bytes": ([_i64, _i64, _i32, _i32], _sz),
     "jzt_pos_zorder_sort_segmented": ([_p, _p, _p, _p, _p, _i64, _p, _sz, _i64, _i32, _i32], _i32),
     "jzt_segment_sort": ([_p, _p, _p, _p, _p, _p, _i64, _i64], _i32),
+    "jzt_search_sorted_z": ([_p, _p, _p, _i64, _i64, _i32, _i32, _i32, _p], _i32),
     "jzt_flag_leaf_boundaries": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32, _i32], _i32),
     "jzt_find_node_boundaries": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
     "jzt_flag_leaf_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
@@ -130,4 +131,7 @@ def dtype_code(dt: torch.dtype) -> int:
 
 
 def call(name: str, *args):
+    # only bound entry points: an unbound ctypes function would pass pointers as 32-bit ints
+    if name not in _SIGNATURES:
+        raise RuntimeError(f"{name} is not bound in jztree_amd._lib")
     check(getattr(LIB, name)(*args), name)

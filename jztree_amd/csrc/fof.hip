@@ -220,6 +220,94 @@ template <typename T> __device__ __forceinline__ T wave_max_t(T v) {
   return v;
 }
 
+// Particle links inside each 64-particle chunk of a leaf, resolved before the leaf pairs (the
+// first pass of FofLeaf2Leaf).  Per chunk: the within-rlink graph of its <= 64 particles (the
+// same canonical r2 and test as every other pair), its connected components by bitmask closure,
+// and then every particle that labels itself is pointed at the first particle of its component
+// (a plain store: nothing else points at it, and only this wave writes it), while a particle
+// carrying a label from the node levels is joined with that first particle by one union-find link
+// per distinct (label, component) pair.  Afterwards the members of a chunk component share one
+// parent, so the leaf-pair pass sees equal labels for them and links each pair of sets once.
+template <bool PER, int D, typename T>
+__global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ spl, const T* __restrict__ pos,
+                                                   int32_t* ig, float r2link_all, const float* __restrict__ r2v,
+                                                   T boxsize, int64_t nitems, int xcd) {
+  __shared__ FofRec<D, T> s_r[64];
+  const int lane = threadIdx.x;
+  const ItemRange items = xcd_items(active_items(spl, nitems), xcd != 0);
+  for (int64_t lq = items.first; lq < items.end; lq += items.step) {
+    const int q0 = spl[lq], q1 = spl[lq + 1];
+    const T half = (T)(float)(T(0.5) * boxsize);
+    const float r2link = r2v ? r2v[lq] : r2link_all;
+    for (int qoff = q0; qoff < q1; qoff += 64) {
+      const int n = min(64, q1 - qoff);
+      const int iq = min(qoff + lane, q1 - 1);
+      const bool ok = lane < n;
+      const Vec<D, T> x = load_vec<D, T>(pos, iq);
+      const int g = ig[iq];
+      // one label on the whole chunk (a leaf linked on the node levels, or a single particle):
+      // its particles are in one set already
+      const int g0 = __shfl(g, 0, 64);
+      if (__ballot(ok && g != g0) == 0ull) continue;
+      bool need_wrap = false;
+      Vec<D, T> lo, hi;
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        lo[d] = wave_min_t<T>(x[d]);
+        hi[d] = wave_max_t<T>(x[d]);
+      }
+      if (ok) {
+#pragma unroll
+        for (int d = 0; d < D; d++)
+          if (PER) need_wrap |= !(lo[d] - x[d] >= -half) || !(hi[d] - x[d] < half);
+        FofRec<D, T> r;
+#pragma unroll
+        for (int d = 0; d < D; d++) r.p[d] = x[d];
+        r.g = g;
+        s_r[lane] = r;
+      }
+      const bool wrap = PER && wave_any(need_wrap);
+      __syncthreads();
+      uint64_t nb = 0ull;
+      if (ok) {
+        for (int j = 0; j < n; j++) {
+          Vec<D, T> xp;
+#pragma unroll
+          for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
+          const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
+          if (j != lane && r2 <= (T)r2link) nb |= 1ull << j;
+        }
+      }
+      __syncthreads();
+      uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
+      bool changed = true;
+      while (wave_any(changed)) {
+        uint64_t cn = c;
+        for (int j = 0; j < n; j++) {
+          const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
+                              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+          if ((c >> j) & 1ull) cn |= cj;
+        }
+        changed = cn != c;
+        c = cn;
+      }
+      const int lroot = ok ? __builtin_ctzll(c) : lane;
+      const int groot = __shfl(g, lroot, 64);  // label of the component's first particle
+      if (ok && lroot != lane && g == iq) ig[iq] = groot;  // self-labelled: point at the component
+      bool want = ok && lroot != lane && g != iq && g != groot;
+      uint64_t wm = __ballot(want);
+      while (wm) {
+        const int L = __builtin_ctzll(wm);
+        const int gL = __builtin_amdgcn_readlane(g, L), rL = __builtin_amdgcn_readlane(groot, L);
+        const bool mine = want && g == gL && groot == rL;
+        if (lane == L) uf_link(ig, gL, rL);
+        want = want && !mine;
+        wm &= ~__ballot(mine);
+      }
+    }
+  }
+}
+
 // One wave per query leaf; each leaf pair once (source index >= query index).  Source particles
 // are staged 64 at a time through LDS as (position, label) records.  Before staging, each lane
 // tests its source particle p against the bounding box [qlo, qhi] of the query chunk:
@@ -258,79 +346,11 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
       // the last two labels it was joined with
       int gq = ig[iq];
       int glast = -1, glast2 = -1;
-      auto own_leaf_links = [&](const Vec<D, T>& x, bool ok, int n, const Vec<D, T>& lo, const Vec<D, T>& hi, int& g,
-                                int& gl, int& gl2) {
-        bool need_wrap = false;
-        if (ok) {
-#pragma unroll
-          for (int d = 0; d < D; d++)
-            if (PER) need_wrap |= !(lo[d] - x[d] >= -half) || !(hi[d] - x[d] < half);
-          FofRec<D, T> r;
-#pragma unroll
-          for (int d = 0; d < D; d++) r.p[d] = x[d];
-          r.g = g;
-          s_r[lane] = r;
-        }
-        const bool wrap = PER && wave_any(need_wrap);
-        __syncthreads();
-        // neighbours within rlink (the same canonical r2 and test as for any other source chunk)
-        uint64_t nb = 0ull;
-        if (ok) {
-          for (int j = 0; j < n; j++) {
-            Vec<D, T> xp;
-#pragma unroll
-            for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
-            const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
-            if (j != lane && r2 <= (T)r2link) nb |= 1ull << j;
-          }
-        }
-        __syncthreads();
-        // component of each particle: closure of the neighbour sets (c grows by the sets of its
-        // members, so the reach doubles per round)
-        uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
-        bool changed = true;
-        while (wave_any(changed)) {
-          uint64_t cn = c;
-          for (int j = 0; j < n; j++) {
-            const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
-            if ((c >> j) & 1ull) cn |= cj;
-          }
-          changed = cn != c;
-          c = cn;
-        }
-        const int lroot = ok ? __builtin_ctzll(c) : lane;
-        const int groot = __shfl(g, lroot, 64);  // label of the component's first particle
-        bool want = ok && lroot != lane && g != groot;
-        uint64_t wm = __ballot(want);
-        while (wm) {
-          const int L = __builtin_ctzll(wm);
-          const int gL = __builtin_amdgcn_readlane(g, L), rL = __builtin_amdgcn_readlane(groot, L);
-          const bool mine = want && g == gL && groot == rL;
-          const uint64_t mm = __ballot(mine);
-          int root = 0;
-          if (lane == L) root = uf_link(ig, gL, rL);
-          root = __builtin_amdgcn_readlane(root, L);
-          if (mine) {
-            gl2 = gl;
-            gl = groot;
-            g = root;
-            want = false;
-          }
-          wm &= ~mm;
-        }
-      };
       for (int e = e0; e < e1; e++) {
         const int lt = il[e];
         if (lt < lq) continue;  // each leaf pair once
-        if (lt == lq && q1 - q0 <= 64) {
-          // The leaf with itself, in one chunk: its particles' links are resolved in the wave
-          // (components of the within-rlink graph of the <= 64 particles), and only one member
-          // per (label, component label) pair is linked in global memory.
-          own_leaf_links(xq, valid, q1 - q0, qlo, qhi, gq, glast, glast2);
-          continue;
-        }
-        const int t0 = spl[lt], t1 = spl[lt + 1];
+        // a leaf with itself: only its later chunks (a chunk's own links are k_fof_chunks')
+        const int t0 = lt == lq ? qoff + 64 : spl[lt], t1 = spl[lt + 1];
         for (int toff = t0; toff < t1; toff += 64) {
           const int it = toff + lane;
           FofRec<D, T> r;
@@ -351,6 +371,14 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
           }
           uint64_t m = __ballot(reach);
           if (m == 0ull) continue;
+          // one label on every reachable source: if each query already holds it (in its own
+          // label or the last two it joined), no link can come from this chunk
+          const int gs0 = __builtin_amdgcn_readlane(r.g, __builtin_ctzll(m));
+          const bool src_uniform = __ballot(reach && r.g != gs0) == 0ull;
+          auto knows_all = [&]() {
+            return !wave_any(valid && gs0 != gq && gs0 != glast && gs0 != glast2);
+          };
+          if (src_uniform && knows_all()) continue;
           const bool wrap = PER && wave_any(need_wrap);
           s_r[lane] = r;
           __syncthreads();
@@ -392,6 +420,7 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
                 }
               };
               while (m) {
+                if (src_uniform && knows_all()) break;
                 const int j1 = __builtin_ctzll(m);
                 m &= m - 1ull;
                 if (m) {
@@ -465,6 +494,12 @@ int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64
     return fail("FofLeaf2Leaf: copy failed");
   if (size_leaves > 0) {
     const int xcd = tune_get(kTuneXcd);
+    if (boxsize > 0.f)
+      hipLaunchKernelGGL((k_fof_chunks<true, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, spl, (const T*)pos,
+                         ig, r2link, r2v, (T)boxsize, size_leaves, xcd);
+    else
+      hipLaunchKernelGGL((k_fof_chunks<false, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, spl, (const T*)pos,
+                         ig, r2link, r2v, (T)boxsize, size_leaves, xcd);
     if (boxsize > 0.f)
       hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
                          (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd);

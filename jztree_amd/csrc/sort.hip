@@ -935,10 +935,63 @@ int fof_order_impl(hipStream_t s, const int32_t* igroup, const int32_t* npart, i
   return check_launch("FofOrder");
 }
 
+// ---------------------------------------------------------------------------------------------
+// 9. SearchSortedZ (sort.cuh:82-132): insertion indices into a z-sorted array
+// ---------------------------------------------------------------------------------------------
+// For each query x: the index i with have[i-1] < x <= have[i] in the exact z-order of this sort
+// (binary search with the exact comparator), i.e. the left insertion point; n_have past the end.
+// leaf_search: `have` holds one point per leaf (e.g. the leaf centres); the index of the leaf x
+// belongs to, the one of its two z-order neighbours with the smaller level difference (the lower
+// one on ties), as the reference decides it.
+template <int D, typename T>
+__global__ void k_search_sorted_z(const T* __restrict__ have, const T* __restrict__ query, int64_t n_have,
+                                  int64_t n_query, bool leaf_search, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_query) return;
+  T x[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) x[d] = query[i * D + d];
+  int64_t lo = 0, hi = n_have;  // first index with !(have[idx] < x)
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (exact_cmp<D, T>(have + mid * D, x) < 0) lo = mid + 1;
+    else hi = mid;
+  }
+  int64_t r = lo;
+  if (leaf_search) {
+    // leaves lo-1 and lo are the z-order neighbours of x
+    const int64_t a = lo > 0 ? lo - 1 : 0, b = lo < n_have ? lo : n_have - 1;
+    const int lv1 = msb_diff_level<D, T>(load_vec<D, T>(have, a), load_vec<D, T>(x, 0));
+    const int lv2 = b > a ? msb_diff_level<D, T>(load_vec<D, T>(have, b), load_vec<D, T>(x, 0)) : 1 << 20;
+    r = lv1 <= lv2 ? a : b;
+  }
+  out[i] = (int32_t)r;
+}
+
 }  // namespace
 }  // namespace jzt
 
 using namespace jzt;
+
+extern "C" int jzt_search_sorted_z(jzt_stream_t stream, const void* posz_have, const void* posz_query, int64_t n_have,
+                                   int64_t n_query, int dim, int dtype, int leaf_search, int32_t* indices) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n_query <= 0) return 0;
+  if (n_have <= 0) return fail("jzt_search_sorted_z: empty array to search");
+  if (n_have >= (int64_t)INT32_MAX) return fail("jzt_search_sorted_z: n_have must be < 2^31");
+  const unsigned g = (unsigned)div_up(n_query, 256);
+  const bool leaf = leaf_search != 0;
+#define JZT_SSZ(D, T)                                                                                            \
+  hipLaunchKernelGGL((k_search_sorted_z<D, T>), dim3(g), dim3(256), 0, s, (const T*)posz_have, (const T*)posz_query, \
+                     n_have, n_query, leaf, indices)
+  if (dtype == JZT_F32 && dim == 3) JZT_SSZ(3, float);
+  else if (dtype == JZT_F32 && dim == 2) JZT_SSZ(2, float);
+  else if (dtype == JZT_F64 && dim == 3) JZT_SSZ(3, double);
+  else if (dtype == JZT_F64 && dim == 2) JZT_SSZ(2, double);
+  else return fail("jzt_search_sorted_z: unsupported dim/dtype");
+#undef JZT_SSZ
+  return check_launch("SearchSortedZ");
+}
 
 extern "C" size_t jzt_fof_order_tmp_bytes(int64_t n) { return order_layout(n).total; }
 

@@ -20,7 +20,8 @@ from .checks import raise_if
 from .config import KNNConfig
 from .data import InteractionList, Pos, RankIdx, TreeHierarchy, get_num, get_num_total, get_pos, tree_map_by_len
 from .tools import inverse_of_splits, masked_inverse
-from .tree import _tmp, f32_view, top_dense_interaction_list, zsort_and_tree, zsort_and_tree_origin
+from .tree import (_tmp, f32_view, top_dense_interaction_list, zsort_and_tree, zsort_and_tree_multi_type,
+                   zsort_and_tree_origin)
 
 
 @dataclass
@@ -389,25 +390,31 @@ def _fused_retry(th: TreeHierarchy, ilist_p: InteractionList, n2c, posz, k: int,
 
 
 def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, leaf_r2, src_ids=None,
-                          dst_rows=None):
+                          dst_rows=None, splQ=None, xQ=None):
     """Leaf kernel over the guaranteed-radius lists with each leaf's guaranteed k-th radius^2
     (from KnnNode2Node, widened by 2^-10) as the start bound of its queries (extension, k <= 64):
     candidates beyond it are never buffered, so the top-k fills with fewer flushes.  The lists hold
     every leaf within that radius, so a query can only come up short if the float bound rounds
     below its k-th neighbour; such leaves are redone without a bound.  Results equal the
-    unbounded kernel's."""
-    n = len(posz)
-    size = len(spl) - 1
+    unbounded kernel's.  splQ/xQ: separate queries (default: the sources); the guaranteed radius of
+    a leaf holds for any point of its cell."""
+    splQ = spl if splQ is None else splQ
+    xQ = posz if xQ is None else xQ
+    nq = len(xQ)
+    size = len(splQ) - 1
     init = (leaf_r2.to(torch.float64) * (1.0 + 2.0 ** -10)).to(torch.float32).contiguous()
-    rnn, inn, retry = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
+    rnn, inn, retry = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=splQ, xQ=xQ, src_ids=src_ids,
                                      dst_rows=dst_rows, init_r2=init)
     # eagerly the redo runs only when a query came up short (one read-back); under graph capture
     # it always runs, and every leaf without a short query skips it (negative radius)
     if checks.is_deferred() or bool((retry != 0).any()):
-        short = torch.zeros(size + 1, dtype=torch.int32, device=posz.device)
-        short.scatter_reduce_(0, inverse_of_splits(spl, n).long().clamp(0, size), retry, "amax")
-        redo = torch.where(short[:size] != 0, torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
-        _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz, src_ids=src_ids,
+        # per leaf: does it hold a flagged query (a segment maximum over the query splits; a
+        # scatter of every flag onto its leaf costs an atomic per query)
+        short = torch.empty(size, dtype=torch.float32, device=xQ.device)
+        L.call("jzt_segment_max_f32", L.stream(), L.ptr(splQ.contiguous()), L.ptr(retry.to(torch.float32)), nq,
+               L.ptr(short), size, 0.0)
+        redo = torch.where(short > 0, torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
+        _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=splQ, xQ=xQ, src_ids=src_ids,
                        dst_rows=dst_rows, init_r2=redo.contiguous(), out=(rnn, inn, retry))
     return rnn, inn
 
@@ -438,43 +445,64 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
     result is the exact kNN of the rounded points (radii float32).
     """
     assert output_order in ("z", "input")
-    if part_query is not None:
-        raise NotImplementedError("separate query sets (multi-type trees) are not part of this build")
     pos = get_pos(part)
     dev = pos.device
     size = len(pos)
     try:
         num_origin = get_num(part, default_to_length=True)
 
-        if th is not None:
-            partz = part
-            originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
+        if part_query is not None:
+            # one tree over sources (type 0) and queries (type 1), knn.py:259-264
+            assert th is None, "Querying with pre-sorted particles is not supported yet"
+            size_q = len(get_pos(part_query))
+            num_origin_q = get_num(part_query, default_to_length=True)
+            ar = lambda m: RankIdx(rank=None, idx=torch.arange(m, dtype=torch.int32, device=dev))
+            (partz, partz_q), (originz, originz_q), th = zsort_and_tree_multi_type(
+                (part, part_query), cfg.tree, data=(ar(size), ar(size_q)))
         else:
-            partz, originz, th = zsort_and_tree_origin(part, cfg.tree)
+            if th is not None:
+                partz = part
+                originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
+            else:
+                partz, originz, th = zsort_and_tree_origin(part, cfg.tree)
+            partz_q, originz_q, size_q, num_origin_q = partz, originz, size, num_origin
         if output_order == "z":
             originz = RankIdx(rank=None, idx=torch.arange(size, dtype=torch.int32, device=dev))
             num_origin = len(get_pos(partz))
+            originz_q = RankIdx(rank=None, idx=torch.arange(size_q, dtype=torch.int32, device=dev))
+            num_origin_q = len(get_pos(partz_q))
 
         spl = th.splits_leaf_to_part(ptype=0)
+        spl_q = th.splits_leaf_to_part(ptype=0 if part_query is None else 1)
         posz = get_pos(partz)
         th, posz = _distance_view(th, posz, dist_dtype)
+        posz_q = posz if part_query is None else get_pos(partz_q).to(posz.dtype).contiguous()
         keys = result.split("_")
         # rows can be written straight to input order when only radii/indices are requested
         direct = (output_order == "input" and set(keys) <= {"rad", "globalidx"}
-                  and isinstance(num_origin, int) and num_origin == size)
-        if fused and k <= 64:
+                  and isinstance(num_origin_q, int) and num_origin_q == size_q)
+        drows = originz_q.idx if direct else None
+        if part_query is not None and k <= 64:
+            ilist, leaf_r2, _ = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
+                                               return_parents=True)
+            rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx, drows,
+                                                splQ=spl_q, xQ=posz_q)
+        elif part_query is not None:
+            ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
+            rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl_q, xQ=posz_q,
+                                         src_ids=originz.idx, dst_rows=drows)
+        elif fused and k <= 64:
             # neighbours are reported (and tie-broken) by their origin index directly
             rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
-                                        src_ids=originz.idx, dst_rows=originz.idx if direct else None)
+                                        src_ids=originz.idx, dst_rows=drows)
         elif k <= 64:
             ilist, leaf_r2, _ = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                                return_parents=True)
-            rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx,
-                                                originz.idx if direct else None)
+            rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx, drows)
         else:
             ilist = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist)
             rnnz, inn_g = _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=spl, xQ=posz,
-                                         src_ids=originz.idx, dst_rows=originz.idx if direct else None)
+                                         src_ids=originz.idx, dst_rows=drows)
 
         res = []
         zidx_cache = {}
@@ -490,8 +518,7 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
             if key == "rad":
                 res.append(rnnz)
             elif key == "drad":
-                xz = posz
-                res.append(torch.linalg.norm(xz[:, None] - xz[innz()], dim=-1))
+                res.append(torch.linalg.norm(posz_q[:, None] - posz[innz()], dim=-1))
             elif key == "rankidx":
                 res.append(RankIdx(rank=None, idx=inn_g))
             elif key == "globalidx":
@@ -512,9 +539,9 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
         res = tuple(res) if len(res) > 1 else res[0]
 
         if output_order == "input" and not direct:
-            idx = originz.idx
-            inverse = masked_inverse(idx, mask=torch.arange(len(idx), device=dev) < num_origin)
-            res = tree_map_by_len(lambda v: v[inverse.long().clamp(max=size - 1)], res, size)
+            idx = originz_q.idx
+            inverse = masked_inverse(idx, mask=torch.arange(len(idx), device=dev) < num_origin_q)
+            res = tree_map_by_len(lambda v: v[inverse.long().clamp(max=size_q - 1)], res, size_q)
         checks.flush()
     except BaseException:
         checks.clear()

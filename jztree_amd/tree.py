@@ -295,13 +295,22 @@ def _delete_rows(x: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
     return out[:n - m]
 
 
+def _splits_per_type(spl: torch.Tensor, ptype: torch.Tensor, num_types: int, npart) -> torch.Tensor:
+    """tree.py:621-626: per type t, the number of filled type-t particles before each split."""
+    n = len(ptype)
+    filled = torch.arange(n, device=ptype.device) < (npart if npart is not None else n)
+    out = []
+    for t in range(num_types):
+        prefix = cumsum_starting_with_zero(((ptype == t) & filled).to(torch.int32))
+        out.append(prefix[spl.long().clamp(0, n)])
+    return torch.stack(out).to(torch.int32)
+
+
 def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None, num_types=None, lvl_bound=None,
                             cfg_reg: RegularizationConfig | None = None, set_id: torch.Tensor | None = None):
     """tree.py:628-734: leaf splits, then node activity per plane packed into PackedArrays.
 
     set_id (extension): batches of sets; boundaries between sets split every plane."""
-    if ptype is not None:
-        raise NotImplementedError("multi-type trees are not part of this build")
     n, dim = posz.shape
     dev = posz.device
     nlevels = len(node_sizes)
@@ -309,8 +318,8 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     if lvl_bound is None:
         lvl_bound = (linfo.max_lvl(), linfo.max_lvl())
 
-    ispl = detect_leaf_boundaries(posz, leaf_size=node_sizes[0], npart=npart, alloc_size=alloc_size,
-                                  lvl_bound=lvl_bound, cfg_reg=cfg_reg, set_id=set_id)
+    ispl = detect_leaf_boundaries(posz, ptype=ptype, num_types=num_types, leaf_size=node_sizes[0], npart=npart,
+                                  alloc_size=alloc_size, lvl_bound=lvl_bound, cfg_reg=cfg_reg, set_id=set_id)
     # first index of the largest split (= the leaf count); the splits never decrease, so a search
     # finds it without a multi-block reduction (which would record a memset in a captured graph)
     nleaves = torch.searchsorted(ispl, ispl[-1:]).reshape(()).to(torch.int32)
@@ -319,8 +328,13 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     leaf_set = None if set_id is None else set_id[first].contiguous()
     lvl, lbound, rbound = determine_znode_boundaries(reps, nleaves=nleaves, set_id=leaf_set)
     lbl, rbl = lbound.long(), rbound.long()
-    npins = ispl[rbl] - ispl[lbl]
-    ispl_per_type = ispl.reshape((1,) + ispl.shape)
+    if ptype is not None:
+        # several types: a node may not hold more than the node size of any one type (tree.py:661-664)
+        ispl_per_type = _splits_per_type(ispl, ptype, num_types, npart)
+        npins = torch.amax(ispl_per_type[:, rbl] - ispl_per_type[:, lbl], dim=0)
+    else:
+        npins = ispl[rbl] - ispl[lbl]
+        ispl_per_type = ispl.reshape((1,) + ispl.shape)
 
     sizes = _i32_list(node_sizes, dev)
     isl = npins[None, :] > sizes[:, None]
@@ -422,11 +436,9 @@ def zsort_and_tree(part, cfg_tree: TreeConfig = TreeConfig(), data: Any | None =
     """tree.py:363-421 (single GPU): z-sort (carrying `data`) and build the hierarchy.
 
     Returns (partz, th), or (partz, dataz, th) when data is given."""
-    if ptype is not None:
-        raise NotImplementedError("multi-type trees are not part of this build")
     partz, isort = zsort(part)
     dtz = tree_map_by_len(lambda v: v[isort.long()], (data, ptype), len(get_pos(partz)))
-    th = build_tree_hierarchy(partz, cfg_tree)
+    th = build_tree_hierarchy(partz, cfg_tree, ptype=dtz[1], num_types=num_types)
     out = [partz]
     if (data is not None) or not shrink:
         out.append(dtz[0])
@@ -436,6 +448,79 @@ def zsort_and_tree(part, cfg_tree: TreeConfig = TreeConfig(), data: Any | None =
 
 
 zsort_and_tree.jit = zsort_and_tree
+
+
+def zsort_and_tree_multi_type(part: Tuple, cfg_tree: TreeConfig = TreeConfig(), data: Any | None = None):
+    """tree.py:423-477 (single GPU): one tree over several particle types (e.g. sources and
+    queries); leaves hold at most max_leaf_size particles of each type and
+    th.ispl_l2p_per_type[t] splits the z-ordered particles of type t.  Returns (partz, th) or
+    (partz, dataz, th), tuples with one entry per type, each type's particles in z-order."""
+    num_types = len(part)
+    sizes = [len(get_pos(p)) for p in part]
+    dev = get_pos(part[0]).device
+    if data is None:
+        data = tuple(None for _ in part)
+    # filled rows of each type first (rows with a NaN position last), as concatenate_pytrees does
+    valid = torch.cat([~torch.isnan(get_pos(p)[:, 0]) for p in part])
+    compress = torch.argsort((~valid).to(torch.int8), stable=True)
+    ptype = torch.repeat_interleave(torch.arange(num_types, dtype=torch.uint8, device=dev),
+                                    torch.tensor(sizes, device=dev))[compress]
+
+    def stack(items):
+        first = items[0]
+        if torch.is_tensor(first):
+            return torch.cat(items)[compress]
+        if isinstance(first, RankIdx):
+            return RankIdx(rank=None, idx=torch.cat([it.idx for it in items])[compress])
+        if dataclasses.is_dataclass(first):
+            vals = {}
+            for f in dataclasses.fields(first):
+                vs = [getattr(it, f.name) for it in items]
+                if torch.is_tensor(vs[0]) and vs[0].ndim > 0 and len(vs[0]) == len(get_pos(first)):
+                    vals[f.name] = torch.cat(vs)[compress]
+                else:
+                    vals[f.name] = vs[0]
+            if "num" in vals:
+                vals["num"] = None
+            if "num_total" in vals:
+                vals["num_total"] = None
+            return dataclasses.replace(first, **vals)
+        raise TypeError(f"cannot stack {type(first)}")
+
+    part_stacked = stack(list(part))
+    data_stacked = stack(list(data)) if data[0] is not None else None
+    partz_s, dataz_s, ptypez, th = zsort_and_tree(part_stacked, cfg_tree=cfg_tree, data=data_stacked, ptype=ptype,
+                                                  num_types=num_types, shrink=False)
+    # separate the types, each in z-order (a stable sort by type)
+    order = torch.argsort(ptypez.to(torch.int8), stable=True)
+    out_p, out_d = [], []
+    off = 0
+    for t, sz in enumerate(sizes):
+        sel = order[off:off + sz]
+        off += sz
+        out_p.append(tree_map_by_len(lambda v: v[sel], partz_s, len(ptypez)))
+        out_d.append(None if dataz_s is None else tree_map_by_len(lambda v: v[sel], dataz_s, len(ptypez)))
+    if data[0] is not None:
+        return tuple(out_p), tuple(out_d), th
+    return tuple(out_p), th
+
+
+zsort_and_tree_multi_type.jit = zsort_and_tree_multi_type
+
+
+def search_sorted_z(posz: torch.Tensor, posz_query: torch.Tensor, block_size: int = 64, leaf_search: bool = False):
+    """tree.py:143-158: indices where the points of posz_query would be inserted into the z-sorted
+    posz to keep the z-order (posz[idx-1] < x <= posz[idx]); with leaf_search, posz holds one point
+    per leaf and the result is the leaf each query point belongs to (SearchSortedZ)."""
+    assert posz.dtype == posz_query.dtype
+    assert posz.shape[-1] == posz_query.shape[-1]
+    out = torch.empty(len(posz_query), dtype=torch.int32, device=posz.device)
+    L.call("jzt_search_sorted_z", L.stream(), L.ptr(posz.contiguous()), L.ptr(posz_query.contiguous()), len(posz),
+           len(posz_query), posz.shape[-1], L.dtype_code(posz.dtype), int(bool(leaf_search)), L.ptr(out))
+    return out
+
+
+search_sorted_z.jit = search_sorted_z
 
 
 def zsort_and_tree_origin(part, cfg_tree: TreeConfig = TreeConfig()):

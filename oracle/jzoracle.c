@@ -402,7 +402,8 @@ static void topk_insert(float* r2s, int64_t* ids, int k, float r2, int64_t id) {
 /* Brute force O(Nq*Ns): for every query, the k smallest (r2, id).  Output r2 (not sqrt). */
 void orc_knn_brute_f32(const float* xs, const int64_t* ids, int64_t ns, const float* xq, int64_t nq,
                        int dim, int k, float boxsize, float* r2_out, int64_t* id_out) {
-    #pragma omp parallel for schedule(dynamic, 64)
+    /* one query per scheduling unit: a few queries against many sources still use every thread */
+    #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t q = 0; q < nq; q++) {
         float* r2s = r2_out + q * k;
         int64_t* ii = id_out + q * k;

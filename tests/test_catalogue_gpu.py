@@ -1,7 +1,7 @@
 """FoF catalogue on the GPU (fof_and_catalogue, _fof_order, _fof_catalogue_from_groups) against the
 oracle: group order and integer fields bit-exact, floating-point properties within the particle
-type's rounding (rtol 2e-5 for float32 sums that the reference itself forms in float32 in an
-unspecified order; 1e-12 for float64), and the reference's cosmo-2D answer (164 groups >= 10 in
+type's rounding (float32: rtol 2e-5, atol 2e-8, looser than the reference's own float32 sums in an
+unspecified order; float64: rtol 1e-12, atol 1e-10 for means that nearly cancel), and the reference's cosmo-2D answer (164 groups >= 10 in
 a catalogue of 1639 slots, docs/quickstart.md:219-251)."""
 import dataclasses
 
@@ -22,7 +22,7 @@ def _cfg(npart_min=20):
                                catalogue=jz.config.FofCatalogueConfig(npart_min=npart_min))
 
 
-def _compare(cat, ref, rtol):
+def _compare(cat, ref, rtol, atol):
     ng = int(cat.ngroups)
     assert ng == ref["ngroups"]
     assert np.array_equal(cat.count.cpu().numpy(), ref["count"])
@@ -40,7 +40,7 @@ def _compare(cat, ref, rtol):
             d = np.minimum(d, ref["box"] - d)
             assert np.all(d <= rtol * ref["box"]), f"{name}: max diff {d.max()}"
         else:
-            assert np.allclose(got[:ng], exp[:ng], rtol=rtol, atol=rtol * 1e-3), \
+            assert np.allclose(got[:ng], exp[:ng], rtol=rtol, atol=atol), \
                 f"{name}: max rel diff {np.max(np.abs(got[:ng] / exp[:ng] - 1))}"
 
 
@@ -64,7 +64,7 @@ def test_cosmo2d_catalogue(dev, oracle):
     assert np.array_equal(partf.vel.cpu().numpy(), vel[idz][isort])
     ref = oracle.fof_catalogue(pz[isort], counts, npart_min=10, boxsize=100.0, mass=1.0, vel=vel[idz][isort])
     ref["box"] = 100.0
-    _compare(cat, ref, 2e-5)
+    _compare(cat, ref, 2e-5, 2e-8)
     # biggest group: its particles are the contiguous run at its offset
     imax = int(torch.argmax(cat.count))
     o, c = int(cat.offset[imax]), int(cat.count[imax])
@@ -107,7 +107,8 @@ def test_clustered_catalogue_all_fields(dev, oracle, dtype, box):
                                v_rad=vrad[perm], zplus1=zp1[perm])
     ref["box"] = box
     assert ref["ngroups"] > 100
-    _compare(cat, ref, 2e-5 if dtype == np.float32 else 1e-12)
+    # means of O(1) values that nearly cancel (velocities): an absolute term for the summation order
+    _compare(cat, ref, *((2e-5, 2e-8) if dtype == np.float32 else (1e-12, 1e-10)))
     srt = jz.data.sort_catalogue(jz.data.squeeze_catalogue(cat), by="count")
     assert bool(torch.all(srt.count[1:] <= srt.count[:-1]))
 
