@@ -267,31 +267,51 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
         s_r[lane] = r;
       }
       const bool wrap = PER && wave_any(need_wrap);
-      __syncthreads();
-      uint64_t nb = 0ull;
-      if (ok) {
-        for (int j = 0; j < n; j++) {
-          Vec<D, T> xp;
+      // A chunk whose bounding box has a diagonal within rlink is one component: every pair's
+      // canonical r2 is at most the box diagonal's (each |fl(q - p)| <= fl(hi - lo) by monotone
+      // rounding, and the FMA chain is monotone in each difference), so no pair test is needed.
+      const bool one_component = !wrap && dist2P<false, D, T>(hi, lo, boxsize, half) <= (T)r2link;
+      int lroot = lane;
+      if (one_component) {
+        lroot = 0;
+      } else {
+        __syncthreads();
+        uint64_t nb = 0ull;
+        if (ok) {
+          // four staged records per step: their LDS reads are in flight together
+          for (int j0 = 0; j0 < n; j0 += 4) {
+            FofRec<D, T> rr[4];
 #pragma unroll
-          for (int d = 0; d < D; d++) xp[d] = s_r[j].p[d];
-          const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
-          if (j != lane && r2 <= (T)r2link) nb |= 1ull << j;
+            for (int u = 0; u < 4; u++) rr[u] = s_r[min(j0 + u, n - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              Vec<D, T> xp;
+#pragma unroll
+              for (int d = 0; d < D; d++) xp[d] = rr[u].p[d];
+              const int j = j0 + u;
+              const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
+              if (j < n && j != lane && r2 <= (T)r2link) nb |= 1ull << j;
+            }
+          }
         }
+        uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
+        // only particles with a neighbour can be in another's set: the closure runs over them
+        const uint64_t act = __ballot(nb != 0ull);
+        bool changed = act != 0ull;
+        while (wave_any(changed)) {
+          uint64_t cn = c;
+          for (uint64_t a = act; a; a &= a - 1ull) {
+            const int j = __builtin_ctzll(a);
+            const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
+            if ((c >> j) & 1ull) cn |= cj;
+          }
+          changed = cn != c;
+          c = cn;
+        }
+        lroot = ok ? __builtin_ctzll(c) : lane;
       }
       __syncthreads();
-      uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
-      bool changed = true;
-      while (wave_any(changed)) {
-        uint64_t cn = c;
-        for (int j = 0; j < n; j++) {
-          const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
-                              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
-          if ((c >> j) & 1ull) cn |= cj;
-        }
-        changed = cn != c;
-        c = cn;
-      }
-      const int lroot = ok ? __builtin_ctzll(c) : lane;
       const int groot = __shfl(g, lroot, 64);  // label of the component's first particle
       if (ok && lroot != lane && g == iq) ig[iq] = groot;  // self-labelled: point at the component
       bool want = ok && lroot != lane && g != iq && g != groot;
@@ -318,13 +338,32 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
 //     cannot link with any query of the chunk, because the box gap bounds every |q - p| per dim
 //     from below (the slack covers the rounding of the gap and of the canonical distance).
 // Only reachable sources are visited; a chunk with none is skipped.
+// Links are not made where they are found: the (query label, source label) pair goes into a queue
+// in LDS, and the queue is worked off by all 64 lanes at once (one link per lane, their memory
+// round trips overlapping) when it is half full and at the end.  Each lane remembers the labels
+// its set was queued with, so the dedup tests stay valid: a queued pair is joined before the
+// kernel ends, and the components are those of any order of the same links.  (Should a step find
+// the queue full, its leader lane links at once.)
 template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
                                                  float r2link_all, const float* __restrict__ r2v, T boxsize,
                                                  int64_t il_size, int64_t nitems, int xcd) {
+  constexpr int kQueue = 256;
   __shared__ FofRec<D, T> s_r[64];
+  __shared__ int2 s_q[kQueue];
   const int lane = threadIdx.x;
+  int qn = 0;  // queued pairs (wave-uniform)
+  // every lane active: the queued links, one per lane
+  auto flush = [&]() {
+    __syncthreads();
+    for (int i = lane; i < qn; i += 64) {
+      const int2 pr = s_q[i];
+      uf_link(ig, pr.x, pr.y);
+    }
+    qn = 0;
+    __syncthreads();
+  };
   const ItemRange items = xcd_items(active_items(spl, nitems), xcd != 0);
   for (int64_t lq = items.first; lq < items.end; lq += items.step) {
     const int q0 = spl[lq], q1 = spl[lq + 1];
@@ -342,8 +381,7 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
         qlo[d] = wave_min_t<T>(xq[d]);
         qhi[d] = wave_max_t<T>(xq[d]);
       }
-      // labels known to be in q's set: its own label, refreshed to the root after each join, and
-      // the last two labels it was joined with
+      // labels known to be in q's set: its own label and the last two labels it was queued with
       int gq = ig[iq];
       int glast = -1, glast2 = -1;
       for (int e = e0; e < e1; e++) {
@@ -369,83 +407,106 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
             r.g = ig[it];
             reach = need_wrap || g2 <= r2skip;
           }
-          uint64_t m = __ballot(reach);
-          if (m == 0ull) continue;
-          // one label on every reachable source: if each query already holds it (in its own
-          // label or the last two it joined), no link can come from this chunk
-          const int gs0 = __builtin_amdgcn_readlane(r.g, __builtin_ctzll(m));
-          const bool src_uniform = __ballot(reach && r.g != gs0) == 0ull;
-          auto knows_all = [&]() {
-            return !wave_any(valid && gs0 != gq && gs0 != glast && gs0 != glast2);
-          };
-          if (src_uniform && knows_all()) continue;
+          uint64_t rem = __ballot(reach);
+          if (rem == 0ull) continue;
           const bool wrap = PER && wave_any(need_wrap);
+          if (qn > kQueue / 2) flush();
           s_r[lane] = r;
           __syncthreads();
-          if (valid) {
-            // Sources go two at a time, their distances and tests without branches (a label test
-            // that would skip a source only saves arithmetic): the loop's scalar control and exec
-            // mask work is paid once per pair.  The periodic and plain variants are separate loops.
-            auto sources = [&](auto wrap_c) {
-              constexpr bool W = decltype(wrap_c)::value;
-              auto test = [&](int j, int& gj) {
-                const FofRec<D, T> rj = s_r[j];
-                gj = rj.g;
-                Vec<D, T> xp;
-  #pragma unroll
-                for (int d = 0; d < D; d++) xp[d] = rj.p[d];
-                const T r2 = dist2P<W, D, T>(xq, xp, boxsize, half);
-                // labels are snapshots of set members; a label equal to q's own, or to one q was
-                // already joined with, belongs to q's set by now
-                return (gj != gq) & (gj != glast) & (gj != glast2) & (r2 <= (T)r2link);
-              };
-              // Lanes whose q holds the same label are in one set: a single link of that set with
-              // p's joins them all, so one lane links and the others take its root.
-              auto link = [&](bool want, int gj) {
-                uint64_t wm = __ballot(want);
-                while (wm) {
-                  const int L = __builtin_ctzll(wm);
-                  const bool mine = want && gq == __builtin_amdgcn_readlane(gq, L);
-                  const uint64_t mm = __ballot(mine);
-                  int root = 0;
-                  if (lane == L) root = uf_link(ig, gq, gj);
-                  root = __builtin_amdgcn_readlane(root, L);
-                  if (mine) {
-                    glast2 = glast;
-                    glast = gj;
-                    gq = root;
-                    want = false;
-                  }
-                  wm &= ~mm;
+          // The reachable sources label by label.  Labels are snapshots of set members: a query
+          // whose own label, or one of the last two labels it was queued with, equals the label
+          // needs no test of its sources.  For a label held by several sources, the others test
+          // them two at a time until each has found one within rlink (one link per (query set,
+          // label) is all it takes); labels held by a single source are collected and their
+          // sources tested afterwards, two at a time.
+          auto by_label = [&](auto wrap_c) {
+            constexpr bool W = decltype(wrap_c)::value;
+            auto near = [&](int j, int& gj) {
+              const FofRec<D, T> rj = s_r[j];
+              gj = rj.g;
+              Vec<D, T> xp;
+#pragma unroll
+              for (int d = 0; d < D; d++) xp[d] = rj.p[d];
+              return dist2P<W, D, T>(xq, xp, boxsize, half) <= (T)r2link;
+            };
+            auto knows = [&](int g) { return !valid || g == gq || g == glast || g == glast2; };
+            // Lanes whose q holds the same label are in one set: a single link of that set with
+            // the label's joins them all, so one pair is queued for them.
+            auto queue_link = [&](bool want, int G) {
+              uint64_t wm = __ballot(want);
+              while (wm) {
+                const int L = __builtin_ctzll(wm);
+                const bool mine = want && gq == __builtin_amdgcn_readlane(gq, L);
+                const uint64_t mm = __ballot(mine);
+                if (qn < kQueue) {
+                  if (lane == L) s_q[qn] = make_int2(gq, G);
+                  qn++;
+                } else if (lane == L) {
+                  uf_link(ig, gq, G);
                 }
-              };
-              while (m) {
-                if (src_uniform && knows_all()) break;
-                const int j1 = __builtin_ctzll(m);
-                m &= m - 1ull;
-                if (m) {
-                  const int j2 = __builtin_ctzll(m);
-                  m &= m - 1ull;
-                  int g1, g2;
-                  const bool w1 = test(j1, g1), w2 = test(j2, g2);
-                  link(w1, g1);
-                  link(w2 && (g2 != gq) && (g2 != glast) && (g2 != glast2), g2);
-                } else {
-                  int g1;
-                  const bool w1 = test(j1, g1);
-                  link(w1, g1);
+                if (mine) {
+                  glast2 = glast;
+                  glast = G;
+                  want = false;
                 }
+                wm &= ~mm;
               }
             };
-            if (wrap) sources(std::integral_constant<bool, true>());
-            else sources(std::integral_constant<bool, false>());
-          }
+            uint64_t single = 0ull;
+            while (rem) {
+              const int jg = __builtin_ctzll(rem);
+              const int G = __builtin_amdgcn_readlane(r.g, jg);
+              uint64_t ms = __ballot(reach && r.g == G);
+              rem &= ~ms;
+              if (ms == (1ull << jg)) {
+                single |= ms;
+                continue;
+              }
+              const bool need = !knows(G);
+              if (!wave_any(need)) continue;
+              bool hit = false;
+              while (ms) {
+                int gj;
+                const int j1 = __builtin_ctzll(ms);
+                ms &= ms - 1ull;
+                if (ms) {
+                  const int j2 = __builtin_ctzll(ms);
+                  ms &= ms - 1ull;
+                  const bool h1 = near(j1, gj), h2 = near(j2, gj);
+                  hit = hit || h1 || h2;
+                } else {
+                  hit = hit || near(j1, gj);
+                }
+                if (!wave_any(need && !hit)) break;
+              }
+              queue_link(need && hit, G);
+            }
+            // single-source labels: distinct, so a link for one leaves the others' tests as they were
+            while (single) {
+              const int j1 = __builtin_ctzll(single);
+              single &= single - 1ull;
+              int g1, g2;
+              if (single) {
+                const int j2 = __builtin_ctzll(single);
+                single &= single - 1ull;
+                const bool h1 = near(j1, g1), h2 = near(j2, g2);
+                const bool w1 = h1 && !knows(g1), w2 = h2 && !knows(g2);
+                queue_link(w1, g1);
+                queue_link(w2, g2);
+              } else {
+                const bool h1 = near(j1, g1);
+                queue_link(h1 && !knows(g1), g1);
+              }
+            }
+          };
+          if (wrap) by_label(std::integral_constant<bool, true>());
+          else by_label(std::integral_constant<bool, false>());
           __syncthreads();
         }
       }
     }
-  
   }
+  flush();
 }
 
 template <int D, typename T>
