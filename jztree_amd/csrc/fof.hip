@@ -421,13 +421,23 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
           // sources tested afterwards, two at a time.
           auto by_label = [&](auto wrap_c) {
             constexpr bool W = decltype(wrap_c)::value;
-            auto near = [&](int j, int& gj) {
+            auto r2_of = [&](int j, int& gj) {
               const FofRec<D, T> rj = s_r[j];
               gj = rj.g;
               Vec<D, T> xp;
 #pragma unroll
               for (int d = 0; d < D; d++) xp[d] = rj.p[d];
-              return dist2P<W, D, T>(xq, xp, boxsize, half) <= (T)r2link;
+              return dist2P<W, D, T>(xq, xp, boxsize, half);
+            };
+            // the next four set bits of a source mask (repeating the first where fewer are left)
+            auto take4 = [](uint64_t& ms, int (&j)[4]) {
+              j[0] = __builtin_ctzll(ms);
+              ms &= ms - 1ull;
+#pragma unroll
+              for (int u = 1; u < 4; u++) {
+                j[u] = ms ? __builtin_ctzll(ms) : j[0];
+                ms &= ms - 1ull;
+              }
             };
             auto knows = [&](int g) { return !valid || g == gq || g == glast || g == glast2; };
             // Lanes whose q holds the same label are in one set: a single link of that set with
@@ -464,39 +474,34 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
               }
               const bool need = !knows(G);
               if (!wave_any(need)) continue;
-              bool hit = false;
+              // the lane's nearest source of the label so far, four sources per step
+              T best = (T)__builtin_inff();
               while (ms) {
-                int gj;
-                const int j1 = __builtin_ctzll(ms);
-                ms &= ms - 1ull;
-                if (ms) {
-                  const int j2 = __builtin_ctzll(ms);
-                  ms &= ms - 1ull;
-                  const bool h1 = near(j1, gj), h2 = near(j2, gj);
-                  hit = hit || h1 || h2;
-                } else {
-                  hit = hit || near(j1, gj);
-                }
-                if (!wave_any(need && !hit)) break;
+                int jj[4], gj;
+                take4(ms, jj);
+                T r2[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) r2[u] = r2_of(jj[u], gj);
+                best = min(best, min(min(r2[0], r2[1]), min(r2[2], r2[3])));
+                if (!wave_any(need && !(best <= (T)r2link))) break;
               }
-              queue_link(need && hit, G);
+              queue_link(need && best <= (T)r2link, G);
             }
-            // single-source labels: distinct, so a link for one leaves the others' tests as they were
+            // single-source labels, four per step: distinct labels, so a link for one leaves the
+            // others' tests as they were
             while (single) {
-              const int j1 = __builtin_ctzll(single);
-              single &= single - 1ull;
-              int g1, g2;
-              if (single) {
-                const int j2 = __builtin_ctzll(single);
-                single &= single - 1ull;
-                const bool h1 = near(j1, g1), h2 = near(j2, g2);
-                const bool w1 = h1 && !knows(g1), w2 = h2 && !knows(g2);
-                queue_link(w1, g1);
-                queue_link(w2, g2);
-              } else {
-                const bool h1 = near(j1, g1);
-                queue_link(h1 && !knows(g1), g1);
+              const int n4 = min(4, __builtin_popcountll(single));
+              int jj[4], gj[4];
+              take4(single, jj);
+              bool w[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const T r2 = r2_of(jj[u], gj[u]);
+                w[u] = r2 <= (T)r2link && !knows(gj[u]);
               }
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                if (u < n4) queue_link(w[u], gj[u]);
             }
           };
           if (wrap) by_label(std::integral_constant<bool, true>());
