@@ -277,8 +277,10 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
       } else {
         __syncthreads();
         uint64_t nb = 0ull;
-        if (ok) {
-          // four staged records per step: their LDS reads are in flight together
+        // four staged records per step (their LDS reads in flight together); the periodic and
+        // plain distance loops are separate
+        auto pairs = [&](auto wrap_c) {
+          constexpr bool W = decltype(wrap_c)::value;
           for (int j0 = 0; j0 < n; j0 += 4) {
             FofRec<D, T> rr[4];
 #pragma unroll
@@ -289,10 +291,14 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
 #pragma unroll
               for (int d = 0; d < D; d++) xp[d] = rr[u].p[d];
               const int j = j0 + u;
-              const T r2 = wrap ? dist2P<true, D, T>(x, xp, boxsize, half) : dist2P<false, D, T>(x, xp, boxsize, half);
+              const T r2 = dist2P<W, D, T>(x, xp, boxsize, half);
               if (j < n && j != lane && r2 <= (T)r2link) nb |= 1ull << j;
             }
           }
+        };
+        if (ok) {
+          if (wrap) pairs(std::integral_constant<bool, true>());
+          else pairs(std::integral_constant<bool, false>());
         }
         uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
         // only particles with a neighbour can be in another's set: the closure runs over them

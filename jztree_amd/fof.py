@@ -159,10 +159,11 @@ def _fof_leaf2leaf(leaf_data: FofNodeData, ilist: InteractionList, posz, rlink: 
     assert len(leaf_data.spl) == len(ilist.ispl) == len(leaf_data.igroup) + 1 == len(leaf_data.lvl) + 1
     dev = posz.device
     rl = rlink if leaf_r2link is None else leaf_r2link
+    # (num=spl[-1] of fof.py:160 masks nothing here: NodeToChildLabel writes only children below
+    # the parents' last split and zero-fills the rest)
     part_igroup = _node_to_child_label(leaf_data.igroup, leaf_data.lvl, leaf_data.spl, rlink=rl,
-                                       size_child=len(posz), num=leaf_data.spl[-1], dim=posz.shape[-1],
-                                       dtype=posz.dtype)
-    out = torch.empty(len(posz), dtype=torch.int32, device=dev)
+                                       size_child=len(posz), dim=posz.shape[-1], dtype=posz.dtype)
+    out = part_igroup  # linked in place
     if leaf_r2link is not None:
         L.call("jzt_fof_leaf2leaf_sets", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
                L.ptr(leaf_data.spl.contiguous()), L.ptr(_r2_items(leaf_r2link, len(leaf_data.spl) - 1)),
