@@ -18,7 +18,8 @@ namespace {
 constexpr int kTreeThreads = 256;
 constexpr int kFlagPer = 4;  // boundaries per thread in k_flag_leaf: the LDS window's halo is shared by more of them
 constexpr int kFlagSpan = kTreeThreads * kFlagPer;
-constexpr int kFlagChunk = 8;  // levels read per step of k_flag_leaf's outward scans
+constexpr int kFlagChunk = 8;  // levels per block in k_flag_leaf's outward scans
+constexpr int kFlagPad = 8 * kFlagChunk + kFlagChunk;  // slots past either end of its LDS window
 
 template <int D, typename T>
 __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict__ posz, const uint8_t* __restrict__ ptype,
@@ -29,9 +30,12 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
                                                             int num_types, const int32_t* __restrict__ seg) {
   extern __shared__ int32_t s_mem[];
   const int nload = kFlagSpan + 2 * scan_size + 1;
-  // kFlagChunk unused slots on either side: a scan step may read past the window (masked out)
-  int32_t* const s_level = s_mem + kFlagChunk;
-  uint8_t* s_types = reinterpret_cast<uint8_t*>(s_mem + nload + 2 * kFlagChunk);
+  const int nslot = nload + 2 * kFlagPad;
+  // levels, and the maximum of each run of kFlagChunk levels ending at a slot; kFlagPad slots on
+  // either side hold INT_MIN (the scans may read past the window: those reads never hit)
+  int32_t* const s_level = s_mem + kFlagPad;
+  int32_t* const s_max8 = s_mem + nslot + kFlagPad;
+  uint8_t* s_types = reinterpret_cast<uint8_t*>(s_mem + 2 * nslot);
   const int64_t nump = npart_p[0];
   const int lb_l = lvl_bound[0], lb_r = lvl_bound[1];
   const int64_t ioff = (int64_t)blockIdx.x * kFlagSpan - scan_size - 1;
@@ -53,7 +57,21 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
     s_level[i] = L;
     s_types[i] = ty;
   }
+  for (int i = threadIdx.x; i < 2 * kFlagPad + 1; i += kTreeThreads) {
+    const int j = i < kFlagPad ? i - kFlagPad : nload - 1 + (i - kFlagPad);
+    s_level[j] = INT32_MIN;
+    s_max8[j] = INT32_MIN;
+  }
   __syncthreads();
+  if (num_types == 1) {
+    for (int i = threadIdx.x; i < nload - 1; i += kTreeThreads) {
+      int m = s_level[i];
+#pragma unroll
+      for (int u = 1; u < kFlagChunk; u++) m = max(m, s_level[i - u]);
+      s_max8[i] = m;
+    }
+    __syncthreads();
+  }
 
   for (int rep = 0; rep < kFlagPer; rep++) {
     const int64_t node_idx = (int64_t)blockIdx.x * kFlagSpan + rep * kTreeThreads + threadIdx.x;
@@ -62,40 +80,40 @@ __global__ __launch_bounds__(kTreeThreads) void k_flag_leaf(const T* __restrict_
     bool is_split = false;
     if (num_types == 1) {
       // one type: lsize = idx - (the nearest i < idx in the window with s_level[i] >= my), or the whole
-      // window if there is none -- what the reset-and-count loop below leaves; scanning outward from
-      // idx stops there instead of reading the whole window (likewise rsize to the right).  The scan
-      // reads kFlagChunk levels per step (those past the window land in the pad slots and are
-      // masked out), all issued before any test and without branches, so a wave waits for one LDS
-      // round trip per step instead of one per level.
+      // window if there is none -- what the reset-and-count loop below leaves -- i.e. the distance
+      // to the nearest level >= my on the left, capped at scan_size (likewise rsize to the right).
+      // The scan goes by blocks of kFlagChunk levels: eight block maxima per step (one LDS round
+      // trip covers 64 levels), then the levels of the first block that holds a hit.  Reads past
+      // the window land in the INT_MIN pads or beyond scan_size, where the cap hides them.
       const int32_t* const here = s_level + idx;
-      int lsize = scan_size;
-      for (int base = 1; base <= scan_size; base += kFlagChunk) {
-        int lv[kFlagChunk];
+      const int32_t* const here8 = s_max8 + idx;
+      auto nearest = [&](auto right_c) {
+        constexpr bool R = decltype(right_c)::value;
+        for (int d0 = 1; d0 <= scan_size; d0 += 8 * kFlagChunk) {
+          int mb[8];
 #pragma unroll
-        for (int u = 0; u < kFlagChunk; u++) lv[u] = here[-(base + u)];
-        const int room = scan_size - base;  // u <= room stays inside the window
-        int hit = 0;
+          for (int u = 0; u < 8; u++) {
+            const int d = d0 + u * kFlagChunk;  // block of distances [d, d + kFlagChunk - 1]
+            mb[u] = R ? here8[d + kFlagChunk - 1] : here8[-d];
+          }
+          int blk = -1;
 #pragma unroll
-        for (int u = kFlagChunk - 1; u >= 0; u--) hit = ((u <= room) & (lv[u] >= my)) ? base + u : hit;
-        if (hit) {
-          lsize = hit;
-          break;
+          for (int u = 7; u >= 0; u--) blk = mb[u] >= my ? u : blk;
+          if (blk >= 0) {
+            const int d = d0 + blk * kFlagChunk;
+            int lv[kFlagChunk];
+#pragma unroll
+            for (int v = 0; v < kFlagChunk; v++) lv[v] = R ? here[d + v] : here[-(d + v)];
+            int hit = d + kFlagChunk - 1;  // the block max is a hit, so some v matches
+#pragma unroll
+            for (int v = kFlagChunk - 1; v >= 0; v--) hit = lv[v] >= my ? d + v : hit;
+            return min(hit, scan_size);
+          }
         }
-      }
-      int rsize = scan_size;
-      for (int base = 1; base <= scan_size; base += kFlagChunk) {
-        int lv[kFlagChunk];
-#pragma unroll
-        for (int u = 0; u < kFlagChunk; u++) lv[u] = here[base + u];
-        const int room = scan_size - base;
-        int hit = 0;
-#pragma unroll
-        for (int u = kFlagChunk - 1; u >= 0; u--) hit = ((u <= room) & (lv[u] >= my)) ? base + u : hit;
-        if (hit) {
-          rsize = hit;
-          break;
-        }
-      }
+        return scan_size;
+      };
+      const int lsize = nearest(std::false_type());
+      const int rsize = nearest(std::true_type());
       is_split |= (lsize + rsize > max_size);
       is_split |= (my > lb_l) && (node_idx - lsize <= 0);
       is_split |= (my > lb_r) && (node_idx + rsize >= nump);
@@ -237,7 +255,7 @@ int flag_leaf_impl(hipStream_t s, const void* posz, const uint8_t* ptype, const 
                    const int32_t* seg) {
   int64_t nb = size_part + 1;
   int nload = kFlagSpan + 2 * scan_size + 1;
-  size_t shmem = (size_t)(nload + 2 * kFlagChunk) * sizeof(int32_t) + (size_t)nload;
+  size_t shmem = (size_t)(2 * (nload + 2 * kFlagPad)) * sizeof(int32_t) + (size_t)nload;
   if (shmem > 60000) return fail("FlagLeafBoundaries: scan_size too large");
   hipLaunchKernelGGL((k_flag_leaf<D, T>), dim3((unsigned)div_up(nb, kFlagSpan)), dim3(kTreeThreads), shmem, s,
                      (const T*)posz, ptype, lvl_bound, npart, flags, lvl, size_part, max_size, scan_size, num_types,
