@@ -101,35 +101,70 @@ int scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, bool inc
   return check_launch("scan_i32");
 }
 
-__global__ void k_flags_to_i32(const int8_t* __restrict__ f, int32_t* __restrict__ o, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) o[i] = f[i] != 0;
+// Compaction of int8 flags: where(flags != 0) in index order, as two passes over the flags (the
+// flags are read as 16-byte vectors, 16 per thread, kWhereTile per workgroup): per-tile counts,
+// their exclusive scan, then each tile writes its indices at its offset plus the in-tile rank.
+constexpr int kWherePer = 16;
+constexpr int kWhereTile = kScanThreads * kWherePer;  // 4096
+
+__device__ __forceinline__ uint32_t load_flags16(const int8_t* __restrict__ f, int64_t i0, int64_t n, bool vec) {
+  // bit k set iff flag i0 + k is nonzero (flags past n read as zero); vec: f is 16-byte aligned
+  uint32_t bits = 0;
+  if (vec && i0 + kWherePer <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(f + i0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) bits |= (((w[q] >> (8 * b)) & 0xffu) != 0u ? 1u : 0u) << (4 * q + b);
+  } else {
+    for (int k = 0; k < kWherePer && i0 + k < n; k++) bits |= (f[i0 + k] != 0 ? 1u : 0u) << k;
+  }
+  return bits;
 }
 
-__global__ void k_where_scatter(const int8_t* __restrict__ f, const int32_t* __restrict__ ex, int64_t n,
-                                int32_t* __restrict__ out, int64_t size, int32_t fill, const int32_t* fill_dev,
-                                int32_t* count_out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (fill_dev) fill = fill_dev[0];
-  int32_t total = ex[n - 1] + (f[n - 1] != 0);
-  if (i < n && f[i] != 0) {
-    int32_t j = ex[i];
-    if (j < size) out[j] = (int32_t)i;
+__global__ __launch_bounds__(kScanThreads) void k_where_count(const int8_t* __restrict__ f, int64_t n,
+                                                              int32_t* __restrict__ tile_counts) {
+  __shared__ int sbuf[kScanThreads / 64];
+  const int64_t i0 = (int64_t)blockIdx.x * kWhereTile + (int64_t)threadIdx.x * kWherePer;
+  const int c = __builtin_popcount(load_flags16(f, i0, n, ((uintptr_t)f & 15u) == 0));
+  int tot;
+  block_excl_scan<kScanThreads>(c, sbuf, &tot);
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_where_write(const int8_t* __restrict__ f, int64_t n, int64_t ntiles,
+                                                              const int32_t* __restrict__ tile_off,
+                                                              int32_t* __restrict__ out, int64_t size, int32_t fill,
+                                                              const int32_t* fill_dev, int32_t* count_out) {
+  __shared__ int sbuf[kScanThreads / 64];
+  const int32_t total = tile_off[ntiles];
+  if (blockIdx.x < ntiles) {
+    const int64_t i0 = (int64_t)blockIdx.x * kWhereTile + (int64_t)threadIdx.x * kWherePer;
+    uint32_t bits = load_flags16(f, i0, n, ((uintptr_t)f & 15u) == 0);
+    int tot;
+    int j = block_excl_scan<kScanThreads>(__builtin_popcount(bits), sbuf, &tot) + tile_off[blockIdx.x];
+    for (; bits; bits &= bits - 1u, j++)
+      if (j < size) out[j] = (int32_t)(i0 + __builtin_ctz(bits));
   }
-  // the remaining slots get the fill value
-  if (i < size && i >= total) out[i] = fill;
-  if (i == 0 && count_out) *count_out = total;
+  // the slots past the count get the fill value
+  if (fill_dev) fill = fill_dev[0];
+  const int64_t t0 = (int64_t)blockIdx.x * kWhereTile;
+  for (int64_t i = t0 + threadIdx.x; i < t0 + kWhereTile && i < size; i += kScanThreads)
+    if (i >= total) out[i] = fill;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && count_out) *count_out = total;
 }
 
 int where_flags_i8(hipStream_t s, const int8_t* flags, int64_t n, int32_t* out, int64_t size, int32_t fill,
                    const int32_t* fill_dev, int32_t* count_out, int32_t* tmp) {
   if (n <= 0) return fail("where_flags_i8: empty input");
-  int32_t* ex = tmp + scan_tmp_words(n);
-  hipLaunchKernelGGL(k_flags_to_i32, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, flags, ex, n);
-  if (scan_i32(s, ex, ex, n, false, tmp)) return 1;
-  int64_t m = n > size ? n : size;
-  hipLaunchKernelGGL(k_where_scatter, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, flags, ex, n, out, size,
-                     fill, fill_dev, count_out);
+  // tmp: ntiles + 1 words (at most scan_tmp_words(n), what callers provide)
+  const int64_t ntiles = div_up(n, kWhereTile);
+  hipLaunchKernelGGL(k_where_count, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, flags, n, tmp);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, tmp, ntiles);
+  const int64_t grid = ntiles > div_up(size, kWhereTile) ? ntiles : div_up(size, kWhereTile);
+  hipLaunchKernelGGL(k_where_write, dim3((unsigned)(grid > 0 ? grid : 1)), dim3(kScanThreads), 0, s, flags, n, ntiles,
+                     tmp, out, size, fill, fill_dev, count_out);
   return check_launch("where_flags_i8");
 }
 
