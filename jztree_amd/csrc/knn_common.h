@@ -92,19 +92,20 @@ __device__ __forceinline__ void ce_asc(uint64_t& a, uint64_t& b) {
   b = f64_as_key(__builtin_fmax(x, y));
 }
 
-// ascending bitonic sort of a register array (N a power of two)
+// ascending sort of a register array by Batcher's odd-even merge sort (N a power of two; 63
+// compare-exchanges for N = 16 against 80 for the bitonic network): for each merge width 2p, merge
+// steps of distance k = p, p/2, ..., 1 compare i+j with i+j+k inside one 2p-block.
 template <int N>
-__device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[N]) {
+__device__ __forceinline__ void oddeven_sort_regs(uint64_t (&v)[N]) {
 #pragma unroll
-  for (int size = 2; size <= N; size <<= 1) {
+  for (int p = 1; p < N; p <<= 1) {
 #pragma unroll
-    for (int stride = size / 2; stride > 0; stride >>= 1) {
+    for (int k = p; k >= 1; k >>= 1) {
 #pragma unroll
-      for (int i = 0; i < N; i++) {
-        const int j = i ^ stride;
-        if (j > i) {
-          if ((i & size) == 0) ce_asc(v[i], v[j]);
-          else ce_asc(v[j], v[i]);
+      for (int j = k % p; j <= N - 1 - k; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          if (i <= k - 1 && i <= N - j - k - 1 && (i + j) / (2 * p) == (i + j + k) / (2 * p)) ce_asc(v[i + j], v[i + j + k]);
         }
       }
     }
@@ -112,15 +113,18 @@ __device__ __forceinline__ void bitonic_sort_regs(uint64_t (&v)[N]) {
 }
 
 // Merge the lane's buffered keys row[0..cnt) into the ascending best[0..K): best becomes the K
-// smallest of the union, ascending.  cnt is reset.
+// smallest of the union, ascending.  cnt is reset.  The row is read whole (slots past cnt hold
+// stale keys, replaced by empty ones), so the reads need no per-slot branch.
 template <int K>
 __device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* row, int& cnt) {
   constexpr int M = kBufCap<K>;
   uint64_t b[M];
 #pragma unroll
-  for (int i = 0; i < M; i++) b[i] = i < cnt ? row[i] : kEmptyKey;
+  for (int i = 0; i < M; i++) b[i] = row[i];
+#pragma unroll
+  for (int i = 0; i < M; i++) b[i] = i < cnt ? b[i] : kEmptyKey;
   cnt = 0;
-  bitonic_sort_regs<M>(b);
+  oddeven_sort_regs<M>(b);
   // half-cleaner of best ++ reverse(b padded to K): the low half holds the K smallest, bitonic
 #pragma unroll
   for (int i = 0; i < M; i++) {
