@@ -267,7 +267,6 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   __shared__ uint32_t s_lstart[kRadix];
   __shared__ uint32_t s_gbase[kRadix];
   __shared__ uint64_t s_keys[(kSortThreads * KPT)];
-  __shared__ uint8_t s_dig[(kSortThreads * KPT)];
   __shared__ int s_scan[kSortThreads / 64];
   __shared__ uint32_t s_tile;
 
@@ -341,31 +340,38 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
   s_gbase[d] = bucket_off[d] + excl - lstart;
   __syncthreads();
 
-  // scatter keys into LDS in tile-local digit order
+  // scatter keys into LDS in tile-local digit order (each key's local slot kept for its value)
+  uint32_t slot[KPT];
 #pragma unroll
   for (int r = 0; r < KPT; r++) {
     uint32_t dd = dig[r];
-    uint32_t p = s_lstart[dd] + s_wcnt[w][dd] + rank[r];
-    s_keys[p] = key[r];
-    s_dig[p] = (uint8_t)dd;
+    slot[r] = s_lstart[dd] + s_wcnt[w][dd] + rank[r];
+    s_keys[slot[r]] = key[r];
   }
   __syncthreads();
+  // write out in local order; a slot's digit comes from its key, its global index is kept for
+  // the value pass (slot i = tid + r * kSortThreads)
   const int tile_n = (int)((n - base) < (kSortThreads * KPT) ? (n - base) : (kSortThreads * KPT));
-  for (int i = tid; i < tile_n; i += kSortThreads) {
-    const uint32_t g = s_gbase[s_dig[i]] + (uint32_t)i;
-    if (g < (uint64_t)n) kout[g] = s_keys[i];
+  uint32_t gidx[KPT];
+#pragma unroll
+  for (int r = 0; r < KPT; r++) {
+    const int i = tid + r * kSortThreads;
+    gidx[r] = 0xffffffffu;
+    if (i < tile_n) {
+      const uint64_t kk = s_keys[i];
+      gidx[r] = s_gbase[(uint32_t)(kk >> shift) & 255u] + (uint32_t)i;
+      if (gidx[r] < (uint64_t)n) kout[gidx[r]] = kk;
+    }
   }
   __syncthreads();
   uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
-  for (int r = 0; r < KPT; r++) {
-    uint32_t dd = dig[r];
-    s_vals[s_lstart[dd] + s_wcnt[w][dd] + rank[r]] = val[r];
-  }
+  for (int r = 0; r < KPT; r++) s_vals[slot[r]] = val[r];
   __syncthreads();
-  for (int i = tid; i < tile_n; i += kSortThreads) {
-    const uint32_t g = s_gbase[s_dig[i]] + (uint32_t)i;
-    if (g < (uint64_t)n) vout[g] = s_vals[i];
+#pragma unroll
+  for (int r = 0; r < KPT; r++) {
+    const int i = tid + r * kSortThreads;
+    if (i < tile_n && gidx[r] < (uint64_t)n) vout[gidx[r]] = s_vals[i];
   }
 }
 
