@@ -381,6 +381,14 @@ int jzt_fof_node2node(jzt_stream_t stream, const int32_t* parent_ilist_spl, cons
 int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist, int64_t ilist_size,
                       const int32_t* spl, const void* pos, const int32_t* part_igroup_in, int32_t* part_igroup,
                       int64_t size_leaves, int64_t size_part, float r2link, float boxsize, int dim, int dtype);
+/* FofLeaf2Leaf with the leaves' cells (extension, no counterpart in the reference): leaf_cells
+ * holds (centre[dim], level bits) per leaf as jzt_fof_node2node's nodes do.  A list entry whose
+ * cell lies beyond rlink of a query chunk's bounding box (and needs no periodic wrap) is dropped
+ * without loading its particles.  Same labels as jzt_fof_leaf2leaf. */
+int jzt_fof_leaf2leaf_cells(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist, int64_t ilist_size,
+                            const int32_t* spl, const void* pos, const void* leaf_cells,
+                            const int32_t* part_igroup_in, int32_t* part_igroup, int64_t size_leaves,
+                            int64_t size_part, float r2link, float boxsize, int dim, int dtype);
 
 /* FoF over a batch of sets, each with its own linking length (no counterpart in the reference;
  * see jzt_pos_zorder_sort_segmented).  As the three calls above with the scalar r2link replaced

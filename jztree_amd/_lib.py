@@ -76,6 +76,7 @@ _SIGNATURES = {
     "jzt_node_to_child_label": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _f32, _i32, _i32], _i32),
     "jzt_fof_node2node": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
     "jzt_fof_leaf2leaf": ([_p, _p, _p, _i64, _p, _p, _p, _p, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
+    "jzt_fof_leaf2leaf_cells": ([_p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _i64, _f32, _f32, _i32, _i32], _i32),
     "jzt_node_to_child_label_sets": ([_p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32], _i32),
     "jzt_fof_node2node_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _f32, _i32, _i32],
                                _i32),

@@ -354,7 +354,7 @@ template <bool PER, int D, typename T>
 __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_spl, const int32_t* __restrict__ il,
                                                  const int32_t* __restrict__ spl, const T* __restrict__ pos, int32_t* ig,
                                                  float r2link_all, const float* __restrict__ r2v, T boxsize,
-                                                 int64_t il_size, int64_t nitems, int xcd) {
+                                                 int64_t il_size, int64_t nitems, int xcd, const T* __restrict__ cells) {
   constexpr int kQueue = 256;
   __shared__ FofRec<D, T> s_r[64];
   __shared__ int2 s_q[kQueue];
@@ -390,11 +390,41 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
       // labels known to be in q's set: its own label and the last two labels it was queued with
       int gq = ig[iq];
       int glast = -1, glast2 = -1;
-      for (int e = e0; e < e1; e++) {
-        const int lt = il[e];
-        if (lt < lq) continue;  // each leaf pair once
-        // a leaf with itself: only its later chunks (a chunk's own links are k_fof_chunks')
-        const int t0 = lt == lq ? qoff + 64 : spl[lt], t1 = spl[lt + 1];
+      for (int eb = e0; eb < e1; eb += 64) {
+        // list entries 64 at a time, one per lane: the source leaf and its particle range (each
+        // leaf pair once; a leaf with itself only from its chunk after this one, a chunk's own
+        // links being k_fof_chunks'), and with the leaves' cells a test of the cell against the
+        // query chunk's box: the cell (widened by 2^-20 of its coordinates against the rounding
+        // of centre +- half extent) bounds every |fl(q - p)| from below by monotone rounding, so
+        // an entry whose gap is beyond reach is dropped (only where no periodic wrap is needed)
+        const int ie = eb + lane;
+        int m0 = 0, m1 = 0;
+        bool keep = false;
+        if (ie < e1) {
+          const int lt = il[ie];
+          if (lt >= lq) {
+            m0 = lt == lq ? qoff + 64 : spl[lt];
+            m1 = spl[lt + 1];
+            keep = m0 < m1;
+            if (keep && cells && lt != lq) {
+              const NodeX<D, T> nx = load_nodex<D, T>(cells, lt);
+              bool wrap_c = false;
+              T g2 = T(0);
+#pragma unroll
+              for (int d = 0; d < D; d++) {
+                const T eps = (abs(nx.c[d]) + nx.h[d]) * T(1.0 / 1048576.0);
+                const T lo = nx.c[d] - nx.h[d] - eps, hi = nx.c[d] + nx.h[d] + eps;
+                if (PER) wrap_c |= !(qlo[d] - hi >= -half) || !(qhi[d] - lo < half);
+                const T gap = max(T(0), max(lo - qhi[d], qlo[d] - hi));
+                g2 = fmaT(gap, gap, g2);
+              }
+              keep = wrap_c || g2 <= r2skip;
+            }
+          }
+        }
+        for (uint64_t em = __ballot(keep); em; em &= em - 1ull) {
+        const int E = __builtin_ctzll(em);
+        const int t0 = __builtin_amdgcn_readlane(m0, E), t1 = __builtin_amdgcn_readlane(m1, E);
         for (int toff = t0; toff < t1; toff += 64) {
           const int it = toff + lane;
           FofRec<D, T> r;
@@ -514,6 +544,7 @@ __global__ __launch_bounds__(64) void k_fof_leaf(const int32_t* __restrict__ il_
           else by_label(std::integral_constant<bool, false>());
           __syncthreads();
         }
+        }
       }
     }
   }
@@ -561,7 +592,7 @@ int fof_node_impl(hipStream_t s, const int32_t* pil_spl, const int32_t* pil, con
 template <int D, typename T>
 int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64_t il_size, const int32_t* spl,
                   const void* pos, const int32_t* ig_in, int32_t* ig, int64_t size_leaves, int64_t size_part,
-                  float r2link, const float* r2v, float boxsize) {
+                  float r2link, const float* r2v, float boxsize, const void* cells) {
   if (ig != ig_in && hipMemcpyAsync(ig, ig_in, (size_t)size_part * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return fail("FofLeaf2Leaf: copy failed");
   if (size_leaves > 0) {
@@ -574,10 +605,10 @@ int fof_leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il, int64
                          ig, r2link, r2v, (T)boxsize, size_leaves, xcd);
     if (boxsize > 0.f)
       hipLaunchKernelGGL((k_fof_leaf<true, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd);
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd, (const T*)cells);
     else
       hipLaunchKernelGGL((k_fof_leaf<false, D, T>), dim3(grid_for8(size_leaves)), dim3(64), 0, s, il_spl, il, spl,
-                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd);
+                         (const T*)pos, ig, r2link, r2v, (T)boxsize, il_size, size_leaves, xcd, (const T*)cells);
   }
   if (size_part > 0)
     hipLaunchKernelGGL(k_contract, dim3((unsigned)div_up(size_part, 256)), dim3(256), 0, s, ig, size_part);
@@ -640,7 +671,16 @@ extern "C" int jzt_fof_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, 
                                  int32_t* part_igroup, int64_t size_leaves, int64_t size_part, float r2link,
                                  float boxsize, int dim, int dtype) {
   JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
-                size_leaves, size_part, r2link, (const float*)nullptr, boxsize);
+                size_leaves, size_part, r2link, (const float*)nullptr, boxsize, (const void*)nullptr);
+}
+
+extern "C" int jzt_fof_leaf2leaf_cells(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist,
+                                       int64_t ilist_size, const int32_t* spl, const void* pos, const void* leaf_cells,
+                                       const int32_t* part_igroup_in, int32_t* part_igroup, int64_t size_leaves,
+                                       int64_t size_part, float r2link, float boxsize, int dim, int dtype) {
+  if (leaf_cells == nullptr) return fail("FofLeaf2Leaf (cells): leaf_cells is required");
+  JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
+                size_leaves, size_part, r2link, (const float*)nullptr, boxsize, leaf_cells);
 }
 
 extern "C" int jzt_fof_leaf2leaf_sets(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist,
@@ -649,5 +689,5 @@ extern "C" int jzt_fof_leaf2leaf_sets(jzt_stream_t stream, const int32_t* ilist_
                                       int64_t size_part, float boxsize, int dim, int dtype) {
   if (leaf_r2link == nullptr) return fail("FofLeaf2Leaf (sets): leaf_r2link is required");
   JZT_DISPATCH2(fof_leaf_impl, (hipStream_t)stream, ilist_spl, ilist, ilist_size, spl, pos, part_igroup_in, part_igroup,
-                size_leaves, size_part, 0.f, leaf_r2link, boxsize);
+                size_leaves, size_part, 0.f, leaf_r2link, boxsize, (const void*)nullptr);
 }

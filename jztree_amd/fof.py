@@ -39,10 +39,12 @@ class PosLvl:
 
 @dataclass
 class FofNodeData:
-    """data.py:566-570"""
+    """data.py:566-570.  cent (extension): the nodes' cell centres, when known (the leaf kernel
+    then drops list entries whose cell is out of reach of a query chunk)."""
     lvl: torch.Tensor
     igroup: torch.Tensor
     spl: torch.Tensor
+    cent: torch.Tensor | None = None
 
 
 def _r2link(rlink: float) -> float:
@@ -147,7 +149,7 @@ def _fof_dual_walk(th: TreeHierarchy, rlink: float, boxsize: float = 0., alloc_f
                                       dim=dim, dtype=th.geom_cent.data.dtype)
         node_data = PosLvl(pos=th.geom_cent.get(level, size), lvl=nlvl.get(level, size))
         igroup, ilist = _fof_node2node(ilist, parent_spl, node_data, igroup, rlink=rl, boxsize=boxsize)
-    node_data = FofNodeData(nlvl.get(0, size), igroup, spl_n2n.get(0, size + 1))
+    node_data = FofNodeData(nlvl.get(0, size), igroup, spl_n2n.get(0, size + 1), cent=th.geom_cent.get(0, size))
     return node_data, ilist
 
 
@@ -169,6 +171,12 @@ def _fof_leaf2leaf(leaf_data: FofNodeData, ilist: InteractionList, posz, rlink: 
                L.ptr(leaf_data.spl.contiguous()), L.ptr(_r2_items(leaf_r2link, len(leaf_data.spl) - 1)),
                L.ptr(posz.contiguous()), L.ptr(part_igroup), L.ptr(out), len(leaf_data.spl) - 1, len(posz),
                float(boxsize or 0.), posz.shape[-1], L.dtype_code(posz.dtype))
+    elif leaf_data.cent is not None and leaf_data.cent.dtype == posz.dtype:
+        cells = PosLvl(pos=leaf_data.cent, lvl=leaf_data.lvl).pos_lvl()
+        L.call("jzt_fof_leaf2leaf_cells", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
+               L.ptr(leaf_data.spl.contiguous()), L.ptr(posz.contiguous()), L.ptr(cells), L.ptr(part_igroup),
+               L.ptr(out), len(leaf_data.spl) - 1, len(posz), _r2link(rlink), float(boxsize or 0.), posz.shape[-1],
+               L.dtype_code(posz.dtype))
     else:
         L.call("jzt_fof_leaf2leaf", L.stream(), L.ptr(ilist.ispl), L.ptr(ilist.isrc), ilist.size(),
                L.ptr(leaf_data.spl.contiguous()),
