@@ -300,22 +300,27 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
           if (wrap) pairs(std::integral_constant<bool, true>());
           else pairs(std::integral_constant<bool, false>());
         }
-        uint64_t c = ok ? (nb | (1ull << lane)) : 0ull;
-        // only particles with a neighbour can be in another's set: the closure runs over them
-        const uint64_t act = __ballot(nb != 0ull);
-        bool changed = act != 0ull;
-        while (wave_any(changed)) {
-          uint64_t cn = c;
-          for (uint64_t a = act; a; a &= a - 1ull) {
-            const int j = __builtin_ctzll(a);
-            const uint64_t cj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c >> 32), j) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c, j);
-            if ((c >> j) & 1ull) cn |= cj;
+        // Connected components by breadth-first search over the neighbour rows, one component at a
+        // time from its lowest particle (wave-uniform: a frontier's rows are read with readlane
+        // and or-ed in scalar registers), so every particle's row is read once.  Only particles
+        // with a neighbour take part; the root of a component is its lowest particle.
+        uint64_t todo = __ballot(nb != 0ull);
+        while (todo) {
+          const int root = __builtin_ctzll(todo);
+          uint64_t comp = 1ull << root, frontier = comp;
+          while (frontier) {
+            uint64_t next = 0ull;
+            for (uint64_t f = frontier; f; f &= f - 1ull) {
+              const int j = __builtin_ctzll(f);
+              next |= ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(nb >> 32), j) << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)nb, j);
+            }
+            frontier = next & ~comp;
+            comp |= next;
           }
-          changed = cn != c;
-          c = cn;
+          if ((comp >> lane) & 1ull) lroot = root;
+          todo &= ~comp;
         }
-        lroot = ok ? __builtin_ctzll(c) : lane;
       }
       __syncthreads();
       const int groot = __shfl(g, lroot, 64);  // label of the component's first particle
