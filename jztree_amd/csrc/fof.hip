@@ -292,7 +292,8 @@ __global__ __launch_bounds__(64) void k_fof_chunks(const int32_t* __restrict__ s
               for (int d = 0; d < D; d++) xp[d] = rr[u].p[d];
               const int j = j0 + u;
               const T r2 = dist2P<W, D, T>(x, xp, boxsize, half);
-              if (j < n && j != lane && r2 <= (T)r2link) nb |= 1ull << j;
+              const bool h = (j < n) & (j != lane) & (r2 <= (T)r2link);
+              nb |= (uint64_t)h << (j & 63);
             }
           }
         };
