@@ -149,6 +149,15 @@ int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const int32_t* l
                           int64_t size_nodes, int64_t size_part, int dim, int dtype, int lvl_invalid,
                           uint32_t mode_flags, int upper_extent);
 
+/* CenterOfMass -- replaces ffi target "CenterOfMass" (kernel tree.cuh:378-429, caller
+ * tree.py:479-499 and :736-754).  Per node i < nnodes: the mass-weighted centre of the points
+ * pos[isplit[i] .. isplit[i+1]) (row-major (n, dim)) with weights mass[] and their total mass,
+ * written as com_out[i*(dim+1) .. ] = (centre[dim], mass); Kahan-compensated sums in point order
+ * when kahan != 0.  Empty ranges and non-positive totals give NaN rows.  (The reference handles
+ * float32 3-D; float64 and 2-D are instantiated too.) */
+int jzt_center_of_mass(jzt_stream_t stream, const int32_t* isplit, const void* pos, const void* mass, void* com_out,
+                       int64_t nnodes, int kahan, int dim, int dtype);
+
 /* jzt_geometry_to_f32 -- no counterpart in the reference (extension for float64 positions with
  * float32 distance kernels).  Per node of a float64 tree (centres cent (n, dim), levels lvl): the
  * smallest enclosing Morton cell whose boundaries and centre are float32 numbers (per-dim side at

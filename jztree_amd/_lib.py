@@ -40,6 +40,7 @@ _SIGNATURES = {
     "jzt_flag_leaf_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
     "jzt_find_node_boundaries_sets": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32], _i32),
     "jzt_get_node_geometry": ([_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i32, _i32, _i32, _u32, _i32], _i32),
+    "jzt_center_of_mass": ([_p, _p, _p, _p, _p, _i64, _i32, _i32, _i32], _i32),
     "jzt_geometry_to_f32": ([_p, _p, _p, _i64, _i32, _i32, _p, _p], _i32),
     "jzt_where_tmp_bytes": ([_i64], _sz),
     "jzt_where_flags": ([_p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _sz], _i32),

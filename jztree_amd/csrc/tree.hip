@@ -450,6 +450,58 @@ extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl,
   return check_launch("regularization level");
 }
 
+// CenterOfMass (tree.cuh:378-429): per node, the mass-weighted mean of the points of its range
+// [isplit[i], isplit[i+1]) and their total mass, out = (centre[D], mass) per node.  One thread per
+// node sums its points in order, with Kahan compensation when `kahan` (the reference's default):
+// y = m * x - c for a coordinate (formed as one fused multiply-add, the form a contracting
+// compiler gives the reference's m * x followed by "add - c"), y = m - c for the mass, then
+// t = sum + y, c = (t - sum) - y, sum = t.  The centre is (1 / mass) * sum.  An empty range or a
+// non-positive total mass gives NaN in every slot.
+template <int D, typename T>
+__global__ void k_center_of_mass(const int32_t* __restrict__ isplit, const T* __restrict__ pos,
+                                 const T* __restrict__ mass, T* __restrict__ out, int64_t nnodes, int kahan) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  const int s0 = isplit[i], s1 = isplit[i + 1];
+  const T nan = (T)__builtin_nan("");
+  T sum[D + 1], c[D + 1];
+#pragma unroll
+  for (int d = 0; d <= D; d++) sum[d] = c[d] = T(0);
+  for (int p = s0; p < s1; p++) {
+    const T m = mass[p];
+#pragma unroll
+    for (int d = 0; d <= D; d++) {
+      const T x = d < D ? pos[(int64_t)p * D + d] : T(1);
+      if (kahan) {
+        const T y = d < D ? fmaT(m, x, -c[d]) : m - c[d];
+        const T t = sum[d] + y;
+        c[d] = (t - sum[d]) - y;
+        sum[d] = t;
+      } else {
+        sum[d] = d < D ? fmaT(m, x, sum[d]) : sum[d] + m;
+      }
+    }
+  }
+  if (s0 < s1 && sum[D] > T(0)) {
+    const T inv = T(1) / sum[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) out[i * (D + 1) + d] = inv * sum[d];
+    out[i * (D + 1) + D] = sum[D];
+  } else {
+#pragma unroll
+    for (int d = 0; d <= D; d++) out[i * (D + 1) + d] = nan;
+  }
+}
+
+template <int D, typename T>
+int com_impl(hipStream_t s, const int32_t* isplit, const void* pos, const void* mass, void* out, int64_t nnodes,
+             int kahan) {
+  if (nnodes <= 0) return 0;
+  hipLaunchKernelGGL((k_center_of_mass<D, T>), dim3((unsigned)div_up(nnodes, 256)), dim3(256), 0, s, isplit,
+                     (const T*)pos, (const T*)mass, (T*)out, nnodes, kahan);
+  return check_launch("CenterOfMass");
+}
+
 #define JZT_DISPATCH(fn, ...)                                                     \
   do {                                                                            \
     if (dtype == JZT_F32 && dim == 3) return fn<3, float>(__VA_ARGS__);           \
@@ -512,4 +564,9 @@ extern "C" int jzt_where_flags(jzt_stream_t stream, const int8_t* flags, int64_t
                                int32_t fill, const int32_t* fill_dev, int32_t* count, void* tmp, size_t tmp_bytes) {
   if (tmp_bytes < jzt_where_tmp_bytes(n)) return fail("jzt_where_flags: tmp too small");
   return where_flags_i8((hipStream_t)stream, flags, n, out, size, fill, fill_dev, count, (int32_t*)tmp);
+}
+
+extern "C" int jzt_center_of_mass(jzt_stream_t stream, const int32_t* isplit, const void* pos, const void* mass,
+                                  void* com_out, int64_t nnodes, int kahan, int dim, int dtype) {
+  JZT_DISPATCH(com_impl, (hipStream_t)stream, isplit, pos, mass, com_out, nnodes, kahan);
 }

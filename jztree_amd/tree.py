@@ -383,20 +383,50 @@ def _define_split_hierarchy(posz, node_sizes, alloc_size, npart=None, ptype=None
     return ispl, ispl_per_type, ispl_n2l, ispl_n2n
 
 
+def center_of_mass(ispl: torch.Tensor, pos: torch.Tensor, mass, kahan_summation: bool = True):
+    """tree.py:479-499 -> CenterOfMass: mass-weighted centre and total mass of the points in every
+    range [ispl[i], ispl[i+1]) (NaN for empty ranges).  mass: per point, or a scalar (broadcast as
+    in the reference).  Returns (pos (n, dim), mass (n,))."""
+    assert ispl.dtype == torch.int32 and pos.ndim == 2
+    dev = pos.device
+    n = ispl.numel() - 1
+    m = torch.as_tensor(mass, dtype=pos.dtype, device=dev)
+    m = torch.broadcast_to(m, pos.shape[:-1]).contiguous()
+    out = torch.empty((max(n, 0), pos.shape[-1] + 1), dtype=pos.dtype, device=dev)
+    L.call("jzt_center_of_mass", L.stream(), L.ptr(ispl.contiguous()), L.ptr(pos.contiguous()), L.ptr(m),
+           L.ptr(out), n, int(bool(kahan_summation)), pos.shape[-1], L.dtype_code(pos.dtype))
+    return out[:, :-1], out[:, -1]
+
+
+def _get_tree_mass_centers(partz, ispl_n2n: PackedArray, prop_array_spl: torch.Tensor):
+    """tree.py:736-754: node mass centres and masses of every plane; plane 0 from the particles,
+    plane p from the mass centres and masses of plane p-1."""
+    pos = get_pos(partz)
+    if getattr(partz, "mass", None) is None:
+        raise ValueError("To use mass centering, please provide PosMass input")
+    cpos, cmass = center_of_mass(ispl_n2n.get(0).to(torch.int32), pos, partz.mass)
+    node_cent = PackedArray.from_data(cpos, ispl=prop_array_spl, fill_values=float("nan"))
+    node_mass = PackedArray.from_data(cmass, ispl=prop_array_spl, fill_values=float("nan"))
+    for i in range(1, ispl_n2n.nlevels()):
+        cpos, cmass = center_of_mass(ispl_n2n.get(i, size=len(cmass) + 1).to(torch.int32), cpos.contiguous(),
+                                     cmass.contiguous())
+        node_mass = node_mass.set(i, cmass, num=ispl_n2n.num(i) - 1)
+        node_cent = node_cent.set(i, cpos, num=ispl_n2n.num(i) - 1)
+    return node_cent, node_mass
+
+
 def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None, num_types=None,
                          set_id: torch.Tensor | None = None, nsets: int = 1) -> TreeHierarchy:
     """Builds a tree hierarchy from z-order positions (tree.py:756-829).
 
     Plane 0 holds leaves of at most cfg_tree.max_leaf_size particles; plane p nodes hold at most
-    max_leaf_size * coarse_fac**p particles.  Geometric centres only (mass-centred trees are not
-    part of this build).
+    max_leaf_size * coarse_fac**p particles.  With cfg_tree.mass_centered, partz carries masses
+    (PosMass) and th.mass / th.mass_cent hold every node's mass and mass centre (CenterOfMass).
 
     set_id, nsets (extension): a batch of nsets independent sets, set_id (int32, non-decreasing)
     naming the set of every row; the result is a forest whose nodes never span two sets, and
     the top-plane interaction list pairs only nodes of one set.  Regularization is skipped for
     batches (it never changes kNN or FoF results, only the leaf shapes)."""
-    if cfg_tree.mass_centered:
-        raise NotImplementedError("mass-centred trees are not part of this build")
     posz = get_pos(partz)
     npart_tot = get_num_total(partz, default_to_length=True)
     np_per_dev = npart_tot
@@ -423,9 +453,12 @@ def build_tree_hierarchy(partz, cfg_tree: TreeConfig, lvl_bound=None, ptype=None
     prop_array_spl = cumsum_starting_with_zero(ispl_n2n.ispl[1:] - ispl_n2n.ispl[:-1] - 1)
     lvl = PackedArray.from_data(lvl, prop_array_spl, fill_values=-2000)
     geom_cent = PackedArray.from_data(geom_cent, prop_array_spl, fill_values=float("nan"))
+    nmass_cent, nmass = None, None
+    if cfg_tree.mass_centered:
+        nmass_cent, nmass = _get_tree_mass_centers(partz, ispl_n2n, prop_array_spl)
     size_leaves = _estimate_node_number(np_per_dev, cfg_tree.max_leaf_size)
-    return TreeHierarchy(size_leaves, ispl_n2n, ispl_n2l, ispl_per_type, lvl, geom_cent, leaf_set=leaf_set,
-                         nsets=int(nsets))
+    return TreeHierarchy(size_leaves, ispl_n2n, ispl_n2l, ispl_per_type, lvl, geom_cent, mass=nmass,
+                         mass_cent=nmass_cent, leaf_set=leaf_set, nsets=int(nsets))
 
 
 build_tree_hierarchy.jit = build_tree_hierarchy

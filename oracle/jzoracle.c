@@ -938,3 +938,42 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* CenterOfMass (tree.cuh:378-429): per node i, the points [isplit[i], isplit[i+1]) in order,
+ * sums of (m * x_d, m) with Kahan compensation when kahan != 0 -- y = fma(m, x_d, -c) for a
+ * coordinate (one rounding, as a contracting compiler forms the reference's m * x followed by
+ * "add - c"), y = m - c for the mass, t = sum + y, c = (t - sum) - y, sum = t -- or plain
+ * sums (sum = fma(m, x_d, sum), sum + m) without.  out row = ((1 / M) * sum_d, M); NaN rows for
+ * empty ranges and M <= 0.  -ffp-contract=off keeps every other operation as written. */
+#define ORC_COM(T, FMA, NAME)                                                                         \
+void NAME(const int32_t* isplit, const T* pos, const T* mass, int64_t nnodes, int dim, int kahan,     \
+          T* out) {                                                                                   \
+    for (int64_t i = 0; i < nnodes; i++) {                                                            \
+        T sum[11], c[11];                                                                             \
+        for (int d = 0; d <= dim; d++) sum[d] = c[d] = (T)0;                                          \
+        const int32_t s0 = isplit[i], s1 = isplit[i + 1];                                             \
+        for (int32_t p = s0; p < s1; p++) {                                                           \
+            const T m = mass[p];                                                                      \
+            for (int d = 0; d <= dim; d++) {                                                          \
+                if (kahan) {                                                                          \
+                    const T y = d < dim ? FMA(m, pos[(int64_t)p * dim + d], -c[d]) : m - c[d];        \
+                    const T t = sum[d] + y;                                                           \
+                    c[d] = (t - sum[d]) - y;                                                          \
+                    sum[d] = t;                                                                       \
+                } else {                                                                              \
+                    sum[d] = d < dim ? FMA(m, pos[(int64_t)p * dim + d], sum[d]) : sum[d] + m;         \
+                }                                                                                     \
+            }                                                                                         \
+        }                                                                                             \
+        T* o = out + i * (dim + 1);                                                                   \
+        if (s0 < s1 && sum[dim] > (T)0) {                                                             \
+            const T inv = (T)1 / sum[dim];                                                            \
+            for (int d = 0; d < dim; d++) o[d] = inv * sum[d];                                        \
+            o[dim] = sum[dim];                                                                        \
+        } else {                                                                                      \
+            for (int d = 0; d <= dim; d++) o[d] = (T)NAN;                                             \
+        }                                                                                             \
+    }                                                                                                 \
+}
+ORC_COM(float, fmaf, orc_center_of_mass_f32)
+ORC_COM(double, fma, orc_center_of_mass_f64)

@@ -6,7 +6,10 @@ measured or shipped.
 
 Two layers:
   * ctypes bindings to ``libjzoracle.so`` (``jzoracle.c``): the z-order math, the tree kernels
-    (FlagLeafBoundaries / FindNodeBoundaries / GetNodeGeometry), brute-force and grid kNN, FoF.
+    (FlagLeafBoundaries / FindNodeBoundaries / GetNodeGeometry / CenterOfMass), brute-force and
+    grid kNN, FoF.  CenterOfMass: parity with the reference's compiled bits unpinned (it depends
+    on the FMA contraction its compiler applies); pinned by closed forms, long-double sums and
+    the plane recursion.
   * a numpy restatement of the reference's JAX tree assembly (``src/jztree/tree.py:55-329``,
     ``:600-829``, ``data.py:251-452``), integer-exact.
 
@@ -77,6 +80,8 @@ def lib():
         L.orc_fof_brute_f32.argtypes = fof_args
         L.orc_fof_cells_f32.argtypes = fof_args
         L.orc_num_threads.restype = _c.c_int
+        L.orc_center_of_mass_f32.argtypes = [_i32p, _f32p, _f32p, _c.c_int64, _c.c_int, _c.c_int, _f32p]
+        L.orc_center_of_mass_f64.argtypes = [_i32p, _f64p, _f64p, _c.c_int64, _c.c_int, _c.c_int, _f64p]
         _LIB = L
     return _LIB
 
@@ -549,3 +554,23 @@ def fof_catalogue(pos, counts, npart_min, boxsize=0.0, mass=None, vel=None, v_ra
     out["v_rad"] = vr if v_rad is not None else None
     out["scale_factor"] = sf if zplus1 is not None else None
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# CenterOfMass (tree.cuh:378-429) and the mass-centred planes (tree.py:736-754)
+# ------------------------------------------------------------------------------------------------
+
+def center_of_mass(ispl, pos, mass, kahan=True):
+    """Mass centre and total mass of the points in every range [ispl[i], ispl[i+1]) (C restatement
+    of tree.cuh:378-429; mass broadcast as tree.py:488).  Returns (centres (n, dim), masses (n,))."""
+    pos = np.ascontiguousarray(pos)
+    dt = np.float64 if pos.dtype == np.float64 else np.float32
+    pos = np.ascontiguousarray(pos, dt)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dt), pos.shape[:-1]))
+    isp = np.ascontiguousarray(ispl, np.int32)
+    n, dim = len(isp) - 1, pos.shape[1]
+    out = np.empty((max(n, 0), dim + 1), dt)
+    fn = lib().orc_center_of_mass_f64 if dt == np.float64 else lib().orc_center_of_mass_f32
+    fn(isp, pos, m, n, dim, int(bool(kahan)), out)
+    return out[:, :dim], out[:, dim]
+

@@ -341,3 +341,57 @@ def test_leaf_parent_in_range(oracle):
     nl = th.num(0)
     ref = np.searchsorted(n2c[:th.num(1) + 1], np.arange(nl), side="right") - 1
     assert np.array_equal(lp[:nl], ref)
+
+
+# ---- CenterOfMass (tree.cuh:378-429) ----------------------------------------------------------
+
+def test_center_of_mass_closed_forms(oracle):
+    """Cube corners with equal weights have their centre at the cube's middle (exact in binary);
+    a weighted pair gives the lever rule; scalar masses broadcast; empty ranges and zero total
+    mass give NaN rows (invalid_val)."""
+    corners = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], np.float32) * 0.5 + 0.25
+    c, m = oracle.center_of_mass(np.array([0, 8], np.int32), corners, np.float32(3.0))
+    assert np.array_equal(c, np.full((1, 3), 0.5, np.float32)) and m[0] == 24.0
+    pair = np.array([[0.0, 0.0], [1.0, 0.5]], np.float32)
+    c, m = oracle.center_of_mass(np.array([0, 2], np.int32), pair, np.array([3.0, 1.0], np.float32))
+    assert np.array_equal(c[0], np.array([0.25, 0.125], np.float32)) and m[0] == 4.0
+    for kahan in (True, False):
+        c, m = oracle.center_of_mass(np.array([0, 0, 2, 4], np.int32), np.ones((4, 3), np.float32),
+                                     np.array([1, 1, 0, 0], np.float32), kahan=kahan)
+        assert np.all(np.isnan(c[0])) and np.isnan(m[0])   # empty range
+        assert np.array_equal(c[1], np.ones(3, np.float32)) and m[1] == 2.0
+        assert np.all(np.isnan(c[2])) and np.isnan(m[2])   # zero total mass
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_center_of_mass_kahan_accuracy(oracle, dtype):
+    """Against sums in extended precision: the compensated sums of 2^18 points stay within a few
+    units in the last place of the exact centre, the plain sums drift far beyond that (so the
+    compensation is really applied)."""
+    rng = np.random.default_rng(5)
+    n = 1 << 18
+    x = rng.random((n, 3)).astype(dtype)
+    w = (rng.random(n) * 3 + 0.5).astype(dtype)
+    exact_m = np.sum(w.astype(np.longdouble))
+    exact_c = (np.sum(w.astype(np.longdouble)[:, None] * x.astype(np.longdouble), axis=0) / exact_m)
+    ulp = np.finfo(dtype).eps
+    ck, mk = oracle.center_of_mass(np.array([0, n], np.int32), x, w, kahan=True)
+    cp, mp = oracle.center_of_mass(np.array([0, n], np.int32), x, w, kahan=False)
+    err_k = np.max(np.abs(ck[0].astype(np.longdouble) - exact_c) / np.abs(exact_c))
+    err_p = np.max(np.abs(cp[0].astype(np.longdouble) - exact_c) / np.abs(exact_c))
+    assert abs(float(mk[0]) - float(exact_m)) / float(exact_m) <= 2 * ulp
+    assert err_k <= 4 * ulp
+    assert err_p > 16 * err_k
+
+
+def test_center_of_mass_plane_recursion(oracle):
+    """A node's centre from its children's centres and masses (tree.py:736-754) equals the centre
+    of its points within rounding, and the masses add up."""
+    rng = np.random.default_rng(9)
+    x = rng.random((1000, 3)).astype(np.float64)
+    w = rng.random(1000) + 0.1
+    leaf_spl = np.array([0, 100, 250, 600, 1000], np.int32)
+    c0, m0 = oracle.center_of_mass(leaf_spl, x, w)
+    c1, m1 = oracle.center_of_mass(np.array([0, 2, 4], np.int32), c0, m0)
+    cd, md = oracle.center_of_mass(np.array([0, 250, 1000], np.int32), x, w)
+    assert np.allclose(c1, cd, rtol=0, atol=1e-14) and np.allclose(m1, md, rtol=1e-15)
