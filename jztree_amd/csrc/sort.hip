@@ -990,11 +990,27 @@ extern "C" int jzt_search_sorted_z(jzt_stream_t stream, const void* posz_have, c
 #define JZT_SSZ(D, T)                                                                                            \
   hipLaunchKernelGGL((k_search_sorted_z<D, T>), dim3(g), dim3(256), 0, s, (const T*)posz_have, (const T*)posz_query, \
                      n_have, n_query, leaf, indices)
-  if (dtype == JZT_F32 && dim == 3) JZT_SSZ(3, float);
-  else if (dtype == JZT_F32 && dim == 2) JZT_SSZ(2, float);
-  else if (dtype == JZT_F64 && dim == 3) JZT_SSZ(3, double);
-  else if (dtype == JZT_F64 && dim == 2) JZT_SSZ(2, double);
-  else return fail("jzt_search_sorted_z: unsupported dim/dtype");
+  switch ((dtype == JZT_F64 ? 100 : dtype == JZT_F32 ? 0 : 1000) + dim) {
+    case 2: JZT_SSZ(2, float); break;
+    case 3: JZT_SSZ(3, float); break;
+    case 4: JZT_SSZ(4, float); break;
+    case 5: JZT_SSZ(5, float); break;
+    case 6: JZT_SSZ(6, float); break;
+    case 7: JZT_SSZ(7, float); break;
+    case 8: JZT_SSZ(8, float); break;
+    case 9: JZT_SSZ(9, float); break;
+    case 10: JZT_SSZ(10, float); break;
+    case 102: JZT_SSZ(2, double); break;
+    case 103: JZT_SSZ(3, double); break;
+    case 104: JZT_SSZ(4, double); break;
+    case 105: JZT_SSZ(5, double); break;
+    case 106: JZT_SSZ(6, double); break;
+    case 107: JZT_SSZ(7, double); break;
+    case 108: JZT_SSZ(8, double); break;
+    case 109: JZT_SSZ(9, double); break;
+    case 110: JZT_SSZ(10, double); break;
+    default: return fail("jzt_search_sorted_z: unsupported dim/dtype");
+  }
 #undef JZT_SSZ
   return check_launch("SearchSortedZ");
 }
@@ -1023,13 +1039,19 @@ extern "C" int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void
     return fail("jzt_pos_zorder_sort: tmp buffer too small (have " + std::to_string(tmp_bytes) + ", need " +
                 std::to_string(need) + ")");
   char* t = (char*)tmp;
-  if (dtype == JZT_F32) {
-    if (dim == 3) return zsort_impl<3, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n);
-    if (dim == 2) return zsort_impl<2, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n);
-  } else if (dtype == JZT_F64) {
-    if (dim == 3) return zsort_impl<3, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
-    if (dim == 2) return zsort_impl<2, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
+  // dims 2-10, as the reference instantiates PosZorderSort (_generate_ffi.py:9-21)
+#define JZT_ZS(D)                                                                                                \
+  case D:                                                                                                        \
+    return dtype == JZT_F32                                                                                      \
+               ? zsort_impl<D, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n)         \
+               : zsort_impl<D, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
+  if (dtype == JZT_F32 || dtype == JZT_F64) {
+    switch (dim) {
+      JZT_ZS(2) JZT_ZS(3) JZT_ZS(4) JZT_ZS(5) JZT_ZS(6) JZT_ZS(7) JZT_ZS(8) JZT_ZS(9) JZT_ZS(10)
+      default: break;
+    }
   }
+#undef JZT_ZS
   return fail("jzt_pos_zorder_sort: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
 }
 
@@ -1079,7 +1101,7 @@ extern "C" int jzt_pos_zorder_sort_noalloc(jzt_stream_t stream, const void* pos_
                                            int64_t n, int dim, int dtype) {
   if (n < 0) return fail("jzt_pos_zorder_sort_noalloc: negative n");
   if (n == 0) return 0;
-  if ((dim != 2 && dim != 3) || (dtype != JZT_F32 && dtype != JZT_F64))
+  if (dim < 2 || dim > 10 || (dtype != JZT_F32 && dtype != JZT_F64))
     return fail("jzt_pos_zorder_sort_noalloc: unsupported dim/dtype (dim " + std::to_string(dim) + ")");
   if (n >= (int64_t)kCountMask) return fail("jzt_pos_zorder_sort_noalloc: n must be < 2^30");
   const size_t need = sort_layout(n, dim, dtype).total;

@@ -502,6 +502,33 @@ int com_impl(hipStream_t s, const int32_t* isplit, const void* pos, const void* 
   return check_launch("CenterOfMass");
 }
 
+// single-set tree kernels: dims 2-10 (the reference's instances, _generate_ffi.py:9-21)
+#define JZT_DISPATCH_ND(fn, ...)                                                  \
+  do {                                                                            \
+    switch ((dtype == JZT_F64 ? 100 : 0) + dim) {                                 \
+      case 2: return fn<2, float>(__VA_ARGS__);                                   \
+      case 3: return fn<3, float>(__VA_ARGS__);                                   \
+      case 4: return fn<4, float>(__VA_ARGS__);                                   \
+      case 5: return fn<5, float>(__VA_ARGS__);                                   \
+      case 6: return fn<6, float>(__VA_ARGS__);                                   \
+      case 7: return fn<7, float>(__VA_ARGS__);                                   \
+      case 8: return fn<8, float>(__VA_ARGS__);                                   \
+      case 9: return fn<9, float>(__VA_ARGS__);                                   \
+      case 10: return fn<10, float>(__VA_ARGS__);                                 \
+      case 102: return fn<2, double>(__VA_ARGS__);                                \
+      case 103: return fn<3, double>(__VA_ARGS__);                                \
+      case 104: return fn<4, double>(__VA_ARGS__);                                \
+      case 105: return fn<5, double>(__VA_ARGS__);                                \
+      case 106: return fn<6, double>(__VA_ARGS__);                                \
+      case 107: return fn<7, double>(__VA_ARGS__);                                \
+      case 108: return fn<8, double>(__VA_ARGS__);                                \
+      case 109: return fn<9, double>(__VA_ARGS__);                                \
+      case 110: return fn<10, double>(__VA_ARGS__);                               \
+      default: break;                                                             \
+    }                                                                             \
+    return fail(std::string(#fn) + ": unsupported dim/dtype");                    \
+  } while (0)
+
 #define JZT_DISPATCH(fn, ...)                                                     \
   do {                                                                            \
     if (dtype == JZT_F32 && dim == 3) return fn<3, float>(__VA_ARGS__);           \
@@ -516,8 +543,8 @@ extern "C" int jzt_flag_leaf_boundaries(jzt_stream_t stream, const void* posz, c
                                         int32_t* lvl, int64_t size_part, int dim, int dtype, int max_size,
                                         int scan_size, int num_types) {
   if (num_types > 1 && ptype == nullptr) return fail("FlagLeafBoundaries: ptype required for num_types > 1");
-  JZT_DISPATCH(flag_leaf_impl, (hipStream_t)stream, posz, ptype, lvl_bound, npart, split_flags, lvl, size_part,
-               max_size, scan_size, num_types, (const int32_t*)nullptr);
+  JZT_DISPATCH_ND(flag_leaf_impl, (hipStream_t)stream, posz, ptype, lvl_bound, npart, split_flags, lvl, size_part,
+                  max_size, scan_size, num_types, (const int32_t*)nullptr);
 }
 
 extern "C" int jzt_flag_leaf_boundaries_sets(jzt_stream_t stream, const void* posz, const int32_t* set_id,
@@ -535,8 +562,8 @@ extern "C" int jzt_find_node_boundaries(jzt_stream_t stream, const void* pos_in,
                                         int lvl_invalid) {
   (void)lvl_max;  // the reference passes it but its kernel never reads it outside dead stores (tree.cuh:144-196)
   if (size_nodes <= 0) return 0;
-  JZT_DISPATCH(find_node_impl, (hipStream_t)stream, pos_in, pos_boundary, nleaves, nodes_levels, nodes_lbound,
-               nodes_rbound, size_nodes, lvl_invalid, (const int32_t*)nullptr);
+  JZT_DISPATCH_ND(find_node_impl, (hipStream_t)stream, pos_in, pos_boundary, nleaves, nodes_levels, nodes_lbound,
+                  nodes_rbound, size_nodes, lvl_invalid, (const int32_t*)nullptr);
 }
 
 extern "C" int jzt_find_node_boundaries_sets(jzt_stream_t stream, const void* pos_in, const int32_t* set_id,
@@ -554,8 +581,8 @@ extern "C" int jzt_get_node_geometry(jzt_stream_t stream, const void* pos, const
                                      int64_t size_nodes, int64_t size_part, int dim, int dtype, int lvl_invalid,
                                      uint32_t mode_flags, int upper_extent) {
   if (size_nodes <= 0) return 0;
-  JZT_DISPATCH(geometry_impl, (hipStream_t)stream, pos, lbound, rbound, nnodes, level, center, extent, size_nodes,
-               size_part, lvl_invalid, mode_flags, upper_extent);
+  JZT_DISPATCH_ND(geometry_impl, (hipStream_t)stream, pos, lbound, rbound, nnodes, level, center, extent, size_nodes,
+                  size_part, lvl_invalid, mode_flags, upper_extent);
 }
 
 extern "C" size_t jzt_where_tmp_bytes(int64_t n) { return (size_t)(scan_tmp_words(n) + n) * 4; }
@@ -568,5 +595,5 @@ extern "C" int jzt_where_flags(jzt_stream_t stream, const int8_t* flags, int64_t
 
 extern "C" int jzt_center_of_mass(jzt_stream_t stream, const int32_t* isplit, const void* pos, const void* mass,
                                   void* com_out, int64_t nnodes, int kahan, int dim, int dtype) {
-  JZT_DISPATCH(com_impl, (hipStream_t)stream, isplit, pos, mass, com_out, nnodes, kahan);
+  JZT_DISPATCH_ND(com_impl, (hipStream_t)stream, isplit, pos, mass, com_out, nnodes, kahan);
 }

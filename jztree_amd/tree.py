@@ -54,7 +54,7 @@ def _zsort_impl(x: torch.Tensor, block_size: int = 64):
     """tree.py:75-93 -> PosZorderSort."""
     assert x.ndim == 2
     n, dim = x.shape
-    assert dim in (2, 3), "this build sorts 2D and 3D positions"
+    assert 2 <= dim <= 10, "positions of 2 to 10 dimensions"
     x = x.contiguous()
     pos = torch.empty_like(x)
     ids = torch.empty(n, dtype=same_width_int(x.dtype), device=x.device)

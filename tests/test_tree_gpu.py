@@ -142,3 +142,52 @@ def test_zsort_and_tree_origin_equals_gathered_arange(dev):
     assert o2.idx.dtype == o1.idx.dtype and torch.equal(o1.idx, o2.idx)
     assert torch.equal(pz1, pz2)
     assert torch.equal(th1.ispl_n2n.data, th2.ispl_n2n.data) and torch.equal(th1.lvl.data, th2.lvl.data)
+
+
+@pytest.mark.parametrize("dim", [4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_zsort_high_dims(dev, oracle, dim, dtype):
+    """PosZorderSort in 4-10 dimensions (the reference's instances, _generate_ffi.py:9-21):
+    positions and permutation equal the oracle's exact Morton order, ties by index -- uniform
+    points, a tight clump (most coordinates share their leading bits), duplicates and negative
+    coordinates."""
+    import jztree_amd as jz
+    rng = np.random.default_rng(100 + dim)
+    x = np.concatenate([rng.random((30000, dim)), 0.5 + 1e-5 * rng.random((3000, dim)),
+                        np.repeat(rng.random((50, dim)), 4, axis=0), rng.normal(size=(2000, dim))]).astype(dtype)
+    x = x[rng.permutation(len(x))]
+    print(f"[zsort] dim={dim} dtype={np.dtype(dtype).name} n={len(x)}", flush=True)
+    pz, idz = jz.tree.zsort(torch.from_numpy(x).to(dev))
+    opz, oidz = oracle.zsort(x)
+    assert np.array_equal(idz.cpu().numpy().astype(np.int64), oidz)
+    assert np.array_equal(pz.cpu().numpy(), opz)
+
+
+@pytest.mark.parametrize("dim", [4, 7, 10])
+def test_build_tree_hierarchy_high_dims(dev, oracle, dim):
+    """Leaves, node planes, levels and geometric centres in 4, 7 and 10 dimensions bit-exact
+    against the oracle's assembly (single-set tree kernels are instantiated for dims 2-10), and
+    search_sorted_z keeps the z-order in those dimensions."""
+    import jztree_amd as jz
+    from jztree_amd.config import TreeConfig
+    import dataclasses
+    rng = np.random.default_rng(200 + dim)
+    x = np.concatenate([rng.random((40000, dim)), 0.25 + 0.01 * rng.random((20000, dim))]).astype(np.float32)
+    pz, _ = oracle.zsort(x)
+    cfg = dataclasses.replace(TreeConfig(), alloc_fac_nodes=1.5, regularization=None)
+    print(f"[tree] dim={dim}", flush=True)
+    th = jz.tree.build_tree_hierarchy(_t(pz, dev), cfg)
+    jz.checks.flush()
+    oth = oracle.build_tree_hierarchy(pz, cfg.max_leaf_size, cfg.coarse_fac, cfg.stop_coarsen,
+                                      cfg.alloc_fac_nodes, regularize=None)
+    assert th.size_leaves == oth.size_leaves
+    for name in ("ispl_n2n", "ispl_n2l", "lvl", "geom_cent"):
+        a, b = getattr(th, name), getattr(oth, name)
+        assert np.array_equal(a.ispl.cpu().numpy(), b.ispl), name
+        assert np.array_equal(a.data.cpu().numpy(), b.data, equal_nan=True), name
+    q = rng.random((3000, dim)).astype(np.float32)
+    qz, _ = oracle.zsort(q)
+    ins = jz.tree.search_sorted_z(_t(pz, dev), _t(qz, dev)).cpu().numpy()
+    merged = np.insert(pz, ins, qz, axis=0)
+    mz, _ = oracle.zsort(merged)
+    assert np.array_equal(merged, mz), "insertion at the returned indices breaks the z-order"
