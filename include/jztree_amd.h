@@ -37,7 +37,7 @@ extern "C" {
 
 typedef struct ihipStream_t* jzt_stream_t; /* a hipStream_t */
 
-enum { JZT_F32 = 0, JZT_F64 = 1 };
+enum { JZT_F32 = 0, JZT_F64 = 1, JZT_I32 = 2, JZT_I64 = 3 }; /* integer positions: sort and SearchSortedZ */
 
 /* message of the last failed call on this host thread */
 const char* jzt_last_error(void);

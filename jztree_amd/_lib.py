@@ -12,7 +12,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjztree_amd.so")
-JZT_F32, JZT_F64 = 0, 1
+JZT_F32, JZT_F64, JZT_I32, JZT_I64 = 0, 1, 2, 3
 
 _c = ctypes
 _p = _c.c_void_p
@@ -130,6 +130,15 @@ def dtype_code(dt: torch.dtype) -> int:
     if dt == torch.float64:
         return JZT_F64
     raise TypeError(f"unsupported position dtype {dt}")
+
+
+def sort_dtype_code(dt: torch.dtype) -> int:
+    """Position dtype code of the sort and SearchSortedZ, which also take int32 / int64 positions."""
+    if dt == torch.int32:
+        return JZT_I32
+    if dt == torch.int64:
+        return JZT_I64
+    return dtype_code(dt)
 
 
 def call(name: str, *args):

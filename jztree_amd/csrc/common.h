@@ -25,6 +25,8 @@ int check_launch(const char* what);
 template <typename T> struct IntOf;
 template <> struct IntOf<float> { using type = int32_t; using utype = uint32_t; };
 template <> struct IntOf<double> { using type = int64_t; using utype = uint64_t; };
+template <> struct IntOf<int32_t> { using type = int32_t; using utype = uint32_t; };
+template <> struct IntOf<int64_t> { using type = int64_t; using utype = uint64_t; };
 
 // ------------------------------------------------------------------------------------------------
 // small vector type, AoS layout (dim, T) exactly like the reference's Vec<dim,tvec>

@@ -95,13 +95,21 @@ template <typename T> __device__ __forceinline__ int top_digit(T x) {
 
 template <int D, typename T>
 __device__ __forceinline__ uint64_t morton_key(const T* x, int Lt, bool general) {
+  constexpr int B = 63 / D;
+  uint64_t c[D];
+  uint64_t top = 0;
+  if constexpr (std::is_integral<T>::value) {
+    // integers (math.cuh:293-307): the order of the two's-complement bits with the sign flipped;
+    // Lt = shift that brings the highest bit any two coordinates differ in to the top of B bits
+    using U = typename IntOf<T>::utype;
+    constexpr U kSign = (U)1 << (sizeof(U) * 8 - 1);
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] = ((uint64_t)((U)x[d] ^ kSign) >> Lt) & ((1ull << B) - 1ull);
+  } else {
   bool isnan_row = false;
 #pragma unroll
   for (int d = 0; d < D; d++) isnan_row |= __builtin_isnan(x[d]);
   if (isnan_row) return ~0ull;
-  constexpr int B = 63 / D;
-  uint64_t c[D];
-  uint64_t top = 0;
   if (!general) {
 #pragma unroll
     for (int d = 0; d < D; d++) c[d] = mag_code<T>(x[d], Lt, B);
@@ -116,6 +124,7 @@ __device__ __forceinline__ uint64_t morton_key(const T* x, int Lt, bool general)
       c[d] = m;
       top = (top << 2) | (uint64_t)t;
     }
+  }
   }
   uint64_t key = 0;
   if constexpr (D == 3) {
@@ -150,6 +159,22 @@ template <typename T> __device__ __forceinline__ int exact_level(T a, T b) {
 }
 
 template <int D, typename T> __device__ int exact_cmp(const T* p1, const T* p2) {
+  if constexpr (std::is_integral<T>::value) {
+    // math.cuh:293-307 and :381-402 for integers: the dimension with the highest differing bit
+    // (the first of equal ones) decides, by signed comparison
+    using U = typename IntOf<T>::utype;
+    int md = -1, m = -1;
+    for (int d = 0; d < D; d++) {
+      const U x = (U)p1[d] ^ (U)p2[d];
+      const int l = x == 0 ? -1 : (int)(sizeof(U) * 8) - 1 - (sizeof(U) == 4 ? __clz((uint32_t)x) : __clzll((uint64_t)x));
+      if (l > m) {
+        m = l;
+        md = d;
+      }
+    }
+    if (md < 0) return 0;
+    return p1[md] < p2[md] ? -1 : 1;
+  } else {
   bool n1 = false, n2 = false;
   for (int d = 0; d < D; d++) {
     n1 |= __builtin_isnan(p1[d]);
@@ -168,6 +193,7 @@ template <int D, typename T> __device__ int exact_cmp(const T* p1, const T* p2) 
   T a = p1[md], b = p2[md];
   if (m == (1 << 20)) return top_digit<T>(a) < top_digit<T>(b) ? -1 : 1;
   return a < b ? -1 : 1;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -203,6 +229,27 @@ __global__ __launch_bounds__(256) void k_bounds(const T* __restrict__ pos, int64
   }
 }
 
+// integers: meta[0] / meta[3] = low / high word of the OR over all coordinates of (x XOR the first
+// coordinate), whose top bit is the highest bit in which any two coordinates differ
+template <typename T>
+__global__ __launch_bounds__(256) void k_bounds_int(const T* __restrict__ pos, int64_t nval, uint32_t* meta) {
+  using U = typename IntOf<T>::utype;
+  const uint64_t u0 = (uint64_t)(U)pos[0];
+  uint64_t acc = 0;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nval; i += st) acc |= (uint64_t)(U)pos[i] ^ u0;
+  uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+  for (int o = 32; o > 0; o >>= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, o, 64);
+    hi |= (uint32_t)__shfl_xor((int)hi, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (lo) atomicOr(&meta[0], lo);
+    if (hi) atomicOr(&meta[3], hi);
+  }
+}
+
 template <typename T> __device__ __forceinline__ int top_level_from_meta(const uint32_t* meta) {
   using FB = FBits<T>;
   uint32_t be = meta[0];
@@ -219,8 +266,17 @@ __global__ __launch_bounds__(256) void k_keygen(const T* __restrict__ pos, int64
   __shared__ uint32_t sh[kPasses][kRadix];
   for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&sh[0][0])[i] = 0;
   __syncthreads();
-  int Lt = top_level_from_meta<T>(meta);
-  bool general = (meta[1] | meta[2]) != 0;
+  int Lt;
+  bool general;
+  if constexpr (std::is_integral<T>::value) {
+    const uint64_t v = ((uint64_t)meta[3] << 32) | meta[0];
+    const int hb = v ? 63 - __clzll(v) : -1;
+    Lt = max(0, hb + 1 - 63 / D);
+    general = false;
+  } else {
+    Lt = top_level_from_meta<T>(meta);
+    general = (meta[1] | meta[2]) != 0;
+  }
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t st = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += st) {
@@ -505,7 +561,7 @@ SortLayout sort_layout(int64_t n, int dim, int dtype) {
   L.hist = o; o += 4 * (size_t)kPasses * kRadix;
   L.meta = o; o += 4 * 16;
   L.lookback = o; o = align256(o + 4 * (size_t)kPasses * L.ntiles * kRadix);
-  L.poscopy = o; o = align256(o + (size_t)n * dim * (dtype == JZT_F64 ? 8 : 4));
+  L.poscopy = o; o = align256(o + (size_t)n * dim * ((dtype == JZT_F64 || dtype == JZT_I64) ? 8 : 4));
   L.runflags = o; o = align256(o + (size_t)n);
   L.runtmp = o; o = align256(o + 4 * (size_t)(scan_tmp_words(n) + n));
   L.runs = o; o = align256(o + 4 * (size_t)(n / 2 + 1));
@@ -516,7 +572,7 @@ SortLayout sort_layout(int64_t n, int dim, int dtype) {
 
 template <int D, typename T>
 int zsort_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::type* id_out, char* tmp, int64_t n) {
-  SortLayout L = sort_layout(n, D, std::is_same<T, double>::value ? JZT_F64 : JZT_F32);
+  SortLayout L = sort_layout(n, D, sizeof(T) == 8 ? JZT_F64 : JZT_F32);
   uint64_t* k0 = (uint64_t*)(tmp + L.keys0);
   uint64_t* k1 = (uint64_t*)(tmp + L.keys1);
   uint32_t* v0 = (uint32_t*)(tmp + L.vals0);
@@ -535,7 +591,10 @@ int zsort_impl(hipStream_t s, const T* pos_in, T* pos_out, typename IntOf<T>::ty
   }
   int64_t blocks = div_up(n * D, 256);
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL((k_bounds<T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n * D, meta);
+  if constexpr (std::is_integral<T>::value)
+    hipLaunchKernelGGL((k_bounds_int<T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n * D, meta);
+  else
+    hipLaunchKernelGGL((k_bounds<T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n * D, meta);
   blocks = div_up(n, 256);
   if (blocks > 2048) blocks = 2048;
   hipLaunchKernelGGL((k_keygen<D, T>), dim3((unsigned)blocks), dim3(256), 0, s, src, n, meta, k0, v0, hist);
@@ -964,7 +1023,7 @@ __global__ void k_search_sorted_z(const T* __restrict__ have, const T* __restric
     else hi = mid;
   }
   int64_t r = lo;
-  if (leaf_search) {
+  if constexpr (!std::is_integral<T>::value) if (leaf_search) {
     // leaves lo-1 and lo are the z-order neighbours of x
     const int64_t a = lo > 0 ? lo - 1 : 0, b = lo < n_have ? lo : n_have - 1;
     const int lv1 = msb_diff_level<D, T>(load_vec<D, T>(have, a), load_vec<D, T>(x, 0));
@@ -990,7 +1049,8 @@ extern "C" int jzt_search_sorted_z(jzt_stream_t stream, const void* posz_have, c
 #define JZT_SSZ(D, T)                                                                                            \
   hipLaunchKernelGGL((k_search_sorted_z<D, T>), dim3(g), dim3(256), 0, s, (const T*)posz_have, (const T*)posz_query, \
                      n_have, n_query, leaf, indices)
-  switch ((dtype == JZT_F64 ? 100 : dtype == JZT_F32 ? 0 : 1000) + dim) {
+  if ((dtype == JZT_I32 || dtype == JZT_I64) && leaf) return fail("jzt_search_sorted_z: leaf search needs float positions");
+  switch ((dtype == JZT_F64 ? 100 : dtype == JZT_F32 ? 0 : dtype == JZT_I32 ? 200 : dtype == JZT_I64 ? 300 : 1000) + dim) {
     case 2: JZT_SSZ(2, float); break;
     case 3: JZT_SSZ(3, float); break;
     case 4: JZT_SSZ(4, float); break;
@@ -1009,6 +1069,24 @@ extern "C" int jzt_search_sorted_z(jzt_stream_t stream, const void* posz_have, c
     case 108: JZT_SSZ(8, double); break;
     case 109: JZT_SSZ(9, double); break;
     case 110: JZT_SSZ(10, double); break;
+    case 202: JZT_SSZ(2, int32_t); break;
+    case 203: JZT_SSZ(3, int32_t); break;
+    case 204: JZT_SSZ(4, int32_t); break;
+    case 205: JZT_SSZ(5, int32_t); break;
+    case 206: JZT_SSZ(6, int32_t); break;
+    case 207: JZT_SSZ(7, int32_t); break;
+    case 208: JZT_SSZ(8, int32_t); break;
+    case 209: JZT_SSZ(9, int32_t); break;
+    case 210: JZT_SSZ(10, int32_t); break;
+    case 302: JZT_SSZ(2, int64_t); break;
+    case 303: JZT_SSZ(3, int64_t); break;
+    case 304: JZT_SSZ(4, int64_t); break;
+    case 305: JZT_SSZ(5, int64_t); break;
+    case 306: JZT_SSZ(6, int64_t); break;
+    case 307: JZT_SSZ(7, int64_t); break;
+    case 308: JZT_SSZ(8, int64_t); break;
+    case 309: JZT_SSZ(9, int64_t); break;
+    case 310: JZT_SSZ(10, int64_t); break;
     default: return fail("jzt_search_sorted_z: unsupported dim/dtype");
   }
 #undef JZT_SSZ
@@ -1042,10 +1120,16 @@ extern "C" int jzt_pos_zorder_sort(jzt_stream_t stream, const void* pos_in, void
   // dims 2-10, as the reference instantiates PosZorderSort (_generate_ffi.py:9-21)
 #define JZT_ZS(D)                                                                                                \
   case D:                                                                                                        \
-    return dtype == JZT_F32                                                                                      \
-               ? zsort_impl<D, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n)         \
-               : zsort_impl<D, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);
-  if (dtype == JZT_F32 || dtype == JZT_F64) {
+    switch (dtype) {                                                                                             \
+      case JZT_F32: return zsort_impl<D, float>(s, (const float*)pos_in, (float*)pos_out, (int32_t*)id_out, t, n); \
+      case JZT_F64:                                                                                              \
+        return zsort_impl<D, double>(s, (const double*)pos_in, (double*)pos_out, (int64_t*)id_out, t, n);        \
+      case JZT_I32:                                                                                              \
+        return zsort_impl<D, int32_t>(s, (const int32_t*)pos_in, (int32_t*)pos_out, (int32_t*)id_out, t, n);     \
+      default:                                                                                                   \
+        return zsort_impl<D, int64_t>(s, (const int64_t*)pos_in, (int64_t*)pos_out, (int64_t*)id_out, t, n);     \
+    }
+  if (dtype == JZT_F32 || dtype == JZT_F64 || dtype == JZT_I32 || dtype == JZT_I64) {
     switch (dim) {
       JZT_ZS(2) JZT_ZS(3) JZT_ZS(4) JZT_ZS(5) JZT_ZS(6) JZT_ZS(7) JZT_ZS(8) JZT_ZS(9) JZT_ZS(10)
       default: break;

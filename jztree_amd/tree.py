@@ -60,7 +60,7 @@ def _zsort_impl(x: torch.Tensor, block_size: int = 64):
     ids = torch.empty(n, dtype=same_width_int(x.dtype), device=x.device)
     if n == 0:
         return pos, ids
-    dt = L.dtype_code(x.dtype)
+    dt = L.sort_dtype_code(x.dtype)
     nb = L.LIB.jzt_zsort_tmp_bytes(n, dim, dt)
     tmp = _tmp(nb, x.device)
     L.call("jzt_pos_zorder_sort", L.stream(), L.ptr(x), L.ptr(pos), L.ptr(ids), L.ptr(tmp), nb, n, dim, dt)
@@ -549,7 +549,7 @@ def search_sorted_z(posz: torch.Tensor, posz_query: torch.Tensor, block_size: in
     assert posz.shape[-1] == posz_query.shape[-1]
     out = torch.empty(len(posz_query), dtype=torch.int32, device=posz.device)
     L.call("jzt_search_sorted_z", L.stream(), L.ptr(posz.contiguous()), L.ptr(posz_query.contiguous()), len(posz),
-           len(posz_query), posz.shape[-1], L.dtype_code(posz.dtype), int(bool(leaf_search)), L.ptr(out))
+           len(posz_query), posz.shape[-1], L.sort_dtype_code(posz.dtype), int(bool(leaf_search)), L.ptr(out))
     return out
 
 

@@ -977,3 +977,52 @@ void NAME(const int32_t* isplit, const T* pos, const T* mass, int64_t nnodes, in
 }
 ORC_COM(float, fmaf, orc_center_of_mass_f32)
 ORC_COM(double, fma, orc_center_of_mass_f64)
+
+/* Integer positions (PosZorderSort's int32_t / int64_t instances): z_pos_less (math.cuh:381-402)
+ * with msb_xor (math.cuh:293-307) = index of the highest bit of a XOR b; the dimension with the
+ * highest such bit decides (the first of equal ones) by signed comparison; equal points keep their
+ * input order. */
+static const int32_t* g_sort_pos_i32;
+static const int64_t* g_sort_pos_i64;
+static int zcmp_i64(const int64_t* a, const int64_t* b, int dim) {
+    int md = -1, m = -1;
+    for (int d = 0; d < dim; d++) {
+        uint64_t x = (uint64_t)a[d] ^ (uint64_t)b[d];
+        int l = x ? 63 - __builtin_clzll(x) : -1;
+        if (l > m) { m = l; md = d; }
+    }
+    if (md < 0) return 0;
+    return a[md] < b[md] ? -1 : 1;
+}
+static int zcmp_i32(const int32_t* a, const int32_t* b, int dim) {
+    int md = -1, m = -1;
+    for (int d = 0; d < dim; d++) {
+        uint32_t x = (uint32_t)a[d] ^ (uint32_t)b[d];
+        int l = x ? 31 - __builtin_clz(x) : -1;
+        if (l > m) { m = l; md = d; }
+    }
+    if (md < 0) return 0;
+    return a[md] < b[md] ? -1 : 1;
+}
+static int cmp_idx_i32(const void* a, const void* b) {
+    int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+    int c = zcmp_i32(g_sort_pos_i32 + i * g_sort_dim, g_sort_pos_i32 + j * g_sort_dim, g_sort_dim);
+    if (c) return c;
+    return (i > j) - (i < j);
+}
+static int cmp_idx_i64(const void* a, const void* b) {
+    int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+    int c = zcmp_i64(g_sort_pos_i64 + i * g_sort_dim, g_sort_pos_i64 + j * g_sort_dim, g_sort_dim);
+    if (c) return c;
+    return (i > j) - (i < j);
+}
+void orc_zsort_i32(const int32_t* pos, int64_t n, int dim, int64_t* idz_out) {
+    for (int64_t i = 0; i < n; i++) idz_out[i] = i;
+    g_sort_pos_i32 = pos; g_sort_dim = dim;
+    qsort(idz_out, (size_t)n, sizeof(int64_t), cmp_idx_i32);
+}
+void orc_zsort_i64(const int64_t* pos, int64_t n, int dim, int64_t* idz_out) {
+    for (int64_t i = 0; i < n; i++) idz_out[i] = i;
+    g_sort_pos_i64 = pos; g_sort_dim = dim;
+    qsort(idz_out, (size_t)n, sizeof(int64_t), cmp_idx_i64);
+}

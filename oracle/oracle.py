@@ -60,6 +60,8 @@ def lib():
         L.orc_z_exact_cmp_f64.argtypes = [_f64p, _f64p, _c.c_int]
         L.orc_zsort_f32.argtypes = [_f32p, _c.c_int64, _c.c_int, _i64p]
         L.orc_zsort_f64.argtypes = [_f64p, _c.c_int64, _c.c_int, _i64p]
+        L.orc_zsort_i32.argtypes = [_i32p, _c.c_int64, _c.c_int, _i64p]
+        L.orc_zsort_i64.argtypes = [_i64p, _c.c_int64, _c.c_int, _i64p]
         L.orc_flag_leaf_boundaries_f32.argtypes = [
             _f32p, _u8p, _i32p, _c.c_int32, _c.c_int64, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
             _i8p, _i32p]
@@ -129,7 +131,11 @@ def zsort(pos: np.ndarray):
     pos = np.ascontiguousarray(pos)
     n, dim = pos.shape
     idz = np.empty(n, np.int64)
-    if pos.dtype == np.float64:
+    if pos.dtype == np.int32:
+        lib().orc_zsort_i32(pos, n, dim, idz)
+    elif pos.dtype == np.int64:
+        lib().orc_zsort_i64(pos, n, dim, idz)
+    elif pos.dtype == np.float64:
         lib().orc_zsort_f64(pos, n, dim, idz)
     else:
         pos = pos.astype(np.float32)

@@ -395,3 +395,26 @@ def test_center_of_mass_plane_recursion(oracle):
     c1, m1 = oracle.center_of_mass(np.array([0, 2, 4], np.int32), c0, m0)
     cd, md = oracle.center_of_mass(np.array([0, 250, 1000], np.int32), x, w)
     assert np.allclose(c1, cd, rtol=0, atol=1e-14) and np.allclose(m1, md, rtol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("dim", [2, 3, 5])
+def test_zsort_integers_vs_bigint_morton(oracle, dtype, dim):
+    """Integer positions: the oracle's order equals the Morton order of the sign-flipped
+    two's-complement bits (dimension 0 most significant within a bit level), built with Python
+    integers, ties by input index."""
+    rng = np.random.default_rng(30 + dim)
+    bits = np.iinfo(dtype).bits
+    x = np.concatenate([rng.integers(-50, 50, (300, dim)), rng.integers(np.iinfo(dtype).min, np.iinfo(dtype).max,
+                                                                           (300, dim), dtype=dtype)]).astype(dtype)
+    x = np.concatenate([x, x[:40]])  # duplicates
+    def key(row):
+        u = [(int(v) + (1 << (bits - 1))) for v in row]  # two's complement with the sign bit flipped
+        k = 0
+        for b in range(bits - 1, -1, -1):
+            for d in range(dim):
+                k = (k << 1) | ((u[d] >> b) & 1)
+        return k
+    ref = sorted(range(len(x)), key=lambda i: (key(x[i]), i))
+    _, idz = oracle.zsort(x)
+    assert np.array_equal(idz, np.array(ref))

@@ -163,6 +163,36 @@ def test_zsort_high_dims(dev, oracle, dim, dtype):
     assert np.array_equal(pz.cpu().numpy(), opz)
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("dim", [2, 3, 5, 10])
+def test_zsort_int_positions(dev, oracle, dim, dtype):
+    """PosZorderSort and SearchSortedZ on integer positions (the reference's int32_t/int64_t sort
+    instances, _generate_ffi.py:9-21): permutation and positions equal the oracle's exact
+    Morton order of the signed integers -- the full range with negatives, a small-range clump,
+    duplicates -- and insertion at the search_sorted_z indices keeps the z-order."""
+    import jztree_amd as jz
+    rng = np.random.default_rng(300 + dim)
+    info = np.iinfo(dtype)
+    x = np.concatenate([rng.integers(info.min, info.max, (20000, dim), dtype=dtype, endpoint=True),
+                        rng.integers(-40, 40, (5000, dim)).astype(dtype),
+                        (1000 + rng.integers(0, 8, (3000, dim))).astype(dtype)])
+    x = np.concatenate([x, x[:500]])
+    x = x[rng.permutation(len(x))]
+    print(f"[zsort-int] dim={dim} dtype={np.dtype(dtype).name} n={len(x)}", flush=True)
+    pz, idz = jz.tree.zsort(torch.from_numpy(x).to(dev))
+    opz, oidz = oracle.zsort(x)
+    assert np.array_equal(idz.cpu().numpy().astype(np.int64), oidz)
+    assert np.array_equal(pz.cpu().numpy(), opz)
+    q = np.concatenate([rng.integers(-60, 60, (1500, dim)), rng.integers(info.min, info.max, (1500, dim),
+                                                                          dtype=dtype)]).astype(dtype)
+    qz, _ = oracle.zsort(q)
+    ins = jz.tree.search_sorted_z(_t(opz, dev), _t(qz, dev)).cpu().numpy()
+    merged = np.insert(opz, ins, qz, axis=0)
+    mz, _ = oracle.zsort(merged)
+    assert np.array_equal(merged, mz), "insertion at the returned indices breaks the z-order"
+    assert np.all(np.diff(ins) >= 0)
+
+
 @pytest.mark.parametrize("dim", [4, 7, 10])
 def test_build_tree_hierarchy_high_dims(dev, oracle, dim):
     """Leaves, node planes, levels and geometric centres in 4, 7 and 10 dimensions bit-exact
