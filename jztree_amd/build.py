@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libjztree_amd.so")
-SOURCES = ["util.hip", "sort.hip", "tree.hip", "knn.hip", "knn_fused.hip", "fof.hip", "catalogue.hip", "proposer.hip"]
+SOURCES = ["util.hip", "sort.hip", "tree.hip", "knn.hip", "knn_nd_f32.hip", "knn_nd_f64.hip", "knn_fused.hip", "fof.hip", "catalogue.hip", "proposer.hip"]
 ARCH = os.environ.get("JZT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

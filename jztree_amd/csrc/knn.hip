@@ -19,6 +19,7 @@
 // visit order (knn.cuh:84-102), so its tie order depends on the traversal; here it does not.
 #include "util.h"
 #include "knn_common.h"
+#include "knn_nd.h"
 #include "../../include/jztree_amd.h"
 
 namespace jzt {
@@ -865,16 +866,24 @@ extern "C" int jzt_knn_node2node_ex(jzt_stream_t stream, const int32_t* parent_i
                                     float boxsize, int dim, int dtype) {
   hipStream_t s = (hipStream_t)stream;
   if (k < 1) return fail("KnnNode2Node: k must be >= 1");
-  if (dtype == JZT_F32 && dim == 3)
-    return node2node_impl<3, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,
-                                    nodes_npart, node_r2_in, node_rmax2, node_ilist_spl, node_ilist_isrc,
-                                    node_ilist_r2, size_parents, size_nodes, parent_ilist_size, node_ilist_size, k,
-                                    boxsize);
-  if (dtype == JZT_F32 && dim == 2)
-    return node2node_impl<2, float>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,
-                                    nodes_npart, node_r2_in, node_rmax2, node_ilist_spl, node_ilist_isrc,
-                                    node_ilist_r2, size_parents, size_nodes, parent_ilist_size, node_ilist_size, k,
-                                    boxsize);
+  // dims 2-10, float or double node geometry (_generate_ffi.py:9-21)
+#define JZT_N2N(D, T)                                                                                              \
+  return node2node_impl<D, T>(s, parent_ilist_spl, parent_ilist_isrc, parent_ilist_r2, parent_spl, nodes,         \
+                              nodes_npart, node_r2_in, node_rmax2, node_ilist_spl, node_ilist_isrc, node_ilist_r2, \
+                              size_parents, size_nodes, parent_ilist_size, node_ilist_size, k, boxsize)
+#define JZT_N2N_DIM(D)                  \
+  case D:                               \
+    if (dtype == JZT_F32) JZT_N2N(D, float); \
+    JZT_N2N(D, double);
+  if (dtype == JZT_F32 || dtype == JZT_F64) {
+    switch (dim) {
+      JZT_N2N_DIM(2) JZT_N2N_DIM(3) JZT_N2N_DIM(4) JZT_N2N_DIM(5) JZT_N2N_DIM(6) JZT_N2N_DIM(7) JZT_N2N_DIM(8)
+      JZT_N2N_DIM(9) JZT_N2N_DIM(10)
+      default: break;
+    }
+  }
+#undef JZT_N2N_DIM
+#undef JZT_N2N
   return fail("KnnNode2Node: unsupported dim/dtype");
 }
 
@@ -898,8 +907,9 @@ extern "C" int jzt_plane_cap(jzt_stream_t stream, const int32_t* n2l, const int3
 }
 
 extern "C" size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype) {
-  (void)dtype;
-  return k > 64 ? (size_t)size_part_query * 8 : 0;
+  // float: the packed kernel's u64 resume key, or the (r2, id) pair of the any-dim kernel, both 8
+  // bytes per query past 64 neighbours; double: a (double r2, u32 id) pair past 32
+  return knn_leaf_nd_tmp_bytes(size_part_query, k, dtype == JZT_F64);
 }
 
 extern "C" int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
@@ -922,7 +932,11 @@ extern "C" int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_sp
   hipStream_t s = (hipStream_t)stream;
   if (k < 1) return fail("KnnLeaf2Leaf: k must be >= 1");
   if (tmp_bytes < jzt_knn_leaf2leaf_tmp_bytes(size_part_query, k, dtype)) return fail("KnnLeaf2Leaf: tmp too small");
-  if (dtype != JZT_F32) return fail("KnnLeaf2Leaf: positions must be float32");
+  if (dtype != JZT_F32 && dtype != JZT_F64) return fail("KnnLeaf2Leaf: positions must be float32 or float64");
+  if (dtype == JZT_F64 || dim > 3)
+    return (dtype == JZT_F64 ? knn_leaf_nd_f64 : knn_leaf_nd_f32)(
+        s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry, knn_rad, knn_id, tmp,
+        size_leaves_query, size_part_query, out_rows, ilist_size, k, boxsize, dim);
   if (dim == 3)
     return leaf2leaf_impl<3>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry,
                              knn_rad, knn_id, tmp, size_leaves_query, size_part_query, out_rows, ilist_size, k,

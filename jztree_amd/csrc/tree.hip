@@ -285,22 +285,31 @@ int geometry_impl(hipStream_t s, const void* pos, const int32_t* lbound, const i
 // regularization level (tree.py:55-68 _level_histogram, :160-187 find_regularization_level)
 // ------------------------------------------------------------------------------------------------
 // Block-private histogram in LDS (levels cluster on a few bins: global atomics alone would
-// serialise on them), merged with one global atomic per nonzero bin and block.
+// serialise on them), merged with one global atomic per nonzero bin and block.  Level ranges past
+// kRegLdsLevels bins (float64 positions in 4 or more dims: 2099 levels per dim) count straight
+// into the global histogram.
+constexpr int kRegLdsLevels = 8192;  // 64 KB of u64 bins
 __global__ __launch_bounds__(256) void k_reg_hist(const int32_t* __restrict__ lvl, const int32_t* __restrict__ w,
                                                   int64_t n, int min_lvl, int nlvl,
                                                   unsigned long long* __restrict__ hist) {
   extern __shared__ unsigned long long s_hist[];
-  for (int j = threadIdx.x; j < nlvl; j += blockDim.x) s_hist[j] = 0ull;
-  __syncthreads();
+  const bool lds = nlvl <= kRegLdsLevels;
+  unsigned long long* h = lds ? s_hist : hist;
+  if (lds) {
+    for (int j = threadIdx.x; j < nlvl; j += blockDim.x) s_hist[j] = 0ull;
+    __syncthreads();
+  }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int b = lvl[i] - min_lvl;
     b = b < 0 ? 0 : b;  // jnp.bincount clips below and drops above
     const int64_t wi = w ? (int64_t)w[i] : 1;
-    if (b < nlvl && wi != 0) atomicAdd(&s_hist[b], (unsigned long long)wi);
+    if (b < nlvl && wi != 0) atomicAdd(&h[b], (unsigned long long)wi);
   }
-  __syncthreads();
-  for (int j = threadIdx.x; j < nlvl; j += blockDim.x)
-    if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+  if (lds) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < nlvl; j += blockDim.x)
+      if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+  }
 }
 
 // FP32 evaluation in ascending level order (the reference reduces under XLA's order; only
@@ -311,12 +320,14 @@ __global__ __launch_bounds__(256) void k_reg_hist(const int32_t* __restrict__ lv
 constexpr int kRegThreads = 256;
 __global__ __launch_bounds__(kRegThreads) void k_reg_level(const unsigned long long* __restrict__ hist, int nlvl,
                                                            int min_lvl, int max_lvl, float fac, float frac,
-                                                           int32_t* out) {
+                                                           float* __restrict__ gbuf, int32_t* out) {
   extern __shared__ float s_f[];  // [0, nlvl) counts, [nlvl, 2 nlvl) running sums
   __shared__ int s_first[2];
   __shared__ float s_lim;
-  float* s_cnt = s_f;
-  float* s_cum = s_f + nlvl;
+  // past kRegLdsLevels levels the two arrays live in global memory (gbuf, 2 nlvl floats); the
+  // barriers below order the workgroup's global accesses as they do its LDS accesses
+  float* s_cnt = nlvl <= kRegLdsLevels ? s_f : gbuf;
+  float* s_cum = s_cnt + nlvl;
   const int tid = threadIdx.x;
   for (int b = tid; b < nlvl; b += kRegThreads) s_cnt[b] = (float)(int64_t)hist[b];
   if (tid == 0) s_first[0] = s_first[1] = INT_MAX;
@@ -428,7 +439,9 @@ extern "C" int jzt_geometry_to_f32(jzt_stream_t stream, const double* cent, cons
 }
 
 extern "C" size_t jzt_regularization_tmp_bytes(int min_lvl, int max_lvl) {
-  return (size_t)(max_lvl - min_lvl + 1) * 8;
+  // the u64 histogram, plus the level pass's float arrays when they do not fit in LDS
+  const size_t nlvl = (size_t)(max_lvl - min_lvl + 1);
+  return nlvl * 8 + (nlvl > (size_t)kRegLdsLevels ? nlvl * 8 : 0);
 }
 
 extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl, const int32_t* weights, int64_t n,
@@ -437,16 +450,18 @@ extern "C" int jzt_regularization_level(jzt_stream_t stream, const int32_t* lvl,
   hipStream_t s = (hipStream_t)stream;
   const int nlvl = max_lvl - min_lvl + 1;
   if (tmp_bytes < jzt_regularization_tmp_bytes(min_lvl, max_lvl)) return fail("regularization: tmp too small");
-  if ((size_t)nlvl * sizeof(unsigned long long) > 65536) return fail("regularization: level range too large");
+  if (nlvl < 1 || nlvl > (1 << 24)) return fail("regularization: level range out of bounds");
+  const bool lds = nlvl <= kRegLdsLevels;
   if (fill_u32(s, tmp, 0u, (int64_t)nlvl * 2)) return 1;
   if (n > 0)
     hipLaunchKernelGGL(k_reg_hist, dim3((unsigned)min(div_up(n, 256), (int64_t)512)), dim3(256),
-                       (size_t)nlvl * sizeof(unsigned long long), s, lvl, weights, n, min_lvl, nlvl,
+                       lds ? (size_t)nlvl * sizeof(unsigned long long) : 0, s, lvl, weights, n, min_lvl, nlvl,
                        (unsigned long long*)tmp);
   // 0.01*percentile is formed in double and rounded to float, as the reference's weak typing does
   const float frac = (float)(0.01 * (double)min_percentile);
-  hipLaunchKernelGGL(k_reg_level, dim3(1), dim3(kRegThreads), (size_t)nlvl * 2 * sizeof(float), s,
-                     (const unsigned long long*)tmp, nlvl, min_lvl, max_lvl, max_volume_fac, frac, out_level);
+  float* gbuf = lds ? nullptr : (float*)((char*)tmp + (size_t)nlvl * 8);
+  hipLaunchKernelGGL(k_reg_level, dim3(1), dim3(kRegThreads), lds ? (size_t)nlvl * 2 * sizeof(float) : 0, s,
+                     (const unsigned long long*)tmp, nlvl, min_lvl, max_lvl, max_volume_fac, frac, gbuf, out_level);
   return check_launch("regularization level");
 }
 

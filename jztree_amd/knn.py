@@ -438,8 +438,10 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
     ascending by (distance, index).  result: "_"-separated keys among rad, drad, rankidx,
     globalidx, part, reduce, or a particle attribute name.  output_order: "input" or "z".
     fused (extension, k <= 64, same results): True = the fused leaf-plane kernel (one wave per query
-    leaf over its parent's list); False runs the reference's KnnNode2Node + KnnLeaf2Leaf
-    decomposition (the default: fastest with the leaf sizes the autotuner picks).
+    leaf over its parent's list; float32 positions in 2-3 dims, ignored otherwise); False runs the
+    reference's KnnNode2Node + KnnLeaf2Leaf decomposition (the default: fastest with the leaf sizes
+    the autotuner picks).  Positions: float32 or float64, 2 to 10 dims (the reference's
+    instances); radii come in the positions' type.
     dist_dtype (extension): torch.float32 with float64 positions sorts and builds the tree in
     float64 (63-bit keys) and searches the float32-rounded positions with the float32 kernels; the
     result is the exact kNN of the rounded points (radii float32).
@@ -482,7 +484,11 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
         direct = (output_order == "input" and set(keys) <= {"rad", "globalidx"}
                   and isinstance(num_origin_q, int) and num_origin_q == size_q)
         drows = originz_q.idx if direct else None
-        if part_query is not None and k <= 64:
+        # the start-bound leaf kernels take up to 64 slots (32 for float64 positions) in one pass;
+        # the fused leaf-plane kernel is float32 in 2-3 dims
+        kb = 64 if posz.dtype == torch.float32 else 32
+        fused = fused and posz.dtype == torch.float32 and posz.shape[-1] <= 3
+        if part_query is not None and k <= kb:
             ilist, leaf_r2, _ = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                                return_parents=True)
             rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx, drows,
@@ -495,7 +501,7 @@ def knn(part, k: int, boxsize: float | None = None, th: TreeHierarchy | None = N
             # neighbours are reported (and tie-broken) by their origin index directly
             rnnz, inn_g, _ = _knn_fused(th, posz, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                         src_ids=originz.idx, dst_rows=drows)
-        elif k <= 64:
+        elif k <= kb:
             ilist, leaf_r2, _ = _knn_dual_walk(th, k, boxsize=boxsize, alloc_fac_ilist=cfg.alloc_fac_ilist,
                                                return_parents=True)
             rnnz, inn_g = _knn_leaf_start_bound(ilist, spl, posz, k, boxsize, leaf_r2, originz.idx, drows)

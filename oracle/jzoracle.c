@@ -662,6 +662,200 @@ void orc_knn_kdtree_f32(const float* xs, const int64_t* ids, int64_t ns, const f
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/*                 exact kNN in any dimension 2-10, float or double positions                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* wrap_dx<double> (math.cuh:471-488): the half box is a float (0.5 * boxsize, exact for the float
+ * boxsize), the difference is wrapped in double by one add/subtract. */
+static inline double wrap_f64(double dx, float boxsize) {
+    if (boxsize > 0.f) {
+        float bh = 0.5f * boxsize;
+        dx = dx < -bh ? dx + (double)boxsize : dx;
+        dx = dx >= bh ? dx - (double)boxsize : dx;
+    }
+    return dx;
+}
+/* distance_squared<dim,double> (math.cuh:511-522) under nvcc's contraction: r2 = dx*dx, then
+ * r2 = fma(d, d, r2) for the further dims, in double. */
+static inline double r2_f64(const double* a, const double* b, int dim, float boxsize) {
+    double dx = wrap_f64(a[0] - b[0], boxsize);
+    double r2 = dx * dx;
+    for (int i = 1; i < dim; i++) {
+        double d = wrap_f64(a[i] - b[i], boxsize);
+        r2 = fma(d, d, r2);
+    }
+    return r2;
+}
+double orc_r2_f64(const double* a, const double* b, int dim, float boxsize) { return r2_f64(a, b, dim, boxsize); }
+
+/* canonical r2 of source s and query q for either position type, as a double (a float r2 converts
+ * exactly, so the (r2, id) order is the float one) */
+static inline double r2_any(const void* xs, int64_t s, const void* xq, int64_t q, int dim, float boxsize, int is64) {
+    if (is64) return r2_f64((const double*)xq + q * dim, (const double*)xs + s * dim, dim, boxsize);
+    return (double)r2_f32((const float*)xq + q * dim, (const float*)xs + s * dim, dim, boxsize);
+}
+static inline int pair_less_d(double r2a, int64_t ia, double r2b, int64_t ib) {
+    return (r2a < r2b) || (r2a == r2b && ia < ib);
+}
+static void topk_insert_d(double* r2s, int64_t* ids, int k, double r2, int64_t id) {
+    if (!pair_less_d(r2, id, r2s[k - 1], ids[k - 1])) return;
+    int i = k - 1;
+    while (i > 0 && pair_less_d(r2, id, r2s[i - 1], ids[i - 1])) { r2s[i] = r2s[i - 1]; ids[i] = ids[i - 1]; i--; }
+    r2s[i] = r2; ids[i] = id;
+}
+
+/* Brute force (knn.cuh:203-306 restated without a tree): per query the k smallest (canonical r2,
+ * id), r2 as double.  Any dim, is64 = double positions. */
+void orc_knn_brute_nd(const void* xs, const int64_t* ids, int64_t ns, const void* xq, int64_t nq, int dim, int k,
+                      float boxsize, int is64, double* r2_out, int64_t* id_out) {
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nq; q++) {
+        double* r2s = r2_out + q * k;
+        int64_t* ii = id_out + q * k;
+        for (int j = 0; j < k; j++) { r2s[j] = INFINITY; ii[j] = INT64_MAX; }
+        for (int64_t s = 0; s < ns; s++)
+            topk_insert_d(r2s, ii, k, r2_any(xs, s, xq, q, dim, boxsize, is64), ids ? ids[s] : s);
+    }
+}
+
+/* k-d tree for dims up to 10 over a double copy of the positions (exact for float positions, so
+ * the splits are those of the float values): median splits of the widest box dim, leaves of <= 16
+ * points; a subtree is skipped once its float64 box distance^2 exceeds the current k-th r2 by a
+ * 1e-5 relative margin (far above the rounding of either canonical r2).  Leaf points are scored
+ * with the canonical r2 of the position type.  Results equal orc_knn_brute_nd. */
+#define KDN_MAXD 10
+typedef struct { double lo[KDN_MAXD], hi[KDN_MAXD]; int64_t start, end, left, right; } kdn_node_t;
+typedef struct { kdn_node_t* nodes; int64_t nnodes, cap; int64_t* idx; const double* x; int dim; } kdn_tree_t;
+
+static void kdn_box(kdn_tree_t* t, int64_t s, int64_t e, double* lo, double* hi) {
+    for (int d = 0; d < t->dim; d++) { lo[d] = INFINITY; hi[d] = -INFINITY; }
+    for (int64_t i = s; i < e; i++)
+        for (int d = 0; d < t->dim; d++) {
+            double v = t->x[t->idx[i] * t->dim + d];
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = v > hi[d] ? v : hi[d];
+        }
+}
+static void kdn_select(kdn_tree_t* t, int64_t s, int64_t e, int64_t kth, int d) {
+    int64_t* a = t->idx;
+    const double* x = t->x;
+    int dim = t->dim;
+    while (e - s > 1) {
+        double piv = x[a[s + (e - s) / 2] * dim + d];
+        int64_t i = s, j = e - 1;
+        while (i <= j) {
+            while (x[a[i] * dim + d] < piv) i++;
+            while (x[a[j] * dim + d] > piv) j--;
+            if (i <= j) { int64_t tmp = a[i]; a[i] = a[j]; a[j] = tmp; i++; j--; }
+        }
+        if (kth <= j) e = j + 1;
+        else if (kth >= i) s = i;
+        else return;
+    }
+}
+static int64_t kdn_alloc(kdn_tree_t* t) {
+    int64_t id;
+    #pragma omp critical(kdnalloc)
+    { id = t->nnodes++; }
+    return id;
+}
+static void kdn_build_rec(kdn_tree_t* t, int64_t node, int64_t s, int64_t e, int depth) {
+    kdn_node_t* nd = &t->nodes[node];
+    kdn_box(t, s, e, nd->lo, nd->hi);
+    nd->start = s; nd->end = e; nd->left = nd->right = -1;
+    if (e - s <= 16) return;
+    int bd = 0;
+    double bw = -1.0;
+    for (int d = 0; d < t->dim; d++) { double w = nd->hi[d] - nd->lo[d]; if (w > bw) { bw = w; bd = d; } }
+    if (!(bw > 0.0)) return;  /* all points identical: keep as a (large) leaf */
+    int64_t mid = s + (e - s) / 2;
+    kdn_select(t, s, e, mid, bd);
+    int64_t l = kdn_alloc(t), r = kdn_alloc(t);
+    t->nodes[node].left = l; t->nodes[node].right = r;
+    if (depth < 6) {
+        #pragma omp task
+        kdn_build_rec(t, l, s, mid, depth + 1);
+        #pragma omp task
+        kdn_build_rec(t, r, mid, e, depth + 1);
+        #pragma omp taskwait
+    } else {
+        kdn_build_rec(t, l, s, mid, depth + 1);
+        kdn_build_rec(t, r, mid, e, depth + 1);
+    }
+}
+static double kdn_mind2(const kdn_node_t* nd, const double* q, int dim, float boxsize) {
+    double r = 0.0;
+    for (int d = 0; d < dim; d++) {
+        double v = q[d], lo = nd->lo[d], hi = nd->hi[d], dd = 0.0;
+        if (v < lo) dd = lo - v; else if (v > hi) dd = v - hi;
+        if (boxsize > 0.f && dd > 0.0) {
+            double L = boxsize;
+            double a = (v + L < lo) ? lo - (v + L) : ((v + L > hi) ? v + L - hi : 0.0);
+            double b = (v - L < lo) ? lo - (v - L) : ((v - L > hi) ? v - L - hi : 0.0);
+            if (a < dd) dd = a;
+            if (b < dd) dd = b;
+        }
+        r += dd * dd;
+    }
+    return r;
+}
+typedef struct { const void* xs; const void* xq; const int64_t* ids; int dim, k, is64; float boxsize; } kdn_q_t;
+static void kdn_query(const kdn_tree_t* t, int64_t node, const kdn_q_t* c, int64_t q, const double* qd,
+                      double* r2s, int64_t* ii) {
+    const kdn_node_t* nd = &t->nodes[node];
+    if (kdn_mind2(nd, qd, t->dim, c->boxsize) > r2s[c->k - 1] * (1.0 + 1e-5)) return;
+    if (nd->left < 0) {
+        for (int64_t a = nd->start; a < nd->end; a++) {
+            int64_t si = t->idx[a];
+            topk_insert_d(r2s, ii, c->k, r2_any(c->xs, si, c->xq, q, t->dim, c->boxsize, c->is64),
+                          c->ids ? c->ids[si] : si);
+        }
+        return;
+    }
+    double dl = kdn_mind2(&t->nodes[nd->left], qd, t->dim, c->boxsize);
+    double dr = kdn_mind2(&t->nodes[nd->right], qd, t->dim, c->boxsize);
+    if (dl <= dr) {
+        kdn_query(t, nd->left, c, q, qd, r2s, ii);
+        kdn_query(t, nd->right, c, q, qd, r2s, ii);
+    } else {
+        kdn_query(t, nd->right, c, q, qd, r2s, ii);
+        kdn_query(t, nd->left, c, q, qd, r2s, ii);
+    }
+}
+static double* to_f64(const void* x, int64_t n, int is64) {
+    double* d = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) d[i] = is64 ? ((const double*)x)[i] : (double)((const float*)x)[i];
+    return d;
+}
+/* returns 0, or -1 for a dim outside 1..10 */
+int orc_knn_kdtree_nd(const void* xs, const int64_t* ids, int64_t ns, const void* xq, int64_t nq, int dim, int k,
+                      float boxsize, int is64, double* r2_out, int64_t* id_out) {
+    if (dim < 1 || dim > KDN_MAXD) return -1;
+    kdn_tree_t t;
+    double* xsd = to_f64(xs, ns * dim, is64);
+    double* xqd = to_f64(xq, nq * dim, is64);
+    t.x = xsd; t.dim = dim;
+    t.cap = 2 * (ns / 8 + 2) + 16;
+    t.nodes = (kdn_node_t*)malloc(sizeof(kdn_node_t) * (size_t)t.cap);
+    t.idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ns > 0 ? ns : 1));
+    for (int64_t i = 0; i < ns; i++) t.idx[i] = i;
+    t.nnodes = 1;
+    #pragma omp parallel
+    #pragma omp single
+    kdn_build_rec(&t, 0, 0, ns, 0);
+    kdn_q_t c = {xs, xq, ids, dim, k, is64, boxsize};
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t q = 0; q < nq; q++) {
+        double* r2s = r2_out + q * k;
+        int64_t* ii = id_out + q * k;
+        for (int j = 0; j < k; j++) { r2s[j] = INFINITY; ii[j] = INT64_MAX; }
+        if (ns > 0) kdn_query(&t, 0, &c, q, xqd + q * dim, r2s, ii);
+    }
+    free(t.nodes); free(t.idx); free(xsd); free(xqd);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /*                                  friends-of-friends                                        */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -77,6 +77,13 @@ def lib():
         L.orc_knn_brute_f32.argtypes = knn_args
         L.orc_knn_grid_f32.argtypes = knn_args
         L.orc_knn_kdtree_f32.argtypes = knn_args
+        L.orc_r2_f64.argtypes = [_f64p, _f64p, _c.c_int, _c.c_float]
+        L.orc_r2_f64.restype = _c.c_double
+        knn_nd_args = [_c.c_void_p, _c.c_void_p, _c.c_int64, _c.c_void_p, _c.c_int64, _c.c_int, _c.c_int,
+                       _c.c_float, _c.c_int, _f64p, _i64p]
+        L.orc_knn_brute_nd.argtypes = knn_nd_args
+        L.orc_knn_kdtree_nd.argtypes = knn_nd_args
+        L.orc_knn_kdtree_nd.restype = _c.c_int
         fof_args = [_f32p, _c.c_int64, _c.c_int, _c.c_float, _c.c_float, _c.c_void_p, _i64p]
         L.orc_fof_f32.argtypes = fof_args
         L.orc_fof_brute_f32.argtypes = fof_args
@@ -411,27 +418,44 @@ def build_tree_hierarchy(posz, max_leaf_size=32, coarse_fac=6.0, stop_coarsen=10
 # kNN and FoF
 # ------------------------------------------------------------------------------------------------
 
-def r2(a, b, boxsize=0.0) -> np.float32:
-    """Canonical FP32 r^2: fma(dz,dz, fma(dy,dy, dx*dx)) of wrapped differences."""
+def r2(a, b, boxsize=0.0):
+    """Canonical r^2: fma(dz,dz, fma(dy,dy, dx*dx)) of wrapped differences, in the positions' type
+    (float32, or float64 with the half box taken as a float, math.cuh:471-522)."""
+    if np.asarray(a).dtype == np.float64:
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        return np.float64(lib().orc_r2_f64(a, b, len(a), float(boxsize)))
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
     return np.float32(lib().orc_r2_f32(a, b, len(a), float(boxsize)))
 
 
 def knn(xs, k, xq=None, boxsize=0.0, ids=None, method="kdtree"):
-    """Exact kNN: per query the k smallest (canonical r2, id).  Returns (r2 (nq,k) f32, id (nq,k) i64).
+    """Exact kNN: per query the k smallest (canonical r2, id).  Returns (r2 (nq,k), id (nq,k) i64),
+    r2 in the positions' type: float32 (any other input is rounded to it) or float64.
 
-    ids: the id reported (and used for tie-breaks) for each source point; default = its index."""
-    xs = np.ascontiguousarray(xs, np.float32)
-    xq = xs if xq is None else np.ascontiguousarray(xq, np.float32)
+    ids: the id reported (and used for tie-breaks) for each source point; default = its index.
+    float32 in 2-3 dims runs the float32 routines (brute, grid, kdtree); float64, or more than 3
+    dims (up to 10), the any-dim routines (brute, kdtree; grid maps to kdtree)."""
+    is64 = np.asarray(xs).dtype == np.float64
+    ty = np.float64 if is64 else np.float32
+    xs = np.ascontiguousarray(xs, ty)
+    xq = xs if xq is None else np.ascontiguousarray(xq, ty)
     ns, dim = xs.shape
     nq = len(xq)
-    r2o = np.empty((nq, k), np.float32)
     io = np.empty((nq, k), np.int64)
     idp = None
     if ids is not None:
         ids = np.ascontiguousarray(ids, np.int64)
         idp = ids.ctypes.data
+    if is64 or dim > 3:
+        r2d = np.empty((nq, k), np.float64)
+        fn = lib().orc_knn_brute_nd if method == "brute" else lib().orc_knn_kdtree_nd
+        rc = fn(xs.ctypes.data, idp, ns, xq.ctypes.data, nq, dim, k, float(boxsize), int(is64), r2d, io)
+        if rc not in (None, 0):
+            raise ValueError(f"knn oracle: unsupported dim {dim}")
+        return (r2d if is64 else r2d.astype(np.float32)), io
+    r2o = np.empty((nq, k), np.float32)
     fn = {"brute": lib().orc_knn_brute_f32, "grid": lib().orc_knn_grid_f32,
           "kdtree": lib().orc_knn_kdtree_f32}[method]
     fn(xs, idp, ns, xq, nq, dim, k, float(boxsize), r2o, io)

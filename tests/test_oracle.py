@@ -418,3 +418,62 @@ def test_zsort_integers_vs_bigint_morton(oracle, dtype, dim):
     ref = sorted(range(len(x)), key=lambda i: (key(x[i]), i))
     _, idz = oracle.zsort(x)
     assert np.array_equal(idz, np.array(ref))
+
+
+def test_r2_f64_exact_cases(oracle):
+    """float64 canonical r2: exact where every operation is (coordinates on a 1/64 grid, so all
+    differences, wraps, squares and sums are exact) -- the minimum-image sum of squares, with
+    wrapped differences in [-L/2, L/2) -- and within dim ulps of the exact value elsewhere."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    for dim in (2, 3, 7, 10):
+        for _ in range(200):
+            a = rng.integers(0, 512, dim) / 64.0
+            b = rng.integers(0, 512, dim) / 64.0
+            d = a - b
+            exp_open = float(np.sum(d * d))
+            dw = np.where(d < -4.0, d + 8.0, d)
+            dw = np.where(dw >= 4.0, dw - 8.0, dw)
+            assert np.all((dw >= -4.0) & (dw < 4.0))
+            assert oracle.r2(a, b) == exp_open
+            assert oracle.r2(a, b, boxsize=8.0) == float(np.sum(dw * dw))
+        a, b = rng.random(dim), rng.random(dim)
+        ex = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(a, b))
+        got = Fraction(float(oracle.r2(a, b)))
+        assert abs(got - ex) <= dim * Fraction(2) ** -52 * ex
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dim", [2, 4, 7, 10])
+def test_knn_nd_kdtree_equals_brute(oracle, dtype, dim):
+    """The any-dim k-d tree equals brute force (open and periodic, duplicates included), and in 2-3
+    dims the float32 any-dim routines equal the float32 ones."""
+    rng = np.random.default_rng(dim)
+    x = np.concatenate([rng.random((1500, dim)), 0.5 + 0.01 * rng.random((500, dim))]).astype(dtype)
+    x = np.concatenate([x, x[:50]])
+    for box in (0.0, 1.0):
+        a = oracle.knn(x, 10, boxsize=box, method="brute")
+        b = oracle.knn(x, 10, boxsize=box, method="kdtree")
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert a[0].dtype == dtype
+    if dtype == np.float32 and dim == 2:
+        lib = oracle.lib()
+        r2d = np.empty((len(x), 10), np.float64)
+        ii = np.empty((len(x), 10), np.int64)
+        lib.orc_knn_kdtree_nd(x.ctypes.data, None, len(x), x.ctypes.data, len(x), dim, 10, 0.0, 0, r2d, ii)
+        a = oracle.knn(x, 10, method="kdtree")
+        assert np.array_equal(r2d.astype(np.float32), a[0]) and np.array_equal(ii, a[1])
+
+
+@pytest.mark.parametrize("dim", [3, 6])
+def test_knn_f64_vs_scipy(oracle, dim):
+    """float64 neighbours equal scipy's cKDTree (an independent exact search) on random points,
+    open and periodic; the radii agree to 1e-12 relative."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(40 + dim)
+    x = rng.random((4000, dim))
+    for box in (None, 1.0):
+        d, i = cKDTree(x, boxsize=box).query(x, k=8)
+        r2, io = oracle.knn(x, 8, boxsize=box or 0.0)
+        assert np.array_equal(io, i)
+        assert np.allclose(np.sqrt(r2), d, rtol=1e-12, atol=0)
