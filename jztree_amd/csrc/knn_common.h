@@ -147,6 +147,12 @@ __device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* 
 // return).  Sources go JZT_CONSUME_UNROLL at a time: their LDS reads are issued together and the
 // wave flushes before a group whenever some lane has less room than a whole group.  Returns
 // whether best[] changed (a flush happened).
+//
+// A candidate is accepted when its r2 is <= the r2 of the lane's k-th key: a superset of "key <
+// best[K-1]" that only adds equal-r2 candidates with larger ids, which the flush (an exact merge of
+// keys) drops again.  The candidate is written to the row's next free slot whether it is accepted
+// or not, and the fill advances by the accept flag: no divergent branch per candidate.  A group
+// starts with at most CAP - U entries, so every such write lands inside the row.
 #ifndef JZT_CONSUME_UNROLL
 #define JZT_CONSUME_UNROLL 2
 #endif
@@ -158,10 +164,13 @@ __device__ __forceinline__ bool consume_chunk(uint64_t (&best)[K], const Vec<D, 
   constexpr int CAP = kBufCap<K>;
   static_assert(U >= 1 && U <= CAP, "unroll must fit a buffer row");
   bool flushed = false;
+  float kr = key_r2(best[K - 1]);
+  uint32_t* const row32 = reinterpret_cast<uint32_t*>(row);
   for (int j = 0; j < n; j += U) {  // one flush site: the network is inlined once
     if (wave_any(cnt > CAP - U)) {
       topk_flush<K>(best, row, cnt);
       flushed = true;
+      kr = key_r2(best[K - 1]);
     }
     SrcRec c[U];
 #pragma unroll
@@ -177,11 +186,11 @@ __device__ __forceinline__ bool consume_chunk(uint64_t (&best)[K], const Vec<D, 
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint64_t key = ((uint64_t)__float_as_uint(r2[u]) << 32) | c[u].id;
-      if (j + u < n && key < best[K - 1] && (!FLOOR || key > floor_key)) {
-        row[cnt] = key;
-        cnt++;
-      }
+      bool acc = j + u < n && r2[u] <= kr;
+      if (FLOOR) acc = acc && (((uint64_t)__float_as_uint(r2[u]) << 32) | c[u].id) > floor_key;
+      row32[2 * cnt] = c[u].id;
+      row32[2 * cnt + 1] = __float_as_uint(r2[u]);
+      cnt += acc ? 1 : 0;
     }
   }
   return flushed;
