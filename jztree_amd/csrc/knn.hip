@@ -316,8 +316,13 @@ __global__ __launch_bounds__(64) void k_count_insert(const int32_t* __restrict__
               const Vec<D, T> ew = add<D, T>(s_qe[c], s_te[my_t]);
               r2 = (float)mindist2<D, T>(s_tc[my_t], s_qc[c], ew, boxsize);
               hit = r2 <= s_qr[c];
-              if (PASS == 1 && hit && r2 == 0.f)  // touching nodes get a tiny radius: the node itself sorts first
-                r2 = 1e-10f * (float)maxdist2<D, T>(s_tc[my_t], s_qc[c], ew, boxsize);
+              // Touching nodes are listed with a negative radius, -1 / maxdist2: they sort before every
+              // node at a positive distance, the node itself first among them (smallest maxdist2), and
+              // a consumer's "r2 <= k-th distance" test keeps them even when the k-th distance is
+              // exactly 0 (k or more copies of a point): such a node may hold copies with smaller ids.
+              // (A tiny positive radius, 1e-10 * maxdist2, failed that test and lost those ties.)
+              if (PASS == 1 && hit && r2 == 0.f)
+                r2 = -1.0f / fmaxf((float)maxdist2<D, T>(s_tc[my_t], s_qc[c], ew, boxsize), 1e-30f);
             }
             const uint64_t hits = __ballot(hit) & block;
             if (PASS == 1 && hit) {

@@ -429,12 +429,16 @@ extern "C" int jzt_geometry_to_f32(jzt_stream_t stream, const double* cent, cons
   hipStream_t s = (hipStream_t)stream;
   if (n <= 0) return 0;
   const unsigned g = (unsigned)div_up(n, 256);
-  if (dim == 3)
-    hipLaunchKernelGGL(k_geometry_to_f32<3>, dim3(g), dim3(256), 0, s, cent, lvl, n, lvl_invalid, cent_out, lvl_out);
-  else if (dim == 2)
-    hipLaunchKernelGGL(k_geometry_to_f32<2>, dim3(g), dim3(256), 0, s, cent, lvl, n, lvl_invalid, cent_out, lvl_out);
-  else
-    return fail("GeometryToF32: unsupported dim");
+#define JZT_G2F(D)                                                                                                 \
+  case D:                                                                                                          \
+    hipLaunchKernelGGL(k_geometry_to_f32<D>, dim3(g), dim3(256), 0, s, cent, lvl, n, lvl_invalid, cent_out, lvl_out); \
+    break;
+  switch (dim) {
+    JZT_G2F(2) JZT_G2F(3) JZT_G2F(4) JZT_G2F(5) JZT_G2F(6) JZT_G2F(7) JZT_G2F(8) JZT_G2F(9) JZT_G2F(10)
+    default:
+      return fail("GeometryToF32: unsupported dim");
+  }
+#undef JZT_G2F
   return check_launch("GeometryToF32");
 }
 

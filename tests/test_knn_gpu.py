@@ -74,6 +74,19 @@ def test_knn_clustered_and_duplicates(dev, oracle):
     _check(x, 32, rnn, inn, oracle)
 
 
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("k", [12, 40])
+def test_knn_many_copies_of_a_point(dev, oracle, fused, k):
+    """More copies of a point than k, spread over several leaves (some of them mixed with other
+    points, so their cells have a positive extent): the k-th distance of a copy is exactly 0, and
+    its k neighbours are the copies with the smallest ids, wherever they sit."""
+    rng = np.random.default_rng(k)
+    x = np.concatenate([rng.random((20000, 3)), np.repeat(rng.random((3, 3)), 400, axis=0)]).astype(np.float32)
+    x = x[rng.permutation(len(x))]
+    rnn, inn = _knn(x, k, dev, fused=fused)
+    _check(x, k, rnn, inn, oracle)
+
+
 def test_knn_output_orders(dev):
     """checks/tests/test_knn.py:102-116: radii depend on the output order only; ids also on the input."""
     import jztree_amd as jz

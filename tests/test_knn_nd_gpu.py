@@ -117,3 +117,17 @@ def test_regularization_level_wide_ranges(dev, dim, dtype):
     lvl2 = torch.cat([lvl, torch.full((5,), -40, dtype=torch.int32, device=dev)])
     w2 = torch.cat([w, torch.full((5,), 200, dtype=torch.int32, device=dev)])
     assert int(jz.tree.find_regularization_level(lvl2, linfo, w2, max_vol_fac=20., min_percentile=90.)[0]) == -36
+
+
+@pytest.mark.parametrize("dim", [4, 7])
+def test_knn_f64_keys_f32_distances_high_dims(dev, oracle, dim):
+    """float64 keys and tree, float32 distance kernels (dist_dtype) in 4 and 7 dims: the exact kNN
+    of the float32-rounded points, including a clump below float32 resolution."""
+    rng = np.random.default_rng(60 + dim)
+    x = np.concatenate([rng.random((6000, dim)), 0.3 + 1e-9 * rng.random((500, dim))])
+    x32 = x.astype(np.float32)
+    rnn, inn = _knn(x, 12, dev, cfg=_cfg(dim), dist_dtype=torch.float32)
+    assert rnn.dtype == np.float32
+    r2o, io = oracle.knn(x32, 12)
+    assert np.array_equal(inn.astype(np.int64), io), f"{np.sum(inn != io)} ids differ"
+    assert np.array_equal(rnn, np.sqrt(r2o))
