@@ -131,3 +131,15 @@ def test_knn_f64_keys_f32_distances_high_dims(dev, oracle, dim):
     r2o, io = oracle.knn(x32, 12)
     assert np.array_equal(inn.astype(np.int64), io), f"{np.sum(inn != io)} ids differ"
     assert np.array_equal(rnn, np.sqrt(r2o))
+
+
+@pytest.mark.parametrize("dim,dtype", [(5, np.float32), (4, np.float64)])
+def test_knn_part_query_high_dims(dev, oracle, dim, dtype):
+    """Separate query points in 4-5 dims (one two-type tree over sources and queries)."""
+    import jztree_amd as jz
+    rng = np.random.default_rng(70 + dim)
+    x = rng.random((12000, dim)).astype(dtype)
+    q = rng.random((3000, dim)).astype(dtype)
+    print(f"[knn-nd] part_query dim={dim} {np.dtype(dtype).name}", flush=True)
+    rnn, inn = jz.knn.knn(torch.from_numpy(x).to(dev), 10, part_query=torch.from_numpy(q).to(dev), cfg=_cfg(dim))
+    _check(x, 10, rnn.cpu().numpy(), inn.cpu().numpy(), oracle, xq=q)
