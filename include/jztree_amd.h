@@ -272,7 +272,10 @@ int jzt_segment_max_f32(jzt_stream_t stream, const int32_t* spl, const float* va
  * knn.cuh:309-356, caller knn.py:28-60).  knn_rad (nq, k) of dtype, knn_id (nq, k) int32.
  * Neighbours ascending by (canonical r2, id); id = z-index in xT, or src_ids[z-index] when src_ids
  * is non-NULL (an extension: lets the caller report input indices directly).  dtype JZT_F32 or
- * JZT_F64, dim 2-10 (the reference's instances, _generate_ffi.py:9-21). */
+ * JZT_F64, dim 2-10 (the reference's instances, _generate_ffi.py:9-21).  tmp: at least
+ * jzt_knn_leaf2leaf_tmp_bytes bytes (the multi-pass state, then 256 bytes of work-queue counters
+ * that balance the query leaves over the waves); a smaller tmp (down to NULL for one pass) runs
+ * the leaves in a fixed order, with the same results.  Calls that share a tmp must not overlap. */
 size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype);
 int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
                       const float* ilist_r2, int64_t ilist_size, const int32_t* splT, const void* xT,

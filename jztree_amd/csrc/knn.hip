@@ -550,6 +550,11 @@ __global__ __launch_bounds__(kSegThreads) void k_segment_sort(const int32_t* __r
 // bounding box.  Rounding is monotone, so then every difference fl(q - p) lies in [-half, half)
 // and the canonical wrap leaves it unchanged: the distances are bit-identical.
 //
+// Query leaves come from a per-XCD item queue (xcd_queue) when the caller's tmp holds its eight
+// counters (jzt_knn_leaf2leaf_tmp_bytes includes them), else in the fixed xcd_items order: the
+// leaves' costs vary widely, and with a fixed stride the last waves of the launch ran long after
+// the others had finished (plain kNN at 2^26: 92.4 -> 87.8 ms with the queue).
+//
 // Optional start bound (init_r2, per query leaf, K <= 64 single pass): the slots start at
 // (r2_0, id = 2^32-1), so only candidates with r2 <= r2_0 enter.  A leaf with r2_0 < 0 is skipped
 // entirely (its rows and retry flags are not written), so a later pass can redo selected leaves.  A query that ends with fewer
@@ -565,12 +570,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void k_
                                                  const float* __restrict__ init_r2, int32_t* __restrict__ retry,
                                                  float* __restrict__ rad_out, int32_t* __restrict__ id_out,
                                                  uint64_t* __restrict__ resume, int koff, int ktot, float boxsize,
-                                                 int64_t il_size, int64_t out_rows, int64_t nitems, int xcd) {
+                                                 int64_t il_size, int64_t out_rows, int64_t nitems, int xcd,
+                                                 uint32_t* __restrict__ queue) {
   __shared__ SrcRec s_rec[64];
   __shared__ uint64_t s_buf[kBufWords];
   const int lane = threadIdx.x;
-  const ItemRange items = xcd_items(active_items(splQ, nitems), xcd != 0);
-  for (int64_t lq = items.first; lq < items.end; lq += items.step) {
+  const int64_t nact = active_items(splQ, nitems);
+  const ItemRange items = xcd_items(nact, xcd != 0);
+  const ItemQueue iq_ = xcd_queue(nact, xcd != 0, queue);
+  const int64_t lq_end = queue ? iq_.end : items.end;
+  for (int64_t lq = queue ? queue_next(iq_) : items.first; lq < lq_end;
+       lq = queue ? queue_next(iq_) : lq + items.step) {
     uint64_t* const row = s_buf + lane * kBufStride;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int q0 = splQ[lq], q1 = splQ[lq + 1];
@@ -800,10 +810,14 @@ template <int D>
 int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, const float* il_r2, const int32_t* splT,
                    const void* xT, const int32_t* splQ, const void* xQ, const int32_t* src_ids,
                    const int32_t* dst_rows, const float* init_r2, int32_t* retry, void* rad, int32_t* ids, void* tmp,
-                   int64_t nleaves_q, int64_t npart_q, int64_t out_rows, int64_t il_size, int k, float boxsize) {
+                   size_t tmp_bytes, int64_t nleaves_q, int64_t npart_q, int64_t out_rows, int64_t il_size, int k,
+                   float boxsize) {
   if (nleaves_q <= 0) return 0;
   if (init_r2 && k > 64) return fail("KnnLeaf2Leaf: a start bound needs k <= 64");
   uint64_t* resume = k > 64 ? (uint64_t*)tmp : nullptr;
+  // item queue counters after the resume keys, when the caller's tmp holds them
+  const size_t qoff = align256(knn_leaf_nd_tmp_bytes(npart_q, k, false));
+  uint32_t* queue = tmp && tmp_bytes >= qoff + 256 ? (uint32_t*)((char*)tmp + qoff) : nullptr;
   dim3 g(grid_for8(nleaves_q)), b(64);
   const float* xt = (const float*)xT;
   const float* xq = (const float*)xQ;
@@ -812,9 +826,11 @@ int leaf2leaf_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, 
   for (int koff = 0; koff < k; koff += 64) {
     const int kn = min(k - koff, 64);
     const bool floor = resume && koff > 0;
+    if (queue && fill_u32(s, queue, 0u, 8)) return 1;
 #define JZT_KNN_LAUNCH(KK, PER, FLOOR, W)                                                                              \
   hipLaunchKernelGGL((k_knn_leaf<KK, PER, FLOOR, D, W>), g, b, 0, s, il_spl, il_src, il_r2, splT, xt, splQ, xq, src_ids, \
-                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q, xcd)
+                     dst_rows, init_r2, retry, (float*)rad, ids, resume, koff, k, boxsize, il_size, out_rows, nleaves_q, xcd, \
+                     queue)
   // waves per EU (the register budget of the kernel): tunable for K <= 32; the multi-pass tail
   // (FLOOR) and K = 64 keep their fixed choice
 #define JZT_KNN_LAUNCH_W(KK, PER)                                                  \
@@ -914,7 +930,8 @@ extern "C" int jzt_plane_cap(jzt_stream_t stream, const int32_t* n2l, const int3
 extern "C" size_t jzt_knn_leaf2leaf_tmp_bytes(int64_t size_part_query, int k, int dtype) {
   // float: the packed kernel's u64 resume key, or the (r2, id) pair of the any-dim kernel, both 8
   // bytes per query past 64 neighbours; double: a (double r2, u32 id) pair past 32
-  return knn_leaf_nd_tmp_bytes(size_part_query, k, dtype == JZT_F64);
+  // ... then 256 bytes of item-queue counters for the packed kernel
+  return align256(knn_leaf_nd_tmp_bytes(size_part_query, k, dtype == JZT_F64)) + 256;
 }
 
 extern "C" int jzt_knn_leaf2leaf(jzt_stream_t stream, const int32_t* ilist_spl, const int32_t* ilist_isrc,
@@ -941,14 +958,14 @@ extern "C" int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_sp
   if (dtype == JZT_F64 || dim > 3)
     return (dtype == JZT_F64 ? knn_leaf_nd_f64 : knn_leaf_nd_f32)(
         s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry, knn_rad, knn_id, tmp,
-        size_leaves_query, size_part_query, out_rows, ilist_size, k, boxsize, dim);
+        tmp_bytes, size_leaves_query, size_part_query, out_rows, ilist_size, k, boxsize, dim);
   if (dim == 3)
     return leaf2leaf_impl<3>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry,
-                             knn_rad, knn_id, tmp, size_leaves_query, size_part_query, out_rows, ilist_size, k,
+                             knn_rad, knn_id, tmp, tmp_bytes, size_leaves_query, size_part_query, out_rows, ilist_size, k,
                              boxsize);
   if (dim == 2)
     return leaf2leaf_impl<2>(s, ilist_spl, ilist_isrc, ilist_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry,
-                             knn_rad, knn_id, tmp, size_leaves_query, size_part_query, out_rows, ilist_size, k,
+                             knn_rad, knn_id, tmp, tmp_bytes, size_leaves_query, size_part_query, out_rows, ilist_size, k,
                              boxsize);
   return fail("KnnLeaf2Leaf: unsupported dim");
 }

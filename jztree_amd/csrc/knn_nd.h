@@ -12,6 +12,9 @@
 // all lanes from LDS.  Exactness of the filter: by monotone rounding, every canonical r2 from the
 // box is >= the gap^2 computed with the same FMA chain.
 //
+// Query leaves are taken from a per-XCD item queue (xcd_queue) when the caller's tmp holds its
+// counters, else in the fixed xcd_items order.
+//
 // k above the per-pass slot count KMAX (64 for float, 32 for double) runs ceil(k / KMAX) passes;
 // pass p > 0 admits only pairs above the last pair of pass p-1 (kept per query in tmp: r2 array,
 // then u32 id array).  Optional start bound init_r2 per query leaf (single pass): slots start at
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(64) void k_knn_leaf_nd(const int32_t* __restrict__ 
                                                     T* __restrict__ rad_out, int32_t* __restrict__ id_out,
                                                     T* __restrict__ res_r2, uint32_t* __restrict__ res_id, int koff,
                                                     int ktot, float boxsize, int64_t il_size, int64_t out_rows,
-                                                    int64_t nitems, int xcd) {
+                                                    int64_t nitems, int xcd, uint32_t* __restrict__ queue) {
   __shared__ Vec<D, T> s_p[64];
   __shared__ uint32_t s_id[64];
   const int lane = threadIdx.x;
@@ -84,8 +87,12 @@ __global__ __launch_bounds__(64) void k_knn_leaf_nd(const int32_t* __restrict__ 
   const T box = (T)boxsize, half = (T)(0.5f * boxsize);
   const T slack = (T)(1.0f + kBoundSlack);
   const bool floor = res_r2 != nullptr && koff > 0;
-  const ItemRange items = xcd_items(active_items(splQ, nitems), xcd != 0);
-  for (int64_t lq = items.first; lq < items.end; lq += items.step) {
+  const int64_t nact = active_items(splQ, nitems);
+  const ItemRange items = xcd_items(nact, xcd != 0);
+  const ItemQueue iq_ = xcd_queue(nact, xcd != 0, queue);
+  const int64_t lq_end = queue ? iq_.end : items.end;
+  for (int64_t lq = queue ? queue_next(iq_) : items.first; lq < lq_end;
+       lq = queue ? queue_next(iq_) : lq + items.step) {
     const int q0 = splQ[lq], q1 = splQ[lq + 1];
     const int e0 = il_spl[lq], e1 = (int)min((int64_t)il_spl[lq + 1], il_size);
     // a negative (or NaN) start bound marks a leaf that this launch leaves untouched
@@ -208,26 +215,36 @@ __global__ __launch_bounds__(64) void k_knn_leaf_nd(const int32_t* __restrict__ 
   }
 }
 
+// per-query tmp of the multi-pass search: (r2, id) of the last pair of the previous pass
+inline size_t knn_leaf_nd_tmp_bytes(int64_t npart_q, int k, bool f64) {
+  const int kmax = f64 ? 32 : 64;
+  return k > kmax ? (size_t)npart_q * (f64 ? 12 : 8) : 0;
+}
+
 // host side: passes of up to KMAX slots over the whole query set (see the file comment)
 template <int D, typename T>
 int knn_leaf_nd_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src, const float* il_r2,
                      const int32_t* splT, const void* xT, const int32_t* splQ, const void* xQ,
                      const int32_t* src_ids, const int32_t* dst_rows, const float* init_r2, int32_t* retry,
-                     void* rad, int32_t* ids, void* tmp, int64_t nleaves_q, int64_t npart_q, int64_t out_rows,
-                     int64_t il_size, int k, float boxsize) {
+                     void* rad, int32_t* ids, void* tmp, size_t tmp_bytes, int64_t nleaves_q, int64_t npart_q,
+                     int64_t out_rows, int64_t il_size, int k, float boxsize) {
   constexpr int KMAX = sizeof(T) == 4 ? 64 : 32;
   if (nleaves_q <= 0) return 0;
   if (init_r2 && k > KMAX) return fail("KnnLeaf2Leaf: a start bound needs k <= " + std::to_string(KMAX));
   T* res_r2 = k > KMAX ? (T*)tmp : nullptr;
   uint32_t* res_id = k > KMAX ? (uint32_t*)((char*)tmp + (size_t)npart_q * sizeof(T)) : nullptr;
+  // item-queue counters after the resume pairs, when the caller's tmp holds them
+  const size_t qoff = align256(knn_leaf_nd_tmp_bytes(npart_q, k, sizeof(T) == 8));
+  uint32_t* queue = tmp && tmp_bytes >= qoff + 256 ? (uint32_t*)((char*)tmp + qoff) : nullptr;
   const dim3 g(grid_for8(nleaves_q)), b(64);
   const int xcd = tune_get(kTuneXcd);
   for (int koff = 0; koff < k; koff += KMAX) {
     const int kn = min(k - koff, KMAX);
+    if (queue && fill_u32(s, queue, 0u, 8)) return 1;
 #define JZT_ND_LAUNCH(KK, PER)                                                                                     \
   hipLaunchKernelGGL((k_knn_leaf_nd<KK, PER, D, T>), g, b, 0, s, il_spl, il_src, il_r2, splT, (const T*)xT, splQ, \
                      (const T*)xQ, src_ids, dst_rows, init_r2, retry, (T*)rad, ids, res_r2, res_id, koff, k,       \
-                     boxsize, il_size, out_rows, nleaves_q, xcd)
+                     boxsize, il_size, out_rows, nleaves_q, xcd, queue)
 #define JZT_ND_LAUNCH_K(KK)                   \
   do {                                        \
     if (boxsize > 0.f) JZT_ND_LAUNCH(KK, true); \
@@ -248,15 +265,10 @@ int knn_leaf_nd_impl(hipStream_t s, const int32_t* il_spl, const int32_t* il_src
 #define JZT_KNN_ND_ARGS                                                                                          \
   hipStream_t s, const int32_t *il_spl, const int32_t *il_src, const float *il_r2, const int32_t *splT,          \
       const void *xT, const int32_t *splQ, const void *xQ, const int32_t *src_ids, const int32_t *dst_rows,      \
-      const float *init_r2, int32_t *retry, void *rad, int32_t *ids, void *tmp, int64_t nleaves_q,              \
-      int64_t npart_q, int64_t out_rows, int64_t il_size, int k, float boxsize, int dim
+      const float *init_r2, int32_t *retry, void *rad, int32_t *ids, void *tmp, size_t tmp_bytes,               \
+      int64_t nleaves_q, int64_t npart_q, int64_t out_rows, int64_t il_size, int k, float boxsize, int dim
 int knn_leaf_nd_f32(JZT_KNN_ND_ARGS);
 int knn_leaf_nd_f64(JZT_KNN_ND_ARGS);
 
-// per-query tmp of the multi-pass search: (r2, id) of the last pair of the previous pass
-inline size_t knn_leaf_nd_tmp_bytes(int64_t npart_q, int k, bool f64) {
-  const int kmax = f64 ? 32 : 64;
-  return k > kmax ? (size_t)npart_q * (f64 ? 12 : 8) : 0;
-}
 
 }  // namespace jzt

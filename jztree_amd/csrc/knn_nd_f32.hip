@@ -7,7 +7,7 @@ int knn_leaf_nd_f32(JZT_KNN_ND_ARGS) {
 #define JZT_ND_DIM(D)                                                                                           \
   case D:                                                                                                       \
     return knn_leaf_nd_impl<D, float>(s, il_spl, il_src, il_r2, splT, xT, splQ, xQ, src_ids, dst_rows, init_r2, retry, \
-                                 rad, ids, tmp, nleaves_q, npart_q, out_rows, il_size, k, boxsize);
+                                 rad, ids, tmp, tmp_bytes, nleaves_q, npart_q, out_rows, il_size, k, boxsize);
   switch (dim) {
     JZT_ND_DIM(4)
     JZT_ND_DIM(5)

@@ -97,6 +97,27 @@ __device__ __forceinline__ ItemRange xcd_items(int64_t n, bool on) {
   const int64_t x = blockIdx.x & 7, per = (n + 7) >> 3;
   return {x * per + (int64_t)(blockIdx.x >> 3), n < (x + 1) * per ? n : (x + 1) * per, g >> 3};
 }
+// Dynamic form of xcd_items for one-wave blocks: the blocks of an XCD take the items of its eighth
+// one at a time from a shared counter (counters[x], zeroed before the launch), so the XCD's blocks
+// finish together however uneven the items are; the item set and the XCD-contiguous order are
+// those of xcd_items.  It pays where items are expensive and uneven (kNN query leaves: -5% kernel
+// time at 2^26); for the cheap, even FoF leaves every fetch order tried was slower than the static
+// one.  queue_next is called by all 64 lanes (wave-uniform result).
+struct ItemQueue {
+  int64_t begin, end;
+  uint32_t* ctr;
+};
+__device__ __forceinline__ ItemQueue xcd_queue(int64_t n, bool on, uint32_t* counters) {
+  const int64_t g = gridDim.x;
+  if (!on || (g & 7) != 0) return {0, n, counters};
+  const int64_t x = blockIdx.x & 7, per = (n + 7) >> 3;
+  return {x * per, n < (x + 1) * per ? n : (x + 1) * per, counters + x};
+}
+__device__ __forceinline__ int64_t queue_next(const ItemQueue& q) {
+  uint32_t v = 0;
+  if (threadIdx.x == 0) v = atomicAdd(q.ctr, 1u);
+  return q.begin + (int64_t)__builtin_amdgcn_readfirstlane(v);
+}
 // grid for xcd_items: at most kMaxGrid blocks, a multiple of 8
 inline unsigned grid_for8(int64_t n) {
   int64_t g = n <= 0 ? 8 : (n < kMaxGrid ? n : kMaxGrid);

@@ -29,8 +29,11 @@ def test_abi_version_and_error_string():
 def test_tmp_size_queries_are_host_only():
     import jztree_amd._lib as L
     assert L.LIB.jzt_zsort_tmp_bytes(1 << 20, 3, 0) >= 24 * (1 << 20)
-    assert L.LIB.jzt_knn_leaf2leaf_tmp_bytes(1000, 16, 0) == 0
-    assert L.LIB.jzt_knn_leaf2leaf_tmp_bytes(1000, 100, 0) == 8000
+    # 256 bytes of work-queue counters after the multi-pass state (aligned to 256 bytes): none for
+    # one pass, 8 bytes per query past k = 64 for float32, 12 past k = 32 for float64
+    assert L.LIB.jzt_knn_leaf2leaf_tmp_bytes(1000, 16, 0) == 256
+    assert L.LIB.jzt_knn_leaf2leaf_tmp_bytes(1000, 100, 0) == 8192 + 256
+    assert L.LIB.jzt_knn_leaf2leaf_tmp_bytes(1000, 40, 1) == 12032 + 256
 
 
 def _param_kinds():
