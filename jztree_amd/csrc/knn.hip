@@ -953,7 +953,9 @@ extern "C" int jzt_knn_leaf2leaf_ex(jzt_stream_t stream, const int32_t* ilist_sp
                                     int dtype) {
   hipStream_t s = (hipStream_t)stream;
   if (k < 1) return fail("KnnLeaf2Leaf: k must be >= 1");
-  if (tmp_bytes < jzt_knn_leaf2leaf_tmp_bytes(size_part_query, k, dtype)) return fail("KnnLeaf2Leaf: tmp too small");
+  // the multi-pass state is required; the work-queue counters after it are optional
+  if (tmp_bytes < knn_leaf_nd_tmp_bytes(size_part_query, k, dtype == JZT_F64))
+    return fail("KnnLeaf2Leaf: tmp too small");
   if (dtype != JZT_F32 && dtype != JZT_F64) return fail("KnnLeaf2Leaf: positions must be float32 or float64");
   if (dtype == JZT_F64 || dim > 3)
     return (dtype == JZT_F64 ? knn_leaf_nd_f64 : knn_leaf_nd_f32)(

@@ -41,7 +41,7 @@ class PosLvlNum:
 
 
 def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, boxsize=0., src_ids=None,
-                   dst_rows=None, out_rows: int | None = None, init_r2=None, out=None):
+                   dst_rows=None, out_rows: int | None = None, init_r2=None, out=None, balance: bool = True):
     """knn.py:28-60 -> KnnLeaf2Leaf.  Returns (rnn (rows,k), inn (rows,k) int32), plus the retry
     flags (nq int32) when init_r2 is given.
 
@@ -49,7 +49,10 @@ def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, b
     output row of each query (default its z-index; rows = out_rows, default nq); init_r2 = starting
     bound per query leaf (k <= 64; the lists must hold every leaf within it), queries left with
     fewer than k neighbours inside it are flagged and their rows must be recomputed; leaves with a
-    negative bound are skipped.  out = (rnn, inn[, retry]) writes into existing buffers."""
+    negative bound are skipped.  out = (rnn, inn[, retry]) writes into existing buffers.
+    balance: hand the query leaves to the waves through the kernel's work queue (one atomic per
+    leaf); False keeps the fixed order, for passes in which most leaves are skipped (negative
+    bounds), where the per-leaf atomics would cost more than the pass itself."""
     if xQ is None:
         xQ = xT
     if splQ is None:
@@ -71,6 +74,8 @@ def _knn_leaf2leaf(ilist: InteractionList, splT, xT, splQ=None, xQ=None, k=32, b
         inn = torch.empty((rows, k), dtype=torch.int32, device=dev)
         retry = None
     nb = L.LIB.jzt_knn_leaf2leaf_tmp_bytes(nq, k, dt)
+    if not balance:  # only the multi-pass state: the kernel then runs the leaves in a fixed order
+        nb = L.LIB.jzt_knn_leaf2leaf_tmp_bytes(nq, k, dt) - 256
     tmp = _tmp(nb, dev)
     sid = None if src_ids is None else src_ids.to(torch.int32).contiguous()
     drow = None if dst_rows is None else dst_rows.to(torch.int32).contiguous()
@@ -307,14 +312,14 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
 
     plane1 = []
 
-    def run(r2, upper=None):
+    def run(r2, upper=None, balance=True):
         ovf = []
         il, p1 = _knn_radius_walk(th, k, _plane_radii(th, r2, upper), boxsize=boxsize,
                                   alloc_fac_ilist=alloc_fac_ilist, overflow=ovf, keep_plane1=True)
         if upper is not None:
             plane1.append(p1)
         _knn_leaf2leaf(il, spl, posz, k=k, boxsize=boxsize, src_ids=src_ids, dst_rows=dst_rows, out_rows=rows,
-                       init_r2=r2, out=(rnn, inn, retry))
+                       init_r2=r2, out=(rnn, inn, retry), balance=balance)
         return torch.stack(ovf).any()
 
     # with direct_retry the planes above the leaves search the guaranteed radii: their plane-1 lists
@@ -356,7 +361,7 @@ def _knn_bounded(th: TreeHierarchy, posz, k: int, leaf_r2: torch.Tensor, boxsize
     for rnd, r2_next in enumerate(later, start=2):
         cs = torch.cat([torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(retry, 0)])
         short = (cs[sl[1:]] - cs[sl[:-1]]) > 0
-        ovf = run(torch.where(short, r2_next(), torch.full_like(r2, -1.0)).contiguous())
+        ovf = run(torch.where(short, r2_next(), torch.full_like(r2, -1.0)).contiguous(), balance=False)
         h = torch.stack([ovf.to(torch.int64), retry.sum().to(torch.int64), short.sum().to(torch.int64)]).cpu()
         st[f"short_leaves_round{rnd}"] = int(h[2])
         st[f"short_round{rnd}"] = int(h[1])
@@ -415,7 +420,7 @@ def _knn_leaf_start_bound(ilist: InteractionList, spl, posz, k: int, boxsize, le
                L.ptr(short), size, 0.0)
         redo = torch.where(short > 0, torch.full_like(init, float("inf")), torch.full_like(init, -1.0))
         _knn_leaf2leaf(ilist, spl, posz, k=k, boxsize=boxsize, splQ=splQ, xQ=xQ, src_ids=src_ids,
-                       dst_rows=dst_rows, init_r2=redo.contiguous(), out=(rnn, inn, retry))
+                       dst_rows=dst_rows, init_r2=redo.contiguous(), out=(rnn, inn, retry), balance=False)
     return rnn, inn
 
 
