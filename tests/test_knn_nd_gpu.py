@@ -143,3 +143,19 @@ def test_knn_part_query_high_dims(dev, oracle, dim, dtype):
     print(f"[knn-nd] part_query dim={dim} {np.dtype(dtype).name}", flush=True)
     rnn, inn = jz.knn.knn(torch.from_numpy(x).to(dev), 10, part_query=torch.from_numpy(q).to(dev), cfg=_cfg(dim))
     _check(x, 10, rnn.cpu().numpy(), inn.cpu().numpy(), oracle, xq=q)
+
+
+def test_knn_double_like_reference(dev, oracle):
+    """checks/tests/test_knn.py:76-80: 2^18 float64 points in [0, 10), k = 13."""
+    x = np.random.default_rng(76).uniform(0.0, 10.0, (1 << 18, 3))
+    rnn, inn = _knn(x, 13, dev)
+    _check(x, 13, rnn, inn, oracle)
+
+
+@pytest.mark.parametrize("boxsize", [0.03, 170.0])
+def test_knn_f64_periodic_box_sizes(dev, oracle, boxsize):
+    """checks/tests/test_knn.py:83-88 on float64 positions."""
+    x = _f64(1 << 14, 3, seed=88) * boxsize
+    x[x >= boxsize] = 0.0
+    rnn, inn = _knn(x, 16, dev, boxsize=boxsize)
+    _check(x, 16, rnn, inn, oracle, boxsize=boxsize)
