@@ -154,7 +154,7 @@ __device__ __forceinline__ void topk_flush(uint64_t (&best)[K], const uint64_t* 
 // or not, and the fill advances by the accept flag: no divergent branch per candidate.  A group
 // starts with at most CAP - U entries, so every such write lands inside the row.
 #ifndef JZT_CONSUME_UNROLL
-#define JZT_CONSUME_UNROLL 2
+#define JZT_CONSUME_UNROLL 4
 #endif
 template <int K, bool PER, bool FLOOR, int D>
 __device__ __forceinline__ bool consume_chunk(uint64_t (&best)[K], const Vec<D, float>& q, const SrcRec* s_rec,
