@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM traffic of the leaf kernel per launch, from two rocprofv3 --pmc passes over this bench
 # (FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a plain copy kernel of known size reads);
 # keyed by (kernel, n, k, halos, leaf size, XCD order knob); None where not measured.
-LEAF_TRAFFIC_BYTES = {("k_knn_leaf", 1 << 26, 32, 65536, 128, 1): 37_547_212_800}
+LEAF_TRAFFIC_BYTES = {("k_knn_leaf", 1 << 26, 32, 65536, 128, 1): 37_532_876_800}
 # algorithmic bytes per point of the FoF stages (f32 positions, u32 labels): end to end
 # (key generation, 8 radix passes, permutation, leaf build, particle links) and the link stage alone
 FOF_BYTES_END_TO_END = 288
